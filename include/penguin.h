@@ -1,0 +1,213 @@
+/*
+ * penguin.h -- C ABI of the MI355X-native PEnGUIn hydro step.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (PN_OK == 0) and never exits the process.  Detail for a non-zero
+ * status is available from pn_last_error(ctx).  Each call completes (its
+ * device work has drained) before it returns, as the reference operators do.
+ * A context is not thread-safe; separate contexts on separate threads are.
+ *
+ * Reference interface replaced (paths relative to the reference checkout):
+ *   types   hydr_ring / body / SymDisk      include/variable_types.h:26-97
+ *   params  compile-time flags + constants  include/global.h:16-96, variable_types.h:11-20
+ *   ops     get_dt                          include/timestep.h:4        (src/timestep/timestep.cu:112-149)
+ *           Index_Shift                     include/orbital_advection.h:4 (src/orbital_advection/orbital_advection.cu:12-69)
+ *           bundle_sweep2D                  include/ppm.h:4             (src/ppm/ppm.cu:358-386)
+ *           GPU_output_reduction            include/output.h:32         (src/output/device_func.cu:155-304)
+ *           save_cells / copy_bfile         include/output.h:26         (src/output/output.cu:48-55, main.cu:37-42)
+ *           init_planet / planet_forces /
+ *           kick_drift_kick                 include/planet.h:7-9        (src/planet/planet.cu:36-93)
+ *   driver  the step loop of kernel()       main.cu:343-441
+ */
+#ifndef PENGUIN_H
+#define PENGUIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PN_NPAD 6          /* ghost rings / halo cells per side, variable_types.h:13 */
+
+/* status codes */
+enum {
+  PN_OK = 0,
+  PN_ERR_ARG = 1,          /* bad argument: null pointer, size mismatch, out-of-range flag */
+  PN_ERR_HIP = 2,          /* a HIP runtime call failed (message carries hipGetErrorString) */
+  PN_ERR_LAYOUT = 3,       /* hydr_ring bytes inconsistent (ring index, phi geometry) */
+  PN_ERR_STATE = 4,        /* call out of order (e.g. sweep before load) */
+  PN_ERR_UNSUPPORTED = 5,  /* flag combination this build does not implement */
+  PN_ERR_NUMERIC = 6,      /* non-finite dt or state detected */
+  PN_ERR_IO = 7            /* snapshot file missing, short or unwritable */
+};
+
+/* arithmetic precision of a context */
+enum {
+  PN_FP64 = 0,             /* plain FP64: the drop-in path */
+  PN_FP32 = 1              /* FP32 bulk with the FP64 steps kept where stability needs them */
+};
+
+/* body, variable_types.h:26-37 (same field order) */
+typedef struct pn_body {
+  double m, x, y, vx, vy, fx, fy, rs;
+} pn_body;
+
+/* SymDisk, variable_types.h:58-65: the analytic background per ring (FARGO speed) */
+typedef struct pn_symdisk {
+  double r, p, u, v, w;
+} pn_symdisk;
+
+/* grid sizes, variable_types.h:18-20; kmax == 1 means a 2-D r-phi disk */
+typedef struct pn_grid {
+  int32_t imax, jmax, kmax;
+} pn_grid;
+
+/* The compile-time flags and constants of global.h as one runtime record.
+ * pn_params_default() fills the values global.h ships with. */
+typedef struct pn_params {
+  /* flags, global.h:16-26 */
+  int32_t ndim;            /* 2 or 3 (variable_types.h:11) */
+  int32_t eos;             /* 0 isothermal, 1 isentropic, 2 adiabatic (global.h:18) */
+  int32_t flat_flag;       /* 1: PPM flattening at shocks (global.h:16) */
+  int32_t plnt_flag;       /* 0 none, 1 fixed, 2 moving planet (global.h:20) */
+  int32_t frrot_flag;      /* 1: frame co-rotates with the planet (global.h:21) */
+  int32_t visc_flag;       /* 1: explicit viscosity (global.h:23) */
+  int32_t fargo_flag;      /* 0 off, 1 background orbital advection, 2 frame (global.h:24) */
+  int32_t ngeomx, ngeomy, ngeomz;  /* geometry codes (global.h:83-85) */
+  int32_t grspx, grspy, grspz;     /* grid spacing codes (global.h:87-89) */
+  int32_t init_p_iso;      /* 1: initial P = c_s^2 Sigma for every EOS (else disk_profile.cu:54-61) */
+  /* disk, global.h:32-37 */
+  double p_alpha, p_beta, ss_alpha, sc_h, vis_nu;
+  /* planet, global.h:43-46 */
+  double M_p, R_p, t_growth, rs_fac;
+  /* hydro, global.h:52-57 */
+  double gam, courant;
+  double endtime, tmovie;
+  /* grid bounds, global.h:76-81 */
+  double xmin, xmax, ymin, ymax, zmin, zmax;
+} pn_params;
+
+/* State a step loop carries between steps (main.cu:270-275, 325-327). */
+typedef struct pn_run_state {
+  double dt;               /* previous dt (endtime before the first step, main.cu:270) */
+  double simtime;
+  double FrRot;            /* frame rotation rate */
+  pn_body planet;
+  int64_t ncycle;
+} pn_run_state;
+
+typedef struct pn_ctx pn_ctx;
+
+/* ---- parameters and layout ------------------------------------------- */
+
+/* Fill p with the values of global.h / variable_types.h. */
+void pn_params_default(pn_params* p);
+
+/* sizeof(hydr_ring) for a given jmax: 72*jmax + 96 (variable_types.h:67-97). */
+size_t pn_ring_bytes(int32_t jmax);
+
+/* ---- host-side synthetic disk (init.cpp:3-543, disk_profile.cu:9-171) --
+ * Writes imax*kmax interior rings (ring order i + imax*k, init.cpp:241),
+ * PN_NPAD*kmax fixed ghost rings on each radial side (lft, rgh), and, in 3-D,
+ * PN_NPAD*imax fixed ghost rings below/above in theta (udr, top; may be NULL
+ * in 2-D).  val receives imax*kmax background records.  Surface density gets
+ * seeded relative noise Sigma *= 1 + noise*U, U = splitmix64(seed ^ cell)
+ * mapped to [-1, 1), cell = j + jmax*(i + imax*k).  Before any step the
+ * caller primes FARGO with pn_index_shift(ctx, endtime, FrRot) as main.cu:330
+ * does.  Returns PN_OK or PN_ERR_ARG. */
+int pn_init_disk(const pn_grid* g, const pn_params* p, uint64_t seed, double noise,
+                 void* rings, void* lft, void* rgh, void* udr, void* top,
+                 pn_symdisk* val);
+
+/* ---- context ----------------------------------------------------------- */
+
+/* Allocate a context on the current HIP device.  precision: PN_FP64 or
+ * PN_FP32.  The library owns all device memory; the caller owns host
+ * buffers. */
+int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out);
+int pn_destroy(pn_ctx* ctx);
+const char* pn_last_error(const pn_ctx* ctx);
+
+/* Upload state in hydr_ring byte layout.  rings: imax*kmax records;
+ * lft/rgh: PN_NPAD*kmax records each; udr/top: PN_NPAD*imax records each
+ * (3-D only, else NULL with udr_bytes 0); val: imax*kmax records.  All rings
+ * must carry the same phi geometry (y, dy, yc, yvol) -- the reference always
+ * builds them that way (init.cpp:201-206) -- else PN_ERR_LAYOUT. */
+int pn_load_rings(pn_ctx* ctx, const void* rings, size_t rings_bytes,
+                  const void* lft, const void* rgh, size_t lr_bytes,
+                  const void* udr, const void* top, size_t ut_bytes,
+                  const pn_symdisk* val, size_t nval);
+
+/* Download interior rings in hydr_ring byte layout (the bytes save_cells
+ * writes, output.cu:48-55). */
+int pn_store_rings(pn_ctx* ctx, void* rings, size_t rings_bytes);
+
+/* ---- step operators ----------------------------------------------------- */
+
+/* get_dt (timestep.cu:112-149): *dt = min(1.1*dt_prev, courant / max over
+ * cells of the signal speed per cell width). */
+int pn_get_dt(pn_ctx* ctx, double dt_prev, double FrRot, double* dt);
+
+/* Index_Shift (orbital_advection.cu:12-43): per ring, split the previous
+ * orbital shift into whole cells (inc_j) and a residual speed (res_v), then
+ * refresh rot_v. */
+int pn_index_shift(pn_ctx* ctx, double dt, double FrRot);
+
+/* bundle_sweep2D (ppm.cu:358-386): radial sweep, azimuthal sweep, integer
+ * phi roll.  For a 3-D context it also runs the theta sweep between the
+ * azimuthal sweep and the roll. */
+int pn_sweep2d(pn_ctx* ctx, double dt, const pn_body* planet, double FrRot);
+
+/* One directional sweep (axis 0 = r, 1 = phi, 2 = theta) without the roll.  No
+ * counterpart in the reference, which only exposes the bundle; it lets callers and tests
+ * drive a single direction (ppm.cu:26-160 or 162-307). */
+int pn_sweep_axis(pn_ctx* ctx, int axis, double dt, const pn_body* planet, double FrRot);
+
+/* GPU_output_reduction (device_func.cu:155-304): mode 0 = total mass,
+ * 8 / 9 = background-subtracted radial / azimuthal force on the planet
+ * (2-D).  exclude is in units of sc_h as in the reference. */
+int pn_reduce(pn_ctx* ctx, const pn_body* planet, double exclude, int mode, double* out);
+
+/* ---- planet (host, planet.cu:28-93) ------------------------------------ */
+void pn_init_planet(const pn_params* p, pn_body* planet, double* FrRot);
+/* mass ramp + forces; for plnt_flag == 2 it runs two pn_reduce calls. */
+int pn_planet_forces(pn_ctx* ctx, pn_body* planet, double FrRot, double simtime);
+void pn_kick_drift_kick(const pn_params* p, pn_body* planet, double* FrRot, double dt, int kick);
+
+/* ---- the step loop ------------------------------------------------------- */
+
+/* Prime a run: st->dt = endtime, planet initialised, FARGO primed
+ * (main.cu:270, 325-330). */
+int pn_run_begin(pn_ctx* ctx, pn_run_state* st);
+
+/* Advance nsteps steps of main.cu:343-387 (no endtime / tmovie landing, no
+ * output), keeping dt, the planet and FrRot on the device between steps.
+ * The step is launched as a captured HIP graph when graph != 0. */
+int pn_run(pn_ctx* ctx, int nsteps, int graph, pn_run_state* st);
+
+/* Time the kernels of one step with HIP events on the context's stream: each of
+ * {CFL, Index_Shift, r sweep, phi sweep, theta sweep, roll} is launched nrep times back to
+ * back on the current state and its mean duration in milliseconds written to ms[0..5]
+ * (0 where a kernel does not run).  No counterpart in the reference (which only prints
+ * seconds per step, main.cu:430-437).  The state advances while it is timed. */
+int pn_time_kernels(pn_ctx* ctx, int nrep, double* ms6);
+
+/* Mass and angular-momentum totals in FP64 with compensated summation. */
+int pn_totals(pn_ctx* ctx, double* mass, double* angmom);
+
+/* ---- snapshots (output.cu:48-120, global.cu:27-38, main.cu:246-262) ----- */
+
+/* Write the imax*kmax rings of ctx as one raw binary_<label>_<NNNNN> file
+ * (no header). */
+int pn_save_snapshot(pn_ctx* ctx, const char* path);
+/* Read a raw snapshot back into ctx (restart, main.cu:246-262); ghost rings
+ * and val must already be loaded.  PN_ERR_IO on a missing or short file. */
+int pn_load_snapshot(pn_ctx* ctx, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PENGUIN_H */

@@ -1,0 +1,264 @@
+"""ctypes mirror of include/penguin.h and a thin Python wrapper over the C ABI.
+
+This module only marshals arguments; every computation happens in libpenguin.so.  Loading
+fails loudly when the library has not been built -- there is no Python or CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libpenguin.so"
+NPAD = 6
+
+PN_OK, PN_ERR_ARG, PN_ERR_HIP, PN_ERR_LAYOUT, PN_ERR_STATE, PN_ERR_UNSUPPORTED, PN_ERR_NUMERIC, PN_ERR_IO = range(8)
+PN_FP64, PN_FP32 = 0, 1
+ERR_NAMES = {0: "PN_OK", 1: "PN_ERR_ARG", 2: "PN_ERR_HIP", 3: "PN_ERR_LAYOUT", 4: "PN_ERR_STATE",
+             5: "PN_ERR_UNSUPPORTED", 6: "PN_ERR_NUMERIC", 7: "PN_ERR_IO"}
+
+
+class PnBody(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("m", "x", "y", "vx", "vy", "fx", "fy", "rs")]
+
+
+class PnSymdisk(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("r", "p", "u", "v", "w")]
+
+
+class PnGrid(C.Structure):
+    _fields_ = [("imax", C.c_int32), ("jmax", C.c_int32), ("kmax", C.c_int32)]
+
+
+class PnParams(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "ndim", "eos", "flat_flag", "plnt_flag", "frrot_flag", "visc_flag", "fargo_flag",
+        "ngeomx", "ngeomy", "ngeomz", "grspx", "grspy", "grspz", "init_p_iso")] + \
+        [(n, C.c_double) for n in (
+            "p_alpha", "p_beta", "ss_alpha", "sc_h", "vis_nu", "M_p", "R_p", "t_growth", "rs_fac",
+            "gam", "courant", "endtime", "tmovie", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax")]
+
+    def replace(self, **kw) -> "PnParams":
+        out = PnParams()
+        C.memmove(C.byref(out), C.byref(self), C.sizeof(PnParams))
+        for k, v in kw.items():
+            if not hasattr(out, k):
+                raise AttributeError(k)
+            setattr(out, k, v)
+        return out
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class PnRunState(C.Structure):
+    _fields_ = [("dt", C.c_double), ("simtime", C.c_double), ("FrRot", C.c_double),
+                ("planet", PnBody), ("ncycle", C.c_int64)]
+
+
+class PnError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def ring_dtype(jmax: int) -> np.dtype:
+    """numpy view of hydr_ring (variable_types.h:67-97) for a runtime jmax."""
+    f8, i4 = np.float64, np.int32
+    dt = np.dtype([("r", f8, (jmax,)), ("p", f8, (jmax,)), ("u", f8, (jmax,)), ("v", f8, (jmax,)),
+                   ("w", f8, (jmax,)), ("x", f8), ("dx", f8), ("xvol", f8), ("xc", f8),
+                   ("y", f8, (jmax,)), ("dy", f8, (jmax,)), ("yvol", f8, (jmax,)), ("yc", f8, (jmax,)),
+                   ("z", f8), ("dz", f8), ("zvol", f8), ("zc", f8), ("i", i4), ("k", i4),
+                   ("rot_v", f8), ("res_v", f8), ("rot_j", i4), ("inc_j", i4)])
+    assert dt.itemsize == 72 * jmax + 96
+    return dt
+
+
+_lib = None
+
+
+def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load libpenguin.so; raises if it is missing (build with penguin_2_0_amd.build)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is not built; run `python -m penguin_2_0_amd.build` first")
+    lib = C.CDLL(str(p))
+    vp, sz, d, i = C.c_void_p, C.c_size_t, C.c_double, C.c_int
+    P = C.POINTER
+    sig = {
+        "pn_params_default": (None, [P(PnParams)]),
+        "pn_ring_bytes": (sz, [C.c_int32]),
+        "pn_init_disk": (i, [P(PnGrid), P(PnParams), C.c_uint64, d, vp, vp, vp, vp, vp, P(PnSymdisk)]),
+        "pn_create": (i, [P(PnGrid), P(PnParams), i, P(vp)]),
+        "pn_destroy": (i, [vp]),
+        "pn_last_error": (C.c_char_p, [vp]),
+        "pn_load_rings": (i, [vp, vp, sz, vp, vp, sz, vp, vp, sz, P(PnSymdisk), sz]),
+        "pn_store_rings": (i, [vp, vp, sz]),
+        "pn_get_dt": (i, [vp, d, d, P(d)]),
+        "pn_index_shift": (i, [vp, d, d]),
+        "pn_sweep2d": (i, [vp, d, P(PnBody), d]),
+        "pn_reduce": (i, [vp, P(PnBody), d, i, P(d)]),
+        "pn_sweep_axis": (i, [vp, i, d, P(PnBody), d]),
+        "pn_time_kernels": (i, [vp, i, P(d)]),
+        "pn_init_planet": (None, [P(PnParams), P(PnBody), P(d)]),
+        "pn_planet_forces": (i, [vp, P(PnBody), d, d]),
+        "pn_kick_drift_kick": (None, [P(PnParams), P(PnBody), P(d), d, i]),
+        "pn_run_begin": (i, [vp, P(PnRunState)]),
+        "pn_run": (i, [vp, i, i, P(PnRunState)]),
+        "pn_totals": (i, [vp, P(d), P(d)]),
+        "pn_save_snapshot": (i, [vp, C.c_char_p]),
+        "pn_load_snapshot": (i, [vp, C.c_char_p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def params_default() -> PnParams:
+    p = PnParams()
+    load_library().pn_params_default(C.byref(p))
+    return p
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Disk:
+    """Host buffers of one disk in the reference's byte layout."""
+
+    def __init__(self, grid: PnGrid, params: PnParams):
+        self.grid, self.params = grid, params
+        self.dtype = ring_dtype(grid.jmax)
+        n = grid.imax * grid.kmax
+        self.rings = np.zeros(n, self.dtype)
+        self.lft = np.zeros(NPAD * grid.kmax, self.dtype)
+        self.rgh = np.zeros(NPAD * grid.kmax, self.dtype)
+        three_d = params.ndim == 3
+        self.udr = np.zeros(NPAD * grid.imax, self.dtype) if three_d else None
+        self.top = np.zeros(NPAD * grid.imax, self.dtype) if three_d else None
+        self.val = (PnSymdisk * n)()
+
+    def copy(self) -> "Disk":
+        d = Disk.__new__(Disk)
+        d.grid, d.params, d.dtype = self.grid, self.params, self.dtype
+        d.rings, d.lft, d.rgh = self.rings.copy(), self.lft.copy(), self.rgh.copy()
+        d.udr = None if self.udr is None else self.udr.copy()
+        d.top = None if self.top is None else self.top.copy()
+        n = len(self.val)
+        d.val = (PnSymdisk * n)()
+        C.memmove(d.val, self.val, C.sizeof(PnSymdisk) * n)
+        return d
+
+    def val_array(self) -> np.ndarray:
+        return np.ctypeslib.as_array(C.cast(self.val, C.POINTER(C.c_double)), shape=(len(self.val), 5)).copy()
+
+
+def init_disk(grid: PnGrid, params: PnParams, seed: int, noise: float) -> Disk:
+    d = Disk(grid, params)
+    rc = load_library().pn_init_disk(C.byref(grid), C.byref(params), seed, noise, _ptr(d.rings), _ptr(d.lft),
+                                     _ptr(d.rgh), _ptr(d.udr), _ptr(d.top), d.val)
+    if rc != PN_OK:
+        raise PnError(rc, "pn_init_disk")
+    return d
+
+
+class Context:
+    """One device context (pn_create / pn_destroy)."""
+
+    def __init__(self, grid: PnGrid, params: PnParams, precision: int = PN_FP64):
+        self.lib = load_library()
+        self.grid, self.params, self.precision = grid, params, precision
+        h = C.c_void_p()
+        rc = self.lib.pn_create(C.byref(grid), C.byref(params), precision, C.byref(h))
+        self.h = h
+        if rc != PN_OK:
+            msg = self.lib.pn_last_error(h).decode() if h.value else ""
+            self.close()
+            raise PnError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.pn_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc: int, what: str):
+        if rc != PN_OK:
+            raise PnError(rc, f"{what}: {self.lib.pn_last_error(self.h).decode()}")
+
+    def load(self, disk: Disk):
+        ut = 0 if disk.udr is None else disk.udr.nbytes
+        self._chk(self.lib.pn_load_rings(self.h, _ptr(disk.rings), disk.rings.nbytes, _ptr(disk.lft), _ptr(disk.rgh),
+                                         disk.lft.nbytes, _ptr(disk.udr), _ptr(disk.top), ut, disk.val,
+                                         len(disk.val)), "pn_load_rings")
+
+    def store(self) -> np.ndarray:
+        out = np.zeros(self.grid.imax * self.grid.kmax, ring_dtype(self.grid.jmax))
+        self._chk(self.lib.pn_store_rings(self.h, _ptr(out), out.nbytes), "pn_store_rings")
+        return out
+
+    def get_dt(self, dt_prev: float, FrRot: float) -> float:
+        d = C.c_double()
+        self._chk(self.lib.pn_get_dt(self.h, dt_prev, FrRot, C.byref(d)), "pn_get_dt")
+        return d.value
+
+    def index_shift(self, dt: float, FrRot: float):
+        self._chk(self.lib.pn_index_shift(self.h, dt, FrRot), "pn_index_shift")
+
+    def sweep2d(self, dt: float, planet: PnBody, FrRot: float):
+        self._chk(self.lib.pn_sweep2d(self.h, dt, C.byref(planet), FrRot), "pn_sweep2d")
+
+    def sweep_axis(self, axis: int, dt: float, planet: PnBody, FrRot: float):
+        self._chk(self.lib.pn_sweep_axis(self.h, axis, dt, C.byref(planet), FrRot), "pn_sweep_axis")
+
+    def time_kernels(self, nrep: int) -> dict:
+        ms = (C.c_double * 6)()
+        self._chk(self.lib.pn_time_kernels(self.h, nrep, ms), "pn_time_kernels")
+        return dict(zip(("cfl", "index_shift", "sweep_x", "sweep_y", "sweep_z", "rotate"), list(ms)))
+
+    def reduce(self, planet: PnBody, exclude: float, mode: int) -> float:
+        d = C.c_double()
+        self._chk(self.lib.pn_reduce(self.h, C.byref(planet), exclude, mode, C.byref(d)), "pn_reduce")
+        return d.value
+
+    def totals(self) -> tuple[float, float]:
+        m, l = C.c_double(), C.c_double()
+        self._chk(self.lib.pn_totals(self.h, C.byref(m), C.byref(l)), "pn_totals")
+        return m.value, l.value
+
+    def run_begin(self) -> PnRunState:
+        st = PnRunState()
+        self._chk(self.lib.pn_run_begin(self.h, C.byref(st)), "pn_run_begin")
+        return st
+
+    def run(self, nsteps: int, st: PnRunState, graph: bool = True) -> PnRunState:
+        self._chk(self.lib.pn_run(self.h, nsteps, 1 if graph else 0, C.byref(st)), "pn_run")
+        return st
+
+    def save_snapshot(self, path: str):
+        self._chk(self.lib.pn_save_snapshot(self.h, str(path).encode()), "pn_save_snapshot")
+
+    def load_snapshot(self, path: str):
+        self._chk(self.lib.pn_load_snapshot(self.h, str(path).encode()), "pn_load_snapshot")

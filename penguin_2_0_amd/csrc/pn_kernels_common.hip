@@ -1,0 +1,322 @@
+// pn_kernels_common.hip -- the small per-step kernels: FARGO split and roll, CFL max,
+// reductions, device-side planet update.  FP64 throughout (-ffp-contract=off).
+#include "pn_launch.hpp"
+#include "ppm_wave.hpp"
+
+namespace pn {
+
+namespace {
+
+constexpr double kPi = 3.1415926535897932384626433832795;
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double kEarthMass = 0.000003;
+constexpr double kSigma0 = 1.0 * 0.00019126835;   // global.h:37 (MMSN_1AU)
+
+// Index_Shift (orbital_advection.cu:12-43): one thread per interior ring
+__global__ void k_index_shift(DevGrid g, DevParams prm, const StepScalars* __restrict__ ss) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.imax * g.kmax) return;
+  const int i = n % g.imax, k = n / g.imax;
+  const int s = slot_of(g, i, k);
+  const double dt = ss->dt, FrRot = ss->FrRot;
+  const double xc = g.xc[s];
+  const double widthy = xc * g.dy[0] / dt;
+  double rot_v;
+  if (prm.fargo == 1) rot_v = g.val[i + g.imax * k].v - FrRot * xc;
+  else if (prm.fargo == 2) rot_v = (1.0 - FrRot) * xc;
+  else rot_v = 0.0;
+  const double jp = g.rot_v[s] / widthy;
+  if (jp >= 0.0) g.inc_j[s] = (int)(jp + 0.5);
+  else g.inc_j[s] = (int)(jp - 0.5);
+  g.res_v[s] = floor(jp + 0.5) * widthy;
+  g.rot_v[s] = rot_v;
+}
+
+// rotate_rings (ppm.cu:309-322)
+__global__ void k_rotate(DevGrid g) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.imax * g.kmax) return;
+  const int s = slot_of(g, n % g.imax, n / g.imax);
+  int j = g.rot_j[s] + g.inc_j[s];
+  if (j < 0) j += g.jmax;
+  if (j >= g.jmax) j -= g.jmax;
+  g.rot_j[s] = j;
+}
+
+__global__ void k_clear_cfl(StepScalars* ss) {
+  if (threadIdx.x == 0) ss->cfl_bits = 0ull;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// CFL signal speed per cell width (timestep.cu:12-73) over every interior cell; the max is
+// order independent, so a wavefront max plus one atomic per wave is exact
+__global__ __launch_bounds__(256) void k_cfl(DevGrid g, DevParams prm, StepScalars* ss, int src) {
+  const long ncell = (long)g.imax * g.kmax * g.jmax;
+  const double FrRot = ss->FrRot;
+  double m = 0.0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(t % g.jmax);
+    const int n = (int)(t / g.jmax);
+    const int s = slot_of(g, n % g.imax, n / g.imax);
+    const long a = (long)s * g.jmax + j;
+    const double rad = g.xc[s], rad_cyl = g.radc[s];
+    double widthy = g.dy[j];
+    if (prm.ngeomy > 2) widthy *= rad_cyl;
+    const double cs = sqrt(prm.gam * g.f[src][FP][a] / g.f[src][FR][a]);
+    const double t0 = (fabs(g.f[src][FU][a]) + cs) / g.dx[s];
+    const double t1 = (fabs(g.f[src][FV][a] - g.rot_v[s] - FrRot * rad_cyl) + cs) / widthy;
+    double v = fmax(t0, t1);
+    if (prm.ndim == 3) {
+      double widthz = g.dz[s];
+      if (prm.ngeomz > 2) widthz *= rad;
+      const double t2 = (fabs(g.f[src][FW][a]) + cs) / widthz;
+      v = max3(t0, t1, t2);
+    }
+    m = fmax(m, v);
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) atomicMax(&ss->cfl_bits, (unsigned long long)__double_as_longlong(m));
+}
+
+// dt = min(1.1 dt_prev, courant / max)  (timestep.cu:194, 224-230)
+__global__ void k_dt_finalize(DevParams prm, StepScalars* ss) {
+  if (threadIdx.x != 0) return;
+  const double mx = __longlong_as_double((long long)ss->cfl_bits);
+  const double dtc = prm.courant / mx;
+  double dt = 1.1 * ss->dt;
+  dt = fmin(dt, dtc);
+  if (!(dt > 0.0) || isinf(dt)) ss->bad = 1;
+  ss->dt_prev = ss->dt;
+  ss->dt = dt;
+}
+
+// output_lv1 (device_func.cu:155-238): per-block sums in a fixed order
+__global__ __launch_bounds__(256) void k_reduce(DevGrid g, DevParams prm, const StepScalars* __restrict__ ss,
+                                                int src, int mode, double exclude, double* partials) {
+  __shared__ double sh[4];
+  const long ncell = (long)g.imax * g.kmax * g.jmax;
+  const Planet pl{ss->planet.m, ss->planet.x, ss->planet.y, ss->planet.vx, ss->planet.vy, ss->planet.rs};
+  double acc = 0.0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(t % g.jmax);
+    const int n = (int)(t / g.jmax);
+    const int s = slot_of(g, n % g.imax, n / g.imax);
+    int j0 = j + g.rot_j[s];
+    if (j0 < 0) j0 += g.jmax;
+    if (j0 >= g.jmax) j0 -= g.jmax;
+    const double r = g.f[src][FR][(long)s * g.jmax + j];
+    double tmp = 0.0;
+    if (mode == 0 || prm.plnt == 0) {
+      tmp = r * g.xvol[s] * g.yvol[j0] * g.zvol[s];
+    } else if (prm.ndim == 2 && (mode == 8 || mode == 9)) {
+      const double rad = g.xc[s], phi = g.yc[j0];
+      const double dd = (rad - pl.x) * (rad - pl.x) + rad * rad * (phi - pl.y) * (phi - pl.y);
+      if (dd > exclude * exclude) {
+        const double bg = pow(g.xc[s], -prm.p_alpha);
+        if (mode == 8) {
+          tmp = -grav_rad_cyl<double>(pl, rad, phi, 0.0, 0.0);
+          tmp *= (r - bg) * g.xvol[s] * g.yvol[j0];
+        } else {
+          tmp = -grav_azi_cyl<double>(pl, rad, phi, 0.0, 0.0);
+          tmp *= rad * (r - bg) * g.xvol[s] * g.yvol[j0];
+        }
+      }
+    }
+    acc += tmp;
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// mass and angular momentum with compensated per-thread accumulation
+__global__ __launch_bounds__(256) void k_totals(DevGrid g, DevParams prm, int src, double* partials) {
+  __shared__ double sh[8];
+  const long ncell = (long)g.imax * g.kmax * g.jmax;
+  double sm = 0.0, cm = 0.0, sl = 0.0, cl = 0.0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(t % g.jmax);
+    const int n = (int)(t / g.jmax);
+    const int s = slot_of(g, n % g.imax, n / g.imax);
+    int j0 = j + g.rot_j[s];
+    if (j0 >= g.jmax) j0 -= g.jmax;
+    const long a = (long)s * g.jmax + j;
+    const double dm = g.f[src][FR][a] * g.xvol[s] * g.yvol[j0] * g.zvol[s];
+    const double dl = dm * g.f[src][FV][a] * g.radc[s];
+    double y = dm - cm, tt = sm + y;
+    cm = (tt - sm) - y;
+    sm = tt;
+    y = dl - cl;
+    tt = sl + y;
+    cl = (tt - sl) - y;
+    sl = tt;
+  }
+  sm = wave_sum(sm - cm);
+  sl = wave_sum(sl - cl);
+  if ((threadIdx.x & 63) == 0) {
+    sh[threadIdx.x >> 6] = sm;
+    sh[4 + (threadIdx.x >> 6)] = sl;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    partials[2 * blockIdx.x + 1] = (sh[4] + sh[5]) + (sh[6] + sh[7]);
+  }
+}
+
+// serial Neumaier sum of nb partials (stride st) into out
+__global__ void k_sum_partials(const double* partials, int nb, int st, double* out) {
+  if (threadIdx.x != 0) return;
+  for (int q = 0; q < st; ++q) {
+    double s = 0.0, c = 0.0;
+    for (int b = 0; b < nb; ++b) {
+      const double x = partials[(long)b * st + q];
+      const double t = s + x;
+      if (fabs(s) >= fabs(x)) c += (s - t) + x;
+      else c += (x - t) + s;
+      s = t;
+    }
+    out[q] = s + c;
+  }
+}
+
+__device__ double cs2_of(const DevParams& prm, double r) { return prm.sc_h * prm.sc_h * pow(r, -prm.p_beta); }
+
+__device__ double set_M_p(const DevParams& prm, double t) {   // planet.cu:28-34
+  if (t < prm.t_growth) return fmax(prm.M_p * sin(kPi * t / (2.0 * prm.t_growth)), 0.01 * kEarthMass);
+  return prm.M_p;
+}
+
+// kick of kick_drift_kick (planet.cu:80-90) with dt = dt_step / 2
+__global__ void k_planet_half_kick(DevParams prm, StepScalars* ss) {
+  if (threadIdx.x != 0) return;
+  const double dt = 0.5 * ss->dt;
+  pn_body& b = ss->planet;
+  b.vx += dt * b.fx;
+  b.vy += ss->FrRot + dt * b.fy / b.x / b.x;
+  ss->FrRot = (prm.frrot == 1) ? b.vy : 0.0;
+  b.vy -= ss->FrRot;
+}
+
+// after the sweeps: time advance, drift, forces and the closing half kick
+// (main.cu:378-387, planet.cu:54-93).  red[1], red[2] hold the mode 8 / 9 reductions
+// when the planet moves.
+__global__ void k_planet_drift(DevParams prm, StepScalars* ss) {
+  if (threadIdx.x != 0) return;
+  const double dt = ss->dt;
+  ss->simtime += dt;
+  ss->ncycle += 1;
+  pn_body& b = ss->planet;
+  const double old_rad = b.x;
+  b.x += dt * b.vx;
+  b.y += dt * b.vy;
+  b.vy = ((b.vy + ss->FrRot) * old_rad * old_rad / b.x / b.x) - ss->FrRot;
+  if (b.y < 0.0) b.y += kTwoPi;
+  if (b.y >= kTwoPi) b.y -= kTwoPi;
+  // get_h(x) = sqrt(c_s^2 x^3); the host computed rs for the current x, so it is only
+  // re-evaluated when the planet actually moved radially
+  if (b.x != old_rad) b.rs = prm.rs_fac * sqrt(cs2_of(prm, b.x) * b.x * b.x * b.x);
+}
+
+__global__ void k_planet_forces_kick(DevParams prm, StepScalars* ss, int do_kick) {
+  if (threadIdx.x != 0) return;
+  pn_body& b = ss->planet;
+  b.m = set_M_p(prm, ss->simtime);
+  if (prm.plnt == 2) {
+    b.fx = kSigma0 * ss->red[1] / b.m - 1.0 / b.x / b.x + b.x * ((b.vy + ss->FrRot) * (b.vy + ss->FrRot));
+    b.fy = kSigma0 * ss->red[2] / b.m;
+  } else {
+    b.fx = -1.0 / b.x / b.x + b.x * ((b.vy + ss->FrRot) * (b.vy + ss->FrRot));
+    b.fy = 0.0;
+  }
+  if (do_kick) {
+    const double dt = 0.5 * ss->dt;
+    b.vx += dt * b.fx;
+    b.vy += ss->FrRot + dt * b.fy / b.x / b.x;
+    ss->FrRot = (prm.frrot == 1) ? b.vy : 0.0;
+    b.vy -= ss->FrRot;
+  }
+}
+
+__global__ void k_time_advance(StepScalars* ss) {
+  if (threadIdx.x != 0) return;
+  ss->simtime += ss->dt;
+  ss->ncycle += 1;
+}
+
+int red_blocks(const DevGrid& g) {
+  long ncell = (long)g.imax * g.kmax * g.jmax;
+  long b = (ncell + 255) / 256;
+  if (b > 2048) b = 2048;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+}  // namespace
+
+hipError_t launch_index_shift(const LaunchCtx& c) {
+  int n = c.g.imax * c.g.kmax;
+  k_index_shift<<<(n + 255) / 256, 256, 0, c.stream>>>(c.g, c.p, c.ss);
+  return hipGetLastError();
+}
+
+hipError_t launch_rotate(const LaunchCtx& c) {
+  int n = c.g.imax * c.g.kmax;
+  k_rotate<<<(n + 255) / 256, 256, 0, c.stream>>>(c.g);
+  return hipGetLastError();
+}
+
+hipError_t launch_cfl(const LaunchCtx& c, int src) {
+  k_clear_cfl<<<1, 64, 0, c.stream>>>(c.ss);
+  k_cfl<<<red_blocks(c.g), 256, 0, c.stream>>>(c.g, c.p, c.ss, src);
+  k_dt_finalize<<<1, 64, 0, c.stream>>>(c.p, c.ss);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(const LaunchCtx& c, int src, int mode, double exclude, int slot) {
+  int nb = red_blocks(c.g);
+  k_reduce<<<nb, 256, 0, c.stream>>>(c.g, c.p, c.ss, src, mode, exclude, c.partials);
+  k_sum_partials<<<1, 64, 0, c.stream>>>(c.partials, nb, 1, &c.ss->red[slot]);
+  return hipGetLastError();
+}
+
+hipError_t launch_totals(const LaunchCtx& c, int src, double* out2) {
+  int nb = red_blocks(c.g);
+  if (nb > kMaxPartials / 2) nb = kMaxPartials / 2;
+  k_totals<<<nb, 256, 0, c.stream>>>(c.g, c.p, src, c.partials);
+  k_sum_partials<<<1, 64, 0, c.stream>>>(c.partials, nb, 2, out2);
+  return hipGetLastError();
+}
+
+hipError_t launch_time_advance(const LaunchCtx& c) {
+  k_time_advance<<<1, 64, 0, c.stream>>>(c.ss);
+  return hipGetLastError();
+}
+
+hipError_t launch_planet_half_kick(const LaunchCtx& c) {
+  k_planet_half_kick<<<1, 64, 0, c.stream>>>(c.p, c.ss);
+  return hipGetLastError();
+}
+
+hipError_t launch_planet_after_sweep(const LaunchCtx& c, int src) {
+  k_planet_drift<<<1, 64, 0, c.stream>>>(c.p, c.ss);
+  if (c.p.plnt == 2) {
+    hipError_t e = launch_reduce(c, src, 8, 0.0, 1);
+    if (e != hipSuccess) return e;
+    e = launch_reduce(c, src, 9, 0.0, 2);
+    if (e != hipSuccess) return e;
+  }
+  k_planet_forces_kick<<<1, 64, 0, c.stream>>>(c.p, c.ss, 1);
+  return hipGetLastError();
+}
+
+}  // namespace pn

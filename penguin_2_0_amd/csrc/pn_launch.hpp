@@ -1,0 +1,35 @@
+// pn_launch.hpp -- host-callable launchers for the HIP kernels (one set per precision).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pn_device.hpp"
+
+namespace pn {
+
+struct LaunchCtx {
+  DevGrid g;
+  DevParams p;
+  StepScalars* ss;      // device scalars
+  double* partials;     // reduction scratch, >= kMaxPartials doubles
+  hipStream_t stream;
+};
+
+constexpr int kMaxPartials = 8192;
+
+// FP64 path (pn_kernels_fp64.hip); each returns hipSuccess or the launch error
+hipError_t fp64_sweep_x(const LaunchCtx& c, int src);
+hipError_t fp64_sweep_y(const LaunchCtx& c, int src);
+hipError_t fp64_sweep_z(const LaunchCtx& c, int src);
+
+// common small kernels (pn_kernels_common.hip)
+hipError_t launch_index_shift(const LaunchCtx& c);
+hipError_t launch_rotate(const LaunchCtx& c);
+hipError_t launch_cfl(const LaunchCtx& c, int src);              // clears, reduces, finalises ss->dt
+hipError_t launch_reduce(const LaunchCtx& c, int src, int mode, double exclude, int slot);  // -> ss->red[slot]
+hipError_t launch_totals(const LaunchCtx& c, int src, double* out2);  // out2: device, mass and angmom
+hipError_t launch_planet_half_kick(const LaunchCtx& c);
+hipError_t launch_planet_after_sweep(const LaunchCtx& c, int src);  // simtime, drift, forces, half kick
+hipError_t launch_time_advance(const LaunchCtx& c);                 // simtime += dt, ncycle++ (no planet)
+
+}  // namespace pn

@@ -1,0 +1,713 @@
+// pn_runtime.cpp -- the C ABI: contexts, hydr_ring upload/download, the step operators and
+// the device-resident step loop (optionally replayed as a HIP graph).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "penguin.h"
+#include "pn_host.hpp"
+#include "pn_launch.hpp"
+
+using namespace pn;
+
+struct pn_ctx {
+  pn_grid g{};
+  pn_params P{};
+  int precision = PN_FP64;
+  std::string err;
+  DevGrid dg{};
+  DevParams dp{};
+  StepScalars* ss = nullptr;        // device
+  double* partials = nullptr;       // device
+  double* out2 = nullptr;           // device, totals
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  int cur = 0;                      // buffer holding the state
+  bool loaded = false;
+  // host copies of the per-ring scalars the device never changes (for packing)
+  std::vector<double> h_x, h_dx, h_xc, h_xvol, h_z, h_dz, h_zc, h_zvol;
+  std::vector<double> h_y, h_dy, h_yc, h_yvol;
+  // graph of one step (2-D) or two steps (3-D, the buffers alternate)
+  hipGraphExec_t gexec = nullptr;
+  int graph_steps = 0;
+  // host copies of the fixed inputs, so a snapshot can be re-entered (restart)
+  std::vector<char> h_lft, h_rgh, h_udr, h_top;
+  std::vector<pn_symdisk> h_val;
+};
+
+namespace {
+
+int fail(pn_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define PN_HIP(c, call)                                                                        \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) return fail((c), PN_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dalloc(pn_ctx* c, T** p, size_t n) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, n * sizeof(T) + 64);
+  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "hipMalloc(%zu): %s", n * sizeof(T), hipGetErrorString(e));
+  c->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return PN_OK;
+}
+
+LaunchCtx lctx(pn_ctx* c) {
+  LaunchCtx L;
+  L.g = c->dg;
+  L.p = c->dp;
+  L.ss = c->ss;
+  L.partials = c->partials;
+  L.stream = c->stream;
+  return L;
+}
+
+int nwin(int len) { return (len + kReal - 1) / kReal; }
+
+int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p) {
+  if (g->imax < 1 || g->jmax < 2 * kNpad || g->kmax < 1) return fail(c, PN_ERR_ARG, "bad grid %d x %d x %d", g->imax, g->jmax, g->kmax);
+  if (p->ndim != 2 && p->ndim != 3) return fail(c, PN_ERR_ARG, "ndim must be 2 or 3");
+  if ((p->ndim == 2) != (g->kmax == 1)) return fail(c, PN_ERR_ARG, "ndim %d with kmax %d", p->ndim, g->kmax);
+  if (p->eos < 0 || p->eos > 2) return fail(c, PN_ERR_ARG, "eos %d", p->eos);
+  const bool cyl = (p->ngeomx == 1 && p->ngeomy == 3 && p->ngeomz == 0);
+  const bool sph = (p->ngeomx == 2 && p->ngeomy == 4 && p->ngeomz == 5);
+  if (p->ndim == 2 && !cyl) return fail(c, PN_ERR_UNSUPPORTED, "2-D needs cylindrical geometry (1,3,0)");
+  if (p->ndim == 3 && !sph) return fail(c, PN_ERR_UNSUPPORTED, "3-D needs spherical geometry (2,4,5)");
+  if (p->ndim == 3 && p->visc_flag) return fail(c, PN_ERR_UNSUPPORTED, "3-D viscosity is not built");
+  if (p->ndim == 3 && p->eos == 1) return fail(c, PN_ERR_UNSUPPORTED, "3-D EOS 1 is not built");
+  if (p->fargo_flag < 0 || p->fargo_flag > 2) return fail(c, PN_ERR_ARG, "fargo_flag %d", p->fargo_flag);
+  if (p->plnt_flag < 0 || p->plnt_flag > 2) return fail(c, PN_ERR_ARG, "plnt_flag %d", p->plnt_flag);
+  if (p->plnt_flag == 2 && p->ndim == 3) return fail(c, PN_ERR_UNSUPPORTED, "moving planet needs the 2-D force reductions");
+  return PN_OK;
+}
+
+// synchronous: the source is a caller's stack record
+int upload_scalars(pn_ctx* c, const StepScalars& h) {
+  PN_HIP(c, hipMemcpyAsync(c->ss, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+  PN_HIP(c, hipStreamSynchronize(c->stream));
+  return PN_OK;
+}
+
+int download_scalars(pn_ctx* c, StepScalars* h) {
+  PN_HIP(c, hipMemcpyAsync(h, c->ss, sizeof *h, hipMemcpyDeviceToHost, c->stream));
+  PN_HIP(c, hipStreamSynchronize(c->stream));
+  return PN_OK;
+}
+
+int sync(pn_ctx* c) {
+  PN_HIP(c, hipStreamSynchronize(c->stream));
+  return PN_OK;
+}
+
+// sweeps of one step; the state returns to c->cur in 2-D and moves to the other buffer in 3-D
+int launch_sweeps(pn_ctx* c, int& cur) {
+  LaunchCtx L = lctx(c);
+  hipError_t e;
+  if (c->precision == PN_FP64) {
+    e = fp64_sweep_x(L, cur);
+    if (e != hipSuccess) return fail(c, PN_ERR_HIP, "sweep_x: %s", hipGetErrorString(e));
+    e = fp64_sweep_y(L, cur ^ 1);
+    if (e != hipSuccess) return fail(c, PN_ERR_HIP, "sweep_y: %s", hipGetErrorString(e));
+    if (c->P.ndim == 3) {
+      e = fp64_sweep_z(L, cur);
+      if (e != hipSuccess) return fail(c, PN_ERR_HIP, "sweep_z: %s", hipGetErrorString(e));
+      cur ^= 1;
+    }
+  } else {
+    return fail(c, PN_ERR_UNSUPPORTED, "precision %d not built", c->precision);
+  }
+  if (c->P.fargo_flag > 0) {
+    e = launch_rotate(L);
+    if (e != hipSuccess) return fail(c, PN_ERR_HIP, "rotate: %s", hipGetErrorString(e));
+  }
+  return PN_OK;
+}
+
+// one full step of main.cu:345-387 on the device, reading and writing c->ss
+int launch_step(pn_ctx* c, int& cur) {
+  LaunchCtx L = lctx(c);
+  hipError_t e = launch_cfl(L, cur);
+  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "cfl: %s", hipGetErrorString(e));
+  e = launch_index_shift(L);
+  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "index_shift: %s", hipGetErrorString(e));
+  if (c->P.plnt_flag > 0) {
+    e = launch_planet_half_kick(L);
+    if (e != hipSuccess) return fail(c, PN_ERR_HIP, "planet kick: %s", hipGetErrorString(e));
+  }
+  int rc = launch_sweeps(c, cur);
+  if (rc != PN_OK) return rc;
+  if (c->P.plnt_flag > 0) {
+    e = launch_planet_after_sweep(L, cur);
+    if (e != hipSuccess) return fail(c, PN_ERR_HIP, "planet update: %s", hipGetErrorString(e));
+  } else {
+    e = launch_time_advance(L);
+    if (e != hipSuccess) return fail(c, PN_ERR_HIP, "time advance: %s", hipGetErrorString(e));
+  }
+  return PN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pn_last_error(const pn_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out) {
+  if (!g || !p || !out) return PN_ERR_ARG;
+  *out = nullptr;
+  pn_ctx* c = new pn_ctx();
+  int rc = check_params(c, g, p);
+  if (rc == PN_OK && precision != PN_FP64 && precision != PN_FP32) rc = fail(c, PN_ERR_ARG, "precision %d", precision);
+  if (rc != PN_OK) {
+    *out = c;   // caller can read the message, then pn_destroy
+    return rc;
+  }
+  c->g = *g;
+  c->P = *p;
+  c->precision = precision;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    *out = c;
+    return fail(c, PN_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  DevGrid& d = c->dg;
+  d.imax = g->imax;
+  d.jmax = g->jmax;
+  d.kmax = g->kmax;
+  d.ndim = p->ndim;
+  d.sr = g->imax + 2 * kNpad;
+  d.sk = (p->ndim == 3) ? g->kmax + 2 * kNpad : 1;
+  d.k0 = (p->ndim == 3) ? kNpad : 0;
+  d.nwin_x = nwin(g->imax);
+  d.nwin_y = nwin(g->jmax);
+  d.nwin_z = nwin(g->kmax);
+  const long nslot = static_cast<long>(d.sr) * d.sk;
+  const size_t ncell = static_cast<size_t>(nslot) * g->jmax;
+  int r2 = PN_OK;
+  for (int b = 0; b < 2 && r2 == PN_OK; ++b)
+    for (int f = 0; f < kNField && r2 == PN_OK; ++f) r2 = dalloc(c, &d.f[b][f], ncell);
+  double *x, *dx, *xc, *xvol, *z, *dz, *zc, *zvol, *radc, *cs2, *nu, *y, *dy, *yc, *yvol;
+  pn_symdisk* val;
+  if (r2 == PN_OK) r2 = dalloc(c, &x, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &dx, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &xc, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &xvol, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &z, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &dz, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &zc, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &zvol, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &radc, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &cs2, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &nu, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &d.rot_v, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &d.res_v, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &d.rot_j, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &d.inc_j, nslot);
+  if (r2 == PN_OK) r2 = dalloc(c, &y, g->jmax);
+  if (r2 == PN_OK) r2 = dalloc(c, &dy, g->jmax);
+  if (r2 == PN_OK) r2 = dalloc(c, &yc, g->jmax);
+  if (r2 == PN_OK) r2 = dalloc(c, &yvol, g->jmax);
+  if (r2 == PN_OK) r2 = dalloc(c, &val, static_cast<size_t>(g->imax) * g->kmax);
+  if (r2 == PN_OK) r2 = dalloc(c, &c->ss, 1);
+  if (r2 == PN_OK) r2 = dalloc(c, &c->partials, kMaxPartials);
+  if (r2 == PN_OK) r2 = dalloc(c, &c->out2, 2);
+  if (r2 != PN_OK) {
+    *out = c;
+    return r2;
+  }
+  d.x = x; d.dx = dx; d.xc = xc; d.xvol = xvol; d.z = z; d.dz = dz; d.zc = zc; d.zvol = zvol;
+  d.radc = radc; d.cs2 = cs2; d.nu = nu; d.y = y; d.dy = dy; d.yc = yc; d.yvol = yvol; d.val = val;
+  DevParams& q = c->dp;
+  q.gam = p->gam;
+  q.courant = p->courant;
+  q.sc_h = p->sc_h;
+  q.p_alpha = p->p_alpha;
+  q.p_beta = p->p_beta;
+  q.vis_nu = p->vis_nu;
+  q.M_p = p->M_p;
+  q.t_growth = p->t_growth;
+  q.rs_fac = p->rs_fac;
+  q.eos = p->eos;
+  q.flat = p->flat_flag;
+  q.plnt = p->plnt_flag;
+  q.visc = p->visc_flag;
+  q.fargo = p->fargo_flag;
+  q.frrot = p->frrot_flag;
+  q.ngeomx = p->ngeomx;
+  q.ngeomy = p->ngeomy;
+  q.ngeomz = p->ngeomz;
+  q.ndim = p->ndim;
+  q.has_w = (p->ndim == 3) ? 1 : 0;
+  StepScalars s0{};
+  rc = upload_scalars(c, s0);
+  if (rc == PN_OK) rc = sync(c);
+  *out = c;
+  return rc;
+}
+
+int pn_destroy(pn_ctx* c) {
+  if (!c) return PN_OK;
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void* q : c->allocs) hipFree(q);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return PN_OK;
+}
+
+int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* lft, const void* rgh, size_t lr_bytes,
+                  const void* udr, const void* top, size_t ut_bytes, const pn_symdisk* val, size_t nval) {
+  if (!c) return PN_ERR_ARG;
+  if (c->dg.f[0][0] == nullptr) return fail(c, PN_ERR_STATE, "context was not created");
+  const int I = c->g.imax, J = c->g.jmax, K = c->g.kmax;
+  const size_t rb = pn_ring_bytes(J);
+  const size_t nring = static_cast<size_t>(I) * K;
+  if (!rings || rings_bytes != rb * nring) return fail(c, PN_ERR_ARG, "rings: %zu bytes, expected %zu", rings_bytes, rb * nring);
+  if (!lft || !rgh || lr_bytes != rb * PN_NPAD * K) return fail(c, PN_ERR_ARG, "lft/rgh: %zu bytes, expected %zu", lr_bytes, rb * PN_NPAD * K);
+  if (c->P.ndim == 3 && (!udr || !top || ut_bytes != rb * PN_NPAD * I))
+    return fail(c, PN_ERR_ARG, "udr/top: %zu bytes, expected %zu", ut_bytes, rb * PN_NPAD * I);
+  if (!val || nval != nring) return fail(c, PN_ERR_ARG, "val: %zu records, expected %zu", nval, nring);
+
+  const DevGrid& d = c->dg;
+  const long nslot = static_cast<long>(d.sr) * d.sk;
+  std::vector<double> F[kNField];
+  for (int f = 0; f < kNField; ++f) F[f].assign(static_cast<size_t>(nslot) * J, 0.0);
+  std::vector<double> sx(nslot, 0.0), sdx(nslot, 1.0), sxc(nslot, 1.0), sxvol(nslot, 1.0), sz(nslot, 0.0),
+      sdz(nslot, 1.0), szc(nslot, 0.0), szvol(nslot, 1.0), srot_v(nslot, 0.0), sres_v(nslot, 0.0);
+  std::vector<int32_t> srot_j(nslot, 0), sinc_j(nslot, 0);
+  // field values are copied as bytes so that signed zeros and NaN payloads survive
+  auto take = [&](const void* base, long idx, int slot, bool interior) -> int {
+    RingView R = ring_view(const_cast<void*>(base), J, idx);
+    std::memcpy(&F[FR][static_cast<size_t>(slot) * J], R.r, 8 * J);
+    std::memcpy(&F[FP][static_cast<size_t>(slot) * J], R.p, 8 * J);
+    std::memcpy(&F[FU][static_cast<size_t>(slot) * J], R.u, 8 * J);
+    std::memcpy(&F[FV][static_cast<size_t>(slot) * J], R.v, 8 * J);
+    std::memcpy(&F[FW][static_cast<size_t>(slot) * J], R.w, 8 * J);
+    sx[slot] = *R.x; sdx[slot] = *R.dx; sxc[slot] = *R.xc; sxvol[slot] = *R.xvol;
+    sz[slot] = *R.z; sdz[slot] = *R.dz; szc[slot] = *R.zc; szvol[slot] = *R.zvol;
+    srot_v[slot] = *R.rot_v;
+    if (interior) {
+      sres_v[slot] = *R.res_v;
+      srot_j[slot] = *R.rot_j;
+      sinc_j[slot] = *R.inc_j;
+      if (*R.rot_j < 0 || *R.rot_j >= J) return fail(c, PN_ERR_LAYOUT, "ring %ld: rot_j %d outside [0, %d)", idx, *R.rot_j, J);
+    }
+    // every ring must carry the phi geometry of ring 0 (init.cpp:201-206, 316-319)
+    RingView R0 = ring_view(const_cast<void*>(rings), J, 0);
+    if (std::memcmp(R.y, R0.y, 8 * J) || std::memcmp(R.dy, R0.dy, 8 * J) || std::memcmp(R.yc, R0.yc, 8 * J) ||
+        std::memcmp(R.yvol, R0.yvol, 8 * J))
+      return fail(c, PN_ERR_LAYOUT, "ring %ld: phi geometry differs from ring 0", idx);
+    return PN_OK;
+  };
+  int rc;
+  for (int k = 0; k < K; ++k)
+    for (int i = 0; i < I; ++i) {
+      const long m = i + static_cast<long>(I) * k;
+      RingView R = ring_view(const_cast<void*>(rings), J, m);
+      if (*R.i != i || *R.k != k) return fail(c, PN_ERR_LAYOUT, "ring %ld carries (i,k) = (%d,%d), expected (%d,%d)", m, *R.i, *R.k, i, k);
+      if ((rc = take(rings, m, slot_of(d, i, k), true)) != PN_OK) return rc;
+    }
+  // ghost rings keep rot_j = 0, inc_j = 0, res_v = 0 (never rotated, never shifted)
+  for (int k = 0; k < K; ++k)
+    for (int i = 0; i < PN_NPAD; ++i) {
+      if ((rc = take(lft, i + PN_NPAD * k, slot_of(d, i - PN_NPAD, k), false)) != PN_OK) return rc;
+      if ((rc = take(rgh, i + PN_NPAD * k, slot_of(d, I + i, k), false)) != PN_OK) return rc;
+    }
+  if (c->P.ndim == 3)
+    for (int i = 0; i < I; ++i)
+      for (int kk = 0; kk < PN_NPAD; ++kk) {
+        if ((rc = take(udr, kk + PN_NPAD * i, slot_of(d, i, kk - PN_NPAD), false)) != PN_OK) return rc;
+        if ((rc = take(top, kk + PN_NPAD * i, slot_of(d, i, K + kk), false)) != PN_OK) return rc;
+      }
+  // is w carried?  It is identically zero in a 2-D state built by the reference's init; a
+  // non-zero w is advected passively like any other field
+  int has_w = (c->P.ndim == 3) ? 1 : 0;
+  if (!has_w)
+    for (double wv : F[FW])
+      if (wv != 0.0 || std::signbit(wv)) {
+        has_w = 1;
+        break;
+      }
+  c->dp.has_w = has_w;
+  // hoisted per-ring closures, evaluated on the host in FP64 exactly as the reference's
+  // device code evaluates them per cell (ppm.cu:81-82, 139; disk_profile.cu:39-42, 149-152)
+  Disk D{c->P};
+  std::vector<double> sradc(nslot, 1.0), scs2(nslot, 0.0), snu(nslot, 0.0);
+  for (long s = 0; s < nslot; ++s) {
+    double rad = (c->P.ngeomx > 0) ? sxc[s] : 1.0;
+    double pol = (c->P.ndim == 3) ? szc[s] : 0.0;
+    double rc_ = rad;
+    if (c->P.ngeomz == 5) rc_ *= std::sin(pol);
+    sradc[s] = rc_;
+    scs2[s] = D.cs2(rc_);
+    snu[s] = D.nu(rc_);
+  }
+  RingView R0 = ring_view(const_cast<void*>(rings), J, 0);
+  c->h_y.assign(R0.y, R0.y + J);
+  c->h_dy.assign(R0.dy, R0.dy + J);
+  c->h_yc.assign(R0.yc, R0.yc + J);
+  c->h_yvol.assign(R0.yvol, R0.yvol + J);
+  c->h_x = sx; c->h_dx = sdx; c->h_xc = sxc; c->h_xvol = sxvol;
+  c->h_z = sz; c->h_dz = sdz; c->h_zc = szc; c->h_zvol = szvol;
+
+  hipStream_t st = c->stream;
+  const size_t fb = static_cast<size_t>(nslot) * J * sizeof(double);
+  for (int b = 0; b < 2; ++b)
+    for (int f = 0; f < kNField; ++f) PN_HIP(c, hipMemcpyAsync(d.f[b][f], F[f].data(), fb, hipMemcpyHostToDevice, st));
+  auto up = [&](const void* dst, const void* src, size_t n) {
+    return hipMemcpyAsync(const_cast<void*>(dst), src, n, hipMemcpyHostToDevice, st);
+  };
+  const size_t sb = nslot * sizeof(double);
+  PN_HIP(c, up(d.x, sx.data(), sb));
+  PN_HIP(c, up(d.dx, sdx.data(), sb));
+  PN_HIP(c, up(d.xc, sxc.data(), sb));
+  PN_HIP(c, up(d.xvol, sxvol.data(), sb));
+  PN_HIP(c, up(d.z, sz.data(), sb));
+  PN_HIP(c, up(d.dz, sdz.data(), sb));
+  PN_HIP(c, up(d.zc, szc.data(), sb));
+  PN_HIP(c, up(d.zvol, szvol.data(), sb));
+  PN_HIP(c, up(d.radc, sradc.data(), sb));
+  PN_HIP(c, up(d.cs2, scs2.data(), sb));
+  PN_HIP(c, up(d.nu, snu.data(), sb));
+  PN_HIP(c, up(d.rot_v, srot_v.data(), sb));
+  PN_HIP(c, up(d.res_v, sres_v.data(), sb));
+  PN_HIP(c, up(d.rot_j, srot_j.data(), nslot * sizeof(int32_t)));
+  PN_HIP(c, up(d.inc_j, sinc_j.data(), nslot * sizeof(int32_t)));
+  PN_HIP(c, up(d.y, c->h_y.data(), J * sizeof(double)));
+  PN_HIP(c, up(d.dy, c->h_dy.data(), J * sizeof(double)));
+  PN_HIP(c, up(d.yc, c->h_yc.data(), J * sizeof(double)));
+  PN_HIP(c, up(d.yvol, c->h_yvol.data(), J * sizeof(double)));
+  PN_HIP(c, up(d.val, val, nring * sizeof(pn_symdisk)));
+  rc = sync(c);
+  if (rc != PN_OK) return rc;
+  if (lft != c->h_lft.data()) {
+    c->h_lft.assign(static_cast<const char*>(lft), static_cast<const char*>(lft) + lr_bytes);
+    c->h_rgh.assign(static_cast<const char*>(rgh), static_cast<const char*>(rgh) + lr_bytes);
+    if (c->P.ndim == 3) {
+      c->h_udr.assign(static_cast<const char*>(udr), static_cast<const char*>(udr) + ut_bytes);
+      c->h_top.assign(static_cast<const char*>(top), static_cast<const char*>(top) + ut_bytes);
+    }
+    c->h_val.assign(val, val + nval);
+  }
+  c->cur = 0;
+  c->loaded = true;
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  return PN_OK;
+}
+
+int pn_store_rings(pn_ctx* c, void* rings, size_t rings_bytes) {
+  if (!c) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  const int I = c->g.imax, J = c->g.jmax, K = c->g.kmax;
+  const size_t rb = pn_ring_bytes(J);
+  if (!rings || rings_bytes != rb * I * K) return fail(c, PN_ERR_ARG, "rings: %zu bytes, expected %zu", rings_bytes, rb * I * K);
+  const DevGrid& d = c->dg;
+  const long nslot = static_cast<long>(d.sr) * d.sk;
+  std::vector<double> F[kNField];
+  for (int f = 0; f < kNField; ++f) {
+    F[f].resize(static_cast<size_t>(nslot) * J);
+    PN_HIP(c, hipMemcpyAsync(F[f].data(), d.f[c->cur][f], F[f].size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  std::vector<double> rot_v(nslot), res_v(nslot);
+  std::vector<int32_t> rot_j(nslot), inc_j(nslot);
+  PN_HIP(c, hipMemcpyAsync(rot_v.data(), d.rot_v, nslot * 8, hipMemcpyDeviceToHost, c->stream));
+  PN_HIP(c, hipMemcpyAsync(res_v.data(), d.res_v, nslot * 8, hipMemcpyDeviceToHost, c->stream));
+  PN_HIP(c, hipMemcpyAsync(rot_j.data(), d.rot_j, nslot * 4, hipMemcpyDeviceToHost, c->stream));
+  PN_HIP(c, hipMemcpyAsync(inc_j.data(), d.inc_j, nslot * 4, hipMemcpyDeviceToHost, c->stream));
+  int rc = sync(c);
+  if (rc != PN_OK) return rc;
+  std::memset(rings, 0, rb * I * K);
+  for (int k = 0; k < K; ++k)
+    for (int i = 0; i < I; ++i) {
+      const long m = i + static_cast<long>(I) * k;
+      const int s = slot_of(d, i, k);
+      RingView R = ring_view(rings, J, m);
+      std::memcpy(R.r, &F[FR][static_cast<size_t>(s) * J], 8 * J);
+      std::memcpy(R.p, &F[FP][static_cast<size_t>(s) * J], 8 * J);
+      std::memcpy(R.u, &F[FU][static_cast<size_t>(s) * J], 8 * J);
+      std::memcpy(R.v, &F[FV][static_cast<size_t>(s) * J], 8 * J);
+      std::memcpy(R.w, &F[FW][static_cast<size_t>(s) * J], 8 * J);
+      *R.x = c->h_x[s]; *R.dx = c->h_dx[s]; *R.xvol = c->h_xvol[s]; *R.xc = c->h_xc[s];
+      std::memcpy(R.y, c->h_y.data(), 8 * J);
+      std::memcpy(R.dy, c->h_dy.data(), 8 * J);
+      std::memcpy(R.yvol, c->h_yvol.data(), 8 * J);
+      std::memcpy(R.yc, c->h_yc.data(), 8 * J);
+      *R.z = c->h_z[s]; *R.dz = c->h_dz[s]; *R.zvol = c->h_zvol[s]; *R.zc = c->h_zc[s];
+      *R.i = i;
+      *R.k = k;
+      *R.rot_v = rot_v[s];
+      *R.res_v = res_v[s];
+      *R.rot_j = rot_j[s];
+      *R.inc_j = inc_j[s];
+    }
+  return PN_OK;
+}
+
+int pn_get_dt(pn_ctx* c, double dt_prev, double FrRot, double* dt) {
+  if (!c || !dt) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  StepScalars h{};
+  int rc = download_scalars(c, &h);
+  if (rc != PN_OK) return rc;
+  h.dt = dt_prev;
+  h.FrRot = FrRot;
+  h.bad = 0;
+  if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
+  hipError_t e = launch_cfl(lctx(c), c->cur);
+  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "cfl: %s", hipGetErrorString(e));
+  if ((rc = download_scalars(c, &h)) != PN_OK) return rc;
+  *dt = h.dt;
+  if (h.bad) return fail(c, PN_ERR_NUMERIC, "non-finite or non-positive dt %g", h.dt);
+  return PN_OK;
+}
+
+int pn_index_shift(pn_ctx* c, double dt, double FrRot) {
+  if (!c) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  if (!(dt > 0.0)) return fail(c, PN_ERR_ARG, "dt %g", dt);
+  StepScalars h{};
+  int rc = download_scalars(c, &h);
+  if (rc != PN_OK) return rc;
+  h.dt = dt;
+  h.FrRot = FrRot;
+  if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
+  hipError_t e = launch_index_shift(lctx(c));
+  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "index_shift: %s", hipGetErrorString(e));
+  return sync(c);
+}
+
+int pn_sweep2d(pn_ctx* c, double dt, const pn_body* planet, double FrRot) {
+  if (!c || !planet) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  if (!(dt > 0.0)) return fail(c, PN_ERR_ARG, "dt %g", dt);
+  StepScalars h{};
+  int rc = download_scalars(c, &h);
+  if (rc != PN_OK) return rc;
+  h.dt = dt;
+  h.FrRot = FrRot;
+  h.planet = *planet;
+  if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
+  int cur = c->cur;
+  if ((rc = launch_sweeps(c, cur)) != PN_OK) return rc;
+  if ((rc = sync(c)) != PN_OK) return rc;
+  c->cur = cur;
+  return PN_OK;
+}
+
+int pn_sweep_axis(pn_ctx* c, int axis, double dt, const pn_body* planet, double FrRot) {
+  if (!c || !planet) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  if (!(dt > 0.0)) return fail(c, PN_ERR_ARG, "dt %g", dt);
+  if (axis < 0 || axis > 2 || (axis == 2 && c->P.ndim != 3)) return fail(c, PN_ERR_ARG, "axis %d", axis);
+  if (c->precision != PN_FP64) return fail(c, PN_ERR_UNSUPPORTED, "single sweeps run on the FP64 path");
+  StepScalars h{};
+  int rc = download_scalars(c, &h);
+  if (rc != PN_OK) return rc;
+  h.dt = dt;
+  h.FrRot = FrRot;
+  h.planet = *planet;
+  if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
+  LaunchCtx L = lctx(c);
+  hipError_t e = (axis == 0) ? fp64_sweep_x(L, c->cur) : (axis == 1) ? fp64_sweep_y(L, c->cur) : fp64_sweep_z(L, c->cur);
+  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "sweep %d: %s", axis, hipGetErrorString(e));
+  if ((rc = sync(c)) != PN_OK) return rc;
+  c->cur ^= 1;
+  return PN_OK;
+}
+
+int pn_time_kernels(pn_ctx* c, int nrep, double* ms6) {
+  if (!c || !ms6 || nrep < 1) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  for (int q = 0; q < 6; ++q) ms6[q] = 0.0;
+  hipEvent_t e0, e1;
+  PN_HIP(c, hipEventCreate(&e0));
+  PN_HIP(c, hipEventCreate(&e1));
+  LaunchCtx L = lctx(c);
+  int rc = PN_OK;
+  for (int q = 0; q < 6 && rc == PN_OK; ++q) {
+    if (q == 4 && c->P.ndim != 3) continue;
+    if (q == 5 && c->P.fargo_flag == 0) continue;
+    if (q >= 2 && q <= 4 && c->precision != PN_FP64) continue;
+    hipError_t e = hipEventRecord(e0, c->stream);
+    for (int k = 0; k < nrep && e == hipSuccess; ++k) {
+      switch (q) {
+        case 0: e = launch_cfl(L, c->cur); break;
+        case 1: e = launch_index_shift(L); break;
+        case 2: e = fp64_sweep_x(L, c->cur); c->cur ^= 1; break;
+        case 3: e = fp64_sweep_y(L, c->cur); c->cur ^= 1; break;
+        case 4: e = fp64_sweep_z(L, c->cur); c->cur ^= 1; break;
+        case 5: e = launch_rotate(L); break;
+      }
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) rc = fail(c, PN_ERR_HIP, "timing kernel %d: %s", q, hipGetErrorString(e));
+    ms6[q] = static_cast<double>(ms) / nrep;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return rc;
+}
+
+int pn_reduce(pn_ctx* c, const pn_body* planet, double exclude, int mode, double* out) {
+  if (!c || !planet || !out) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  if (mode != 0 && mode != 8 && mode != 9) return fail(c, PN_ERR_ARG, "mode %d", mode);
+  StepScalars h{};
+  int rc = download_scalars(c, &h);
+  if (rc != PN_OK) return rc;
+  h.planet = *planet;
+  if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
+  hipError_t e = launch_reduce(lctx(c), c->cur, mode, exclude * c->P.sc_h, 3);
+  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "reduce: %s", hipGetErrorString(e));
+  if ((rc = download_scalars(c, &h)) != PN_OK) return rc;
+  *out = h.red[3];
+  return PN_OK;
+}
+
+int pn_totals(pn_ctx* c, double* mass, double* angmom) {
+  if (!c || !mass || !angmom) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  hipError_t e = launch_totals(lctx(c), c->cur, c->out2);
+  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "totals: %s", hipGetErrorString(e));
+  double h[2];
+  PN_HIP(c, hipMemcpyAsync(h, c->out2, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  int rc = sync(c);
+  if (rc != PN_OK) return rc;
+  *mass = h[0];
+  *angmom = h[1];
+  return PN_OK;
+}
+
+int pn_planet_forces(pn_ctx* c, pn_body* b, double FrRot, double simtime) {
+  if (!c || !b) return PN_ERR_ARG;
+  const pn_params& P = c->P;
+  b->m = (simtime < P.t_growth) ? std::fmax(P.M_p * std::sin(3.1415926535897932384626433832795 * simtime / (2.0 * P.t_growth)), 0.01 * 0.000003)
+                                : P.M_p;
+  if (P.plnt_flag == 2) {
+    double f8 = 0.0, f9 = 0.0;
+    int rc = pn_reduce(c, b, 0.0, 8, &f8);
+    if (rc == PN_OK) rc = pn_reduce(c, b, 0.0, 9, &f9);
+    if (rc != PN_OK) return rc;
+    const double Sigma_0 = 1.0 * 0.00019126835;
+    b->fx = Sigma_0 * f8 / b->m - 1.0 / b->x / b->x + b->x * ((b->vy + FrRot) * (b->vy + FrRot));
+    b->fy = Sigma_0 * f9 / b->m;
+  } else {
+    b->fx = -1.0 / b->x / b->x + b->x * ((b->vy + FrRot) * (b->vy + FrRot));
+    b->fy = 0.0;
+  }
+  return PN_OK;
+}
+
+int pn_run_begin(pn_ctx* c, pn_run_state* st) {
+  if (!c || !st) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  std::memset(st, 0, sizeof *st);
+  st->dt = c->P.endtime;
+  st->simtime = 0.0;
+  pn_init_planet(&c->P, &st->planet, &st->FrRot);
+  int rc = pn_planet_forces(c, &st->planet, st->FrRot, st->simtime);
+  if (rc != PN_OK) return rc;
+  return pn_index_shift(c, st->dt, st->FrRot);
+}
+
+int pn_run(pn_ctx* c, int nsteps, int graph, pn_run_state* st) {
+  if (!c || !st || nsteps < 0) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  StepScalars h{};
+  h.dt = st->dt;
+  h.FrRot = st->FrRot;
+  h.simtime = st->simtime;
+  h.planet = st->planet;
+  h.ncycle = st->ncycle;
+  int rc = upload_scalars(c, h);
+  if (rc != PN_OK) return rc;
+  const int per_graph = (c->P.ndim == 3) ? 2 : 1;
+  int done = 0;
+  if (graph && nsteps >= per_graph) {
+    if (!c->gexec) {
+      hipGraph_t gr;
+      PN_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int cur = c->cur;
+      for (int s = 0; s < per_graph && rc == PN_OK; ++s) rc = launch_step(c, cur);
+      hipError_t e = hipStreamEndCapture(c->stream, &gr);
+      if (rc != PN_OK) return rc;
+      if (e != hipSuccess) return fail(c, PN_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+      if (cur != c->cur) return fail(c, PN_ERR_STATE, "captured steps do not return to their buffer");
+      e = hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0);
+      hipGraphDestroy(gr);
+      if (e != hipSuccess) return fail(c, PN_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      c->graph_steps = per_graph;
+    }
+    while (nsteps - done >= per_graph) {
+      PN_HIP(c, hipGraphLaunch(c->gexec, c->stream));
+      done += per_graph;
+    }
+  }
+  int cur = c->cur;
+  for (; done < nsteps; ++done)
+    if ((rc = launch_step(c, cur)) != PN_OK) return rc;
+  c->cur = cur;
+  if ((rc = download_scalars(c, &h)) != PN_OK) return rc;
+  st->dt = h.dt;
+  st->FrRot = h.FrRot;
+  st->simtime = h.simtime;
+  st->planet = h.planet;
+  st->ncycle = h.ncycle;
+  if (h.bad) return fail(c, PN_ERR_NUMERIC, "non-finite or non-positive dt during the run");
+  return PN_OK;
+}
+
+int pn_save_snapshot(pn_ctx* c, const char* path) {
+  if (!c || !path) return PN_ERR_ARG;
+  const size_t n = pn_ring_bytes(c->g.jmax) * c->g.imax * c->g.kmax;
+  std::vector<char> buf(n);
+  int rc = pn_store_rings(c, buf.data(), n);
+  if (rc != PN_OK) return rc;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(c, PN_ERR_IO, "cannot open %s for writing", path);
+  size_t w = std::fwrite(buf.data(), 1, n, f);
+  int cl = std::fclose(f);
+  if (w != n || cl != 0) return fail(c, PN_ERR_IO, "short write to %s", path);
+  return PN_OK;
+}
+
+int pn_load_snapshot(pn_ctx* c, const char* path) {
+  if (!c || !path) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "ghost rings and background must be loaded first");
+  const size_t n = pn_ring_bytes(c->g.jmax) * c->g.imax * c->g.kmax;
+  std::vector<char> buf(n);
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(c, PN_ERR_IO, "snapshot %s does not exist", path);
+  size_t r = std::fread(buf.data(), 1, n, f);
+  int extra = std::fgetc(f);
+  std::fclose(f);
+  if (r != n) return fail(c, PN_ERR_IO, "snapshot %s holds %zu bytes, expected %zu", path, r, n);
+  if (extra != EOF) return fail(c, PN_ERR_IO, "snapshot %s is longer than %zu bytes", path, n);
+  const size_t lr = c->h_lft.size(), ut = c->h_udr.size();
+  return pn_load_rings(c, buf.data(), n, c->h_lft.data(), c->h_rgh.data(), lr, ut ? c->h_udr.data() : nullptr,
+                       ut ? c->h_top.data() : nullptr, ut, c->h_val.data(), c->h_val.size());
+}
+
+}  // extern "C"
