@@ -1,0 +1,141 @@
+"""Host side of the C ABI, no GPU needed: the library loads and exports every declared
+entry point, parameter defaults equal global.h, the hydr_ring layout, synthetic disks equal
+the oracle's byte for byte, the planet's host integrator, and argument errors."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from penguin_2_0_amd import abi, configs
+from penguin_2_0_amd.abi import PnBody, PnGrid, PnParams
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "penguin.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pn_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_entry_point():
+    lib = abi.load_library()
+    names = declared_symbols()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_defaults_are_global_h():
+    p = abi.params_default()
+    twopi = 6.283185307179586476925286766559
+    hpi = 1.570796326794896619231321691639751442
+    assert (p.ndim, p.eos, p.flat_flag, p.plnt_flag, p.frrot_flag, p.visc_flag, p.fargo_flag) == (2, 0, 1, 2, 1, 1, 1)
+    assert (p.ngeomx, p.ngeomy, p.ngeomz, p.grspx, p.grspy, p.grspz) == (1, 3, 0, 1, 0, 0)
+    assert (p.p_alpha, p.p_beta, p.ss_alpha, p.sc_h) == (1.5, 0.5, 0.0001, 0.05)
+    assert p.vis_nu == 0.0001 * 0.05 * 0.05
+    assert (p.M_p, p.R_p, p.t_growth, p.rs_fac) == (0.001, 1.0, 1000.0 * twopi, 0.5)
+    assert (p.gam, p.courant, p.endtime, p.tmovie) == (1.0, 0.5, 10000.0 * twopi, 200.0 * twopi)
+    assert (p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax) == (0.2, 3.0, 0.0, twopi, hpi - 0.065, hpi)
+
+
+def test_ring_layout():
+    lib = abi.load_library()
+    for J in (1, 12, 768, 3072, 12288):
+        assert lib.pn_ring_bytes(J) == 72 * J + 96 == abi.ring_dtype(J).itemsize
+    assert lib.pn_ring_bytes(768) == 55392 and lib.pn_ring_bytes(12288) == 884832
+    dt = abi.ring_dtype(10)
+    off = {n: dt.fields[n][1] for n in dt.names}
+    # member order of variable_types.h:67-97
+    assert off["r"] == 0 and off["w"] == 320 and off["x"] == 400 and off["xc"] == 424
+    assert off["y"] == 432 and off["yc"] == 672 and off["z"] == 752 and off["zc"] == 776
+    assert off["i"] == 784 and off["k"] == 788 and off["rot_v"] == 792 and off["res_v"] == 800
+    assert off["rot_j"] == 808 and off["inc_j"] == 812
+
+
+CASES = {
+    "c1": lambda: configs.c1(),
+    "viscous": lambda: configs.c4_members()[0],
+    "adiabatic_small": lambda: configs.Workload("adi", PnGrid(40, 96, 1), configs.c5().params, 5, 1),
+    "sph3d_small": lambda: configs.Workload("sph", PnGrid(16, 36, 8), configs.c3().params, 3, 1),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_init_disk_matches_oracle_bytes(case):
+    w = CASES[case]()
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    o = oracle.OracleDisk(w.grid, w.params, w.seed, w.noise)
+    assert np.array_equal(d.rings.view(np.uint8), o.rings.view(np.uint8))
+    assert np.array_equal(d.lft.view(np.uint8), o.lft.view(np.uint8))
+    assert np.array_equal(d.rgh.view(np.uint8), o.rgh.view(np.uint8))
+    if w.params.ndim == 3:
+        assert np.array_equal(d.udr.view(np.uint8), o.udr.view(np.uint8))
+        assert np.array_equal(d.top.view(np.uint8), o.top.view(np.uint8))
+    ov = np.ctypeslib.as_array(C.cast(o.val, C.POINTER(C.c_double)), shape=(len(o.val), 5))
+    assert np.array_equal(d.val_array(), ov)
+    # ring order i + imax k and the ring's own (i, k)
+    n = np.arange(w.grid.imax * w.grid.kmax)
+    assert np.array_equal(d.rings["i"], n % w.grid.imax) and np.array_equal(d.rings["k"], n // w.grid.imax)
+
+
+def test_planet_host_integrator_matches_oracle():
+    lib = abi.load_library()
+    olib = oracle.lib()
+    for P in (configs.c1().params, abi.params_default()):
+        b1, b2 = PnBody(), PnBody()
+        f1, f2 = C.c_double(), C.c_double()
+        lib.pn_init_planet(C.byref(P), C.byref(b1), C.byref(f1))
+        olib.orc_init_planet(C.byref(P), C.byref(b2), C.byref(f2))
+        assert bytes(b1) == bytes(b2) and f1.value == f2.value
+        # give it a radial kick so drift and kick do something
+        for b in (b1, b2):
+            b.fx, b.fy, b.vx = 1e-3, 2e-4, 1e-3
+        for dt, kick in ((0.01, 1), (0.02, 0), (0.01, 1)):
+            lib.pn_kick_drift_kick(C.byref(P), C.byref(b1), C.byref(f1), dt, kick)
+            olib.orc_kick_drift_kick(C.byref(P), C.byref(b2), C.byref(f2), dt, kick)
+            assert bytes(b1) == bytes(b2) and f1.value == f2.value
+    # init_planet closed forms: Keplerian planet at R_p, co-rotating frame
+    P = configs.c1().params
+    b, f = PnBody(), C.c_double()
+    lib.pn_init_planet(C.byref(P), C.byref(b), C.byref(f))
+    assert b.x == 1.0 and b.y == math.pi and b.vy == 0.0 and f.value == 1.0
+    assert abs(b.rs - 0.6 * 0.05) < 1e-16 and b.m == P.M_p   # t_growth 0: full mass
+
+
+def test_argument_errors_before_any_device_call():
+    lib = abi.load_library()
+    g = PnGrid(64, 96, 1)
+    for bad in (dict(ndim=3), dict(eos=7), dict(ngeomx=2), dict(fargo_flag=5), dict(plnt_flag=3)):
+        P = configs.disk2d(64, 96).replace(**bad)
+        h = C.c_void_p()
+        rc = lib.pn_create(C.byref(g), C.byref(P), 0, C.byref(h))
+        assert rc in (abi.PN_ERR_ARG, abi.PN_ERR_UNSUPPORTED), (bad, rc)
+        assert lib.pn_last_error(h)
+        lib.pn_destroy(h)
+    with pytest.raises(abi.PnError):
+        abi.Context(PnGrid(64, 8, 1), configs.disk2d(64, 8))   # fewer phi cells than two halos
+    # the synthetic-disk builder rejects a grid that contradicts ndim
+    d = abi.Disk(PnGrid(8, 24, 1), configs.disk2d(8, 24))
+    rc = lib.pn_init_disk(C.byref(PnGrid(8, 24, 2)), C.byref(configs.disk2d(8, 24)), 1, 0.0,
+                          d.rings.ctypes.data_as(C.c_void_p), d.lft.ctypes.data_as(C.c_void_p),
+                          d.rgh.ctypes.data_as(C.c_void_p), None, None, d.val)
+    assert rc == abi.PN_ERR_ARG
+
+
+def _gpu_visible() -> bool:
+    import os
+    return any(Path(p).exists() for p in ("/dev/kfd",)) and os.access("/dev/kfd", os.R_OK | os.W_OK)
+
+
+@pytest.mark.skipif(_gpu_visible(), reason="checks the no-device error path")
+def test_create_without_device_fails_loudly():
+    with pytest.raises(abi.PnError) as e:
+        abi.Context(PnGrid(32, 48, 1), configs.disk2d(32, 48))
+    assert e.value.code == abi.PN_ERR_HIP
