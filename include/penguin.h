@@ -194,6 +194,16 @@ int pn_run(pn_ctx* ctx, int nsteps, int graph, pn_run_state* st);
  * seconds per step, main.cu:430-437).  The state advances while it is timed. */
 int pn_time_kernels(pn_ctx* ctx, int nrep, double* ms6);
 
+/* Autotune the FP32 sweeps: time every (cells per lane, waves per workgroup)
+ * variant of the radial and the azimuthal sweep on the current state (nrep launches
+ * each, state left unchanged) and keep the fastest.  *var_x / *var_y receive the
+ * choice (may be NULL).  FP64 contexts have one variant (0).  pn_run tunes an FP32
+ * context on its first call.  No counterpart in the reference. */
+int pn_tune(pn_ctx* ctx, int nrep, int* var_x, int* var_y);
+/* Pin the FP32 sweep variants (0..3: K = 2, 2, 4, 4 cells per lane with 2, 4, 2, 4
+ * waves per workgroup) instead of tuning. */
+int pn_set_variants(pn_ctx* ctx, int var_x, int var_y);
+
 /* Mass and angular-momentum totals in FP64 with compensated summation. */
 int pn_totals(pn_ctx* ctx, double* mass, double* angmom);
 

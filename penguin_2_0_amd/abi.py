@@ -106,6 +106,8 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_reduce": (i, [vp, P(PnBody), d, i, P(d)]),
         "pn_sweep_axis": (i, [vp, i, d, P(PnBody), d]),
         "pn_time_kernels": (i, [vp, i, P(d)]),
+        "pn_tune": (i, [vp, i, P(i), P(i)]),
+        "pn_set_variants": (i, [vp, i, i]),
         "pn_init_planet": (None, [P(PnParams), P(PnBody), P(d)]),
         "pn_planet_forces": (i, [vp, P(PnBody), d, d]),
         "pn_kick_drift_kick": (None, [P(PnParams), P(PnBody), P(d), d, i]),
@@ -237,6 +239,14 @@ class Context:
         ms = (C.c_double * 6)()
         self._chk(self.lib.pn_time_kernels(self.h, nrep, ms), "pn_time_kernels")
         return dict(zip(("cfl", "index_shift", "sweep_x", "sweep_y", "sweep_z", "rotate"), list(ms)))
+
+    def tune(self, nrep: int = 3) -> tuple[int, int]:
+        vx, vy = C.c_int(), C.c_int()
+        self._chk(self.lib.pn_tune(self.h, nrep, C.byref(vx), C.byref(vy)), "pn_tune")
+        return vx.value, vy.value
+
+    def set_variants(self, var_x: int, var_y: int):
+        self._chk(self.lib.pn_set_variants(self.h, var_x, var_y), "pn_set_variants")
 
     def reduce(self, planet: PnBody, exclude: float, mode: int) -> float:
         d = C.c_double()

@@ -33,8 +33,13 @@ HIP_COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
+# the FP32 sweeps use native reciprocal / sqrt / exp / log (no reassociation: the order of
+# every sum is the source order)
+FAST_F32 = ["-ffp-contract=fast", "-fno-math-errno", "-fno-honor-infinities", "-fno-honor-nans",
+            "-fno-signed-zeros", "-freciprocal-math", "-fapprox-func", "-fgpu-flush-denormals-to-zero"]
 UNITS = {
     "pn_kernels_fp64.hip": ["-ffp-contract=off"],
+    "pn_kernels_fp32.hip": FAST_F32,
     "pn_kernels_common.hip": ["-ffp-contract=off"],
     "pn_runtime.cpp": ["-ffp-contract=off"],
     "pn_host.cpp": ["-ffp-contract=off"],

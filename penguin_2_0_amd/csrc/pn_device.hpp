@@ -57,7 +57,7 @@ struct StepScalars {
   double FrRot;
   double simtime;
   pn_body planet;
-  unsigned long long cfl_bits;   // max signal speed / width as ordered bits (>= 0)
+  double cfl_max;     // max signal speed / width behind the last dt
   double red[4];      // reduction results (mass, modes 8 / 9)
   int64_t ncycle;
   int bad;            // non-finite dt seen

@@ -43,10 +43,6 @@ __global__ void k_rotate(DevGrid g) {
   g.rot_j[s] = j;
 }
 
-__global__ void k_clear_cfl(StepScalars* ss) {
-  if (threadIdx.x == 0) ss->cfl_bits = 0ull;
-}
-
 __device__ __forceinline__ double wave_max(double v) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
   return v;
@@ -56,9 +52,11 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// CFL signal speed per cell width (timestep.cu:12-73) over every interior cell; the max is
-// order independent, so a wavefront max plus one atomic per wave is exact
-__global__ __launch_bounds__(256) void k_cfl(DevGrid g, DevParams prm, StepScalars* ss, int src) {
+// CFL signal speed per cell width (timestep.cu:12-73) over every interior cell, one maximum
+// per block into cflp (a maximum is order independent, so the split is exact)
+__global__ __launch_bounds__(256) void k_cfl(DevGrid g, DevParams prm, const StepScalars* __restrict__ ss,
+                                             double* cflp, int src) {
+  __shared__ double sh[4];
   const long ncell = (long)g.imax * g.kmax * g.jmax;
   const double FrRot = ss->FrRot;
   double m = 0.0;
@@ -83,13 +81,23 @@ __global__ __launch_bounds__(256) void k_cfl(DevGrid g, DevParams prm, StepScala
     m = fmax(m, v);
   }
   m = wave_max(m);
-  if ((threadIdx.x & 63) == 0) atomicMax(&ss->cfl_bits, (unsigned long long)__double_as_longlong(m));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) cflp[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
-// dt = min(1.1 dt_prev, courant / max)  (timestep.cu:194, 224-230)
-__global__ void k_dt_finalize(DevParams prm, StepScalars* ss) {
+// dt = min(1.1 dt_prev, courant / max)  (timestep.cu:194, 224-230), max over n partials
+__global__ __launch_bounds__(256) void k_dt_finalize(DevParams prm, StepScalars* ss, const double* __restrict__ cflp,
+                                                     int n) {
+  __shared__ double sh[4];
+  double m = 0.0;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) m = fmax(m, cflp[t]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
   if (threadIdx.x != 0) return;
-  const double mx = __longlong_as_double((long long)ss->cfl_bits);
+  const double mx = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+  ss->cfl_max = mx;
   const double dtc = prm.courant / mx;
   double dt = 1.1 * ss->dt;
   dt = fmin(dt, dtc);
@@ -276,9 +284,10 @@ hipError_t launch_rotate(const LaunchCtx& c) {
 }
 
 hipError_t launch_cfl(const LaunchCtx& c, int src) {
-  k_clear_cfl<<<1, 64, 0, c.stream>>>(c.ss);
-  k_cfl<<<red_blocks(c.g), 256, 0, c.stream>>>(c.g, c.p, c.ss, src);
-  k_dt_finalize<<<1, 64, 0, c.stream>>>(c.p, c.ss);
+  const int nb = red_blocks(c.g);
+  if (nb > c.ncflp) return hipErrorInvalidValue;
+  k_cfl<<<nb, 256, 0, c.stream>>>(c.g, c.p, c.ss, c.cflp, src);
+  k_dt_finalize<<<1, 256, 0, c.stream>>>(c.p, c.ss, c.cflp, nb);
   return hipGetLastError();
 }
 
@@ -294,6 +303,17 @@ hipError_t launch_totals(const LaunchCtx& c, int src, double* out2) {
   if (nb > kMaxPartials / 2) nb = kMaxPartials / 2;
   k_totals<<<nb, 256, 0, c.stream>>>(c.g, c.p, src, c.partials);
   k_sum_partials<<<1, 64, 0, c.stream>>>(c.partials, nb, 2, out2);
+  return hipGetLastError();
+}
+
+hipError_t launch_dt_finalize(const LaunchCtx& c, int n) {
+  if (n < 1 || n > c.ncflp) return hipErrorInvalidValue;
+  k_dt_finalize<<<1, 256, 0, c.stream>>>(c.p, c.ss, c.cflp, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(const LaunchCtx& c, int nb, int stride, double* out) {
+  k_sum_partials<<<1, 64, 0, c.stream>>>(c.partials, nb, stride, out);
   return hipGetLastError();
 }
 

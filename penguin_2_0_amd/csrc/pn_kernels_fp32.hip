@@ -1,0 +1,421 @@
+// pn_kernels_fp32.hip -- the bulk-FP32 sweeps (2-D isothermal disk) and their helpers.
+//
+// State in HBM: three FP32 fields per cell -- Sigma - Sigma_b, v_r and v_phi - v_bg --
+// where Sigma_b and v_bg are the ring's FP64 background (the SymDisk record; a ghost ring's
+// own uniform values).  P = c_s^2 Sigma is not stored (the isothermal closure the reference
+// applies after every sweep, ppm.cu:139), and w is identically zero in 2-D.  Everything that
+// must stay FP64 stays FP64 outside the sweeps: dt, the FARGO split and roll, the planet,
+// the time, the backgrounds and residual rates, and the totals.
+//
+// Work decomposition as in the FP64 sweeps, with K cells per lane (window 64 K, halo 6) and
+// W waves per workgroup picked by the autotuner; the CFL maximum of the next step is taken
+// in the azimuthal sweep's epilogue from the values it stores, one partial per wave.
+#include "pn_f32.hpp"
+#include "ppm_wave_f32.hpp"
+
+namespace pn {
+
+namespace {
+
+__device__ __forceinline__ int xcd_swizzle(int bid, int nblk) {
+  const int nx = 8;
+  int q = nblk / nx, rem = nblk % nx;
+  int x = bid % nx, idx = bid / nx;
+  return x * q + (x < rem ? x : rem) + idx;
+}
+
+__device__ __forceinline__ int wrapj(int j, int jmax) {
+  if (j < 0) j += jmax;
+  if (j >= jmax) j -= jmax;
+  return j;
+}
+
+template <int K>
+__device__ __forceinline__ int win_len_k(int len, int nwin, int l) {
+  constexpr int WIN = 64 * K, REAL = WIN - 2 * kNpad;
+  return (l < nwin - 1) ? WIN : (len - REAL * (nwin - 1) + 2 * kNpad);
+}
+
+__device__ __forceinline__ PlanetF planet_f(const StepScalars* ss) {
+  PlanetF p;
+  p.m = (float)ss->planet.m;
+  p.x = (float)ss->planet.x;
+  p.rs2 = (float)(ss->planet.rs * ss->planet.rs);
+  p.inv_x2 = (float)(1.0 / (ss->planet.x * ss->planet.x));
+  p.vy = (float)ss->planet.vy;
+  return p;
+}
+
+__device__ __forceinline__ float wave_maxf(float m) {
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  return m;
+}
+
+// per step: sin / cos of half the angle from the planet to each lab column centre (FP64),
+// and each slot's vrot = res_v / r after Index_Shift
+__global__ void k_prep(DevGrid g, DevF32 f, const StepScalars* __restrict__ ss, int nslot) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < g.jmax) {
+    const double a = 0.5 * (g.yc[t] - ss->planet.y);
+    f.trig[t] = make_float2((float)sin(a), (float)cos(a));
+  }
+  if (t < nslot) reinterpret_cast<float*>(f.tab)[t * 4 * kSlotQ + 18] = (float)(g.res_v[t] / g.radc[t]);
+}
+
+// ---------------------------------------------------------------------------------------
+template <int K, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_sweep_x_f32(DevGrid g, DevF32 f, DevParams prm,
+                                                          const StepScalars* __restrict__ ss, int src) {
+  constexpr int REAL = 64 * K - 2 * kNpad;
+  const int lane = threadIdx.x & 63;
+  const int gw = xcd_swizzle(blockIdx.x, gridDim.x) * WPB + (threadIdx.x >> 6);
+  const int jmax = g.jmax;
+  const int j0 = gw % jmax;
+  const int l = gw / jmax;
+  if (l >= f.nwin_x) return;   // wave-uniform; no barriers in this kernel
+  const float dt = (float)ss->dt;
+  const PlanetF pl = planet_f(ss);
+  const int lim = win_len_k<K>(g.imax, f.nwin_x, l);
+  const float* __restrict__ fr = f.f[src][0];
+  const float* __restrict__ fu = f.f[src][1];
+  const float* __restrict__ fv = f.f[src][2];
+  const float4* __restrict__ tab = f.tab;
+  const float2 tr = f.trig[j0];
+
+  WinP<K> W;
+  int sl[K];
+  long addr[K];
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    const int c = K * lane + h;
+    const int i_ring = REAL * l - kNpad + (c < lim ? c : lim - 1);
+    const int s = slot_of(g, i_ring, 0);
+    sl[h] = s;
+    const int j = wrapj(j0 - g.rot_j[s], jmax);
+    const long a = (long)s * jmax + j;
+    addr[h] = a;
+    const float4 q0 = tab[s * kSlotQ], q1 = tab[s * kSlotQ + 1], q2 = tab[s * kSlotQ + 2], q4 = tab[s * kSlotQ + 4];
+    W.rad[h] = q0.x;
+    W.xa0[h] = q0.y;
+    W.dx0[h] = q0.z;
+    W.rb[h] = q1.x;
+    W.drb[h] = q1.y;
+    W.pb[h] = q1.z;
+    W.dpb[h] = q1.w;
+    W.cs2[h] = q2.x;
+    W.cs[h] = q2.y;
+    W.Lb[h] = q2.z;
+    W.dLb[h] = q2.w;
+    W.vrot[h] = q4.z;
+    W.dvol0[h] = q4.w;
+    W.ts[h] = tr.x;
+    W.tc[h] = tr.y;
+    W.r[h] = fr[a];
+    W.u[h] = fu[a];
+    W.v[h] = q0.x * fv[a];   // l - L_b = r (v_phi - v_bg); r_cyl = r in 2-D
+  }
+  ppmlr_pert<K, 0>(W, tab, sl, lim, dt, 0.0f, pl, prm.plnt > 0, prm.flat != 0, sqrtf((float)prm.gam));
+
+  float* __restrict__ dr = f.f[src ^ 1][0];
+  float* __restrict__ du = f.f[src ^ 1][1];
+  float* __restrict__ dv = f.f[src ^ 1][2];
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    const int c = K * lane + h;
+    if (c >= kNpad && c < lim - kNpad) {
+      const long a = addr[h];
+      dr[a] = W.r[h];
+      du[a] = W.u[h];
+      dv[a] = W.v[h] * tab[sl[h] * kSlotQ].w;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+template <int K, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, DevParams prm,
+                                                          const StepScalars* __restrict__ ss, double* cflp, int src,
+                                                          int do_cfl) {
+  constexpr int REAL = 64 * K - 2 * kNpad;
+  const int lane = threadIdx.x & 63;
+  const int gw = xcd_swizzle(blockIdx.x, gridDim.x) * WPB + (threadIdx.x >> 6);
+  const int l = gw % f.nwin_y;
+  const int n = gw / f.nwin_y;
+  if (n >= g.imax) return;   // wave-uniform; no barriers in this kernel
+  const int s = slot_of(g, n, 0);
+  const int jmax = g.jmax;
+  const float dt = (float)ss->dt;
+  const double FrRot = ss->FrRot;
+  const PlanetF pl = planet_f(ss);
+  const int lim = win_len_k<K>(jmax, f.nwin_y, l);
+  const int rot = g.rot_j[s];
+  const double rcd = g.radc[s];
+  const double vbg = f.vbg[s];
+  // grid motion of the shifted sweep: (v_bg - res_v)/r - FrRot; the v_phi offset the CFL
+  // sees: v_bg - rot_v - FrRot r (timestep.cu:62)
+  const float wres = (float)((vbg - g.res_v[s]) / rcd - FrRot);
+  const float cfl_off = (float)(vbg - g.rot_v[s] - FrRot * rcd);
+  const float4 q0 = f.tab[s * kSlotQ], q1 = f.tab[s * kSlotQ + 1], q2 = f.tab[s * kSlotQ + 2],
+               q4 = f.tab[s * kSlotQ + 4];
+  const float rc = q0.x;
+  const long base = (long)s * jmax;
+  const float* __restrict__ fr = f.f[src][0] + base;
+  const float* __restrict__ fu = f.f[src][1] + base;
+  const float* __restrict__ fv = f.f[src][2] + base;
+
+  WinP<K> W;
+  int jd[K];
+  C<float, K> dyc;
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    const int c = K * lane + h;
+    const int jref = REAL * l - kNpad + (c < lim ? c : lim - 1);
+    const int j0 = wrapj(jref, jmax);
+    const int j = wrapj(j0 - rot, jmax);
+    jd[h] = j;
+    const float2 tr = f.trig[j0];
+    W.ts[h] = tr.x;
+    W.tc[h] = tr.y;
+    W.dx0[h] = f.dy[j0];
+    W.dvol0[h] = f.yvol[j0] * rc;
+    W.xa0[h] = 0.0f;
+    W.rad[h] = rc;
+    W.vrot[h] = q4.z;
+    W.rb[h] = q1.x;
+    W.drb[h] = 0.0f;
+    W.pb[h] = q1.z;
+    W.dpb[h] = 0.0f;
+    W.cs2[h] = q2.x;
+    W.cs[h] = q2.y;
+    W.Lb[h] = 0.0f;
+    W.dLb[h] = 0.0f;
+    dyc[h] = f.dy[j];
+    W.r[h] = fr[j];
+    W.u[h] = fv[j];   // v_phi - v_bg
+    W.v[h] = fu[j];   // v_r
+  }
+  const float sqrt_gam = sqrtf((float)prm.gam);
+  ppmlr_pert<K, 1>(W, nullptr, nullptr, lim, dt, wres, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
+
+  float* __restrict__ dr = f.f[src ^ 1][0] + base;
+  float* __restrict__ du = f.f[src ^ 1][1] + base;
+  float* __restrict__ dv = f.f[src ^ 1][2] + base;
+  const float csnd = sqrt_gam * q2.y;
+  const float idx = 1.0f / q0.z;
+  float m = 0.0f;
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    const int c = K * lane + h;
+    if (c >= kNpad && c < lim - kNpad) {
+      const int j = jd[h];
+      dr[j] = W.r[h];
+      du[j] = W.v[h];
+      dv[j] = W.u[h];
+      // the next step's CFL term of this cell (timestep.cu:48-56)
+      const float t0 = (fabsf(W.v[h]) + csnd) * idx;
+      const float t1 = (fabsf(W.u[h] + cfl_off) + csnd) / (dyc[h] * rc);
+      m = fmaxf(m, fmaxf(t0, t1));
+    }
+  }
+  if (do_cfl) {
+    m = wave_maxf(m);
+    if (lane == 0) cflp[gw] = (double)m;
+  }
+}
+
+__device__ __forceinline__ float block_max256(float m, float* sh) {
+  m = wave_maxf(m);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  return fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
+}
+
+// CFL from the stored FP32 fields, one partial per block
+__global__ __launch_bounds__(256) void k_cfl_f32(DevGrid g, DevF32 f, DevParams prm, const StepScalars* ss,
+                                                 double* cflp, int src) {
+  __shared__ float sh[4];
+  const long ncell = (long)g.imax * g.jmax;
+  const double FrRot = ss->FrRot;
+  const float sqrt_gam = sqrtf((float)prm.gam);
+  float m = 0.0f;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(t % g.jmax);
+    const int n = (int)(t / g.jmax);
+    const int s = slot_of(g, n, 0);
+    const long a = (long)s * g.jmax + j;
+    const float4 q0 = f.tab[s * kSlotQ], q2 = f.tab[s * kSlotQ + 2];
+    const float csnd = sqrt_gam * q2.y;
+    const float cfl_off = (float)(f.vbg[s] - g.rot_v[s] - FrRot * g.radc[s]);
+    const float t0 = (fabsf(f.f[src][1][a]) + csnd) / q0.z;
+    const float t1 = (fabsf(f.f[src][2][a] + cfl_off) + csnd) / (f.dy[j] * q0.x);
+    m = fmaxf(m, fmaxf(t0, t1));
+  }
+  m = block_max256(m, sh);
+  if (threadIdx.x == 0) cflp[blockIdx.x] = (double)m;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// mass (mode 0) and the planet-force integrands (modes 8, 9) of device_func.cu:155-238 on
+// the FP32 state, in FP64 with the FP64 background added back
+__global__ __launch_bounds__(256) void k_reduce_f32(DevGrid g, DevF32 f, DevParams prm,
+                                                    const StepScalars* __restrict__ ss, int src, int mode,
+                                                    double exclude, double* partials) {
+  __shared__ double sh[4];
+  const long ncell = (long)g.imax * g.jmax;
+  const Planet pl{ss->planet.m, ss->planet.x, ss->planet.y, ss->planet.vx, ss->planet.vy, ss->planet.rs};
+  double acc = 0.0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(t % g.jmax);
+    const int n = (int)(t / g.jmax);
+    const int s = slot_of(g, n, 0);
+    int j0 = j + g.rot_j[s];
+    if (j0 >= g.jmax) j0 -= g.jmax;
+    const double r = f.rbg[s] + (double)f.f[src][0][(long)s * g.jmax + j];
+    double tmp = 0.0;
+    if (mode == 0 || prm.plnt == 0) {
+      tmp = r * g.xvol[s] * g.yvol[j0] * g.zvol[s];
+    } else if (mode == 8 || mode == 9) {
+      const double rad = g.xc[s], phi = g.yc[j0];
+      const double dd = (rad - pl.x) * (rad - pl.x) + rad * rad * (phi - pl.y) * (phi - pl.y);
+      if (dd > exclude * exclude) {
+        const double bg = pow(g.xc[s], -prm.p_alpha);
+        if (mode == 8) tmp = -grav_rad_cyl<double>(pl, rad, phi, 0.0, 0.0) * (r - bg) * g.xvol[s] * g.yvol[j0];
+        else tmp = -grav_azi_cyl<double>(pl, rad, phi, 0.0, 0.0) * (rad * (r - bg) * g.xvol[s] * g.yvol[j0]);
+      }
+    }
+    acc += tmp;
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ __launch_bounds__(256) void k_totals_f32(DevGrid g, DevF32 f, int src, double* partials) {
+  __shared__ double sh[8];
+  const long ncell = (long)g.imax * g.jmax;
+  double sm = 0.0, cm = 0.0, sl = 0.0, cl = 0.0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(t % g.jmax);
+    const int n = (int)(t / g.jmax);
+    const int s = slot_of(g, n, 0);
+    int j0 = j + g.rot_j[s];
+    if (j0 >= g.jmax) j0 -= g.jmax;
+    const long a = (long)s * g.jmax + j;
+    const double dm = (f.rbg[s] + (double)f.f[src][0][a]) * g.xvol[s] * g.yvol[j0] * g.zvol[s];
+    const double dl = dm * (f.vbg[s] + (double)f.f[src][2][a]) * g.radc[s];
+    double y = dm - cm, tt = sm + y;
+    cm = (tt - sm) - y;
+    sm = tt;
+    y = dl - cl;
+    tt = sl + y;
+    cl = (tt - sl) - y;
+    sl = tt;
+  }
+  sm = wave_sum(sm - cm);
+  sl = wave_sum(sl - cl);
+  if ((threadIdx.x & 63) == 0) {
+    sh[threadIdx.x >> 6] = sm;
+    sh[4 + (threadIdx.x >> 6)] = sl;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    partials[2 * blockIdx.x + 1] = (sh[4] + sh[5]) + (sh[6] + sh[7]);
+  }
+}
+
+int red_blocks(const DevGrid& g) {
+  long ncell = (long)g.imax * g.jmax;
+  long b = (ncell + 255) / 256;
+  if (b > 2048) b = 2048;
+  return (int)(b < 1 ? 1 : b);
+}
+
+template <int K, int WPB>
+hipError_t sweep_x_k(const LaunchCtx& c, const DevF32& f0, int src) {
+  DevF32 f = f0;
+  f.nwin_x = f32_windows(c.g.imax, K);
+  const long waves = (long)c.g.jmax * f.nwin_x;
+  dim3 grid((unsigned)((waves + WPB - 1) / WPB)), block(64 * WPB);
+  k_sweep_x_f32<K, WPB><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+  return hipGetLastError();
+}
+
+template <int K, int WPB>
+hipError_t sweep_y_k(const LaunchCtx& c, const DevF32& f0, int src, int do_cfl) {
+  DevF32 f = f0;
+  f.nwin_y = f32_windows(c.g.jmax, K);
+  const long waves = (long)f.nwin_y * c.g.imax;
+  if (do_cfl && waves > c.ncflp) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((waves + WPB - 1) / WPB)), block(64 * WPB);
+  k_sweep_y_f32<K, WPB><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int f32_windows(int len, int K) {
+  const int real = 64 * K - 2 * kNpad;
+  return (len + real - 1) / real;
+}
+
+const int kF32Variants[kNumF32Variants][2] = {{2, 2}, {2, 4}, {4, 2}, {4, 4}};
+
+int f32_cfl_partials_y(const DevGrid& g, int variant) {
+  return f32_windows(g.jmax, kF32Variants[variant][0]) * g.imax;
+}
+
+hipError_t f32_prep(const LaunchCtx& c, const DevF32& f) {
+  const int nslot = c.g.sr * c.g.sk;
+  const int n = nslot > c.g.jmax ? nslot : c.g.jmax;
+  k_prep<<<(n + 255) / 256, 256, 0, c.stream>>>(c.g, f, c.ss, nslot);
+  return hipGetLastError();
+}
+
+hipError_t f32_sweep_x(const LaunchCtx& c, const DevF32& f, int variant, int src) {
+  switch (variant) {
+    case 0: return sweep_x_k<2, 2>(c, f, src);
+    case 1: return sweep_x_k<2, 4>(c, f, src);
+    case 2: return sweep_x_k<4, 2>(c, f, src);
+    case 3: return sweep_x_k<4, 4>(c, f, src);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t f32_sweep_y(const LaunchCtx& c, const DevF32& f, int variant, int src, int do_cfl) {
+  switch (variant) {
+    case 0: return sweep_y_k<2, 2>(c, f, src, do_cfl);
+    case 1: return sweep_y_k<2, 4>(c, f, src, do_cfl);
+    case 2: return sweep_y_k<4, 2>(c, f, src, do_cfl);
+    case 3: return sweep_y_k<4, 4>(c, f, src, do_cfl);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t f32_cfl(const LaunchCtx& c, const DevF32& f, int src, int* n) {
+  const int nb = red_blocks(c.g);
+  if (nb > c.ncflp) return hipErrorInvalidValue;
+  k_cfl_f32<<<nb, 256, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src);
+  *n = nb;
+  return hipGetLastError();
+}
+
+hipError_t f32_reduce(const LaunchCtx& c, const DevF32& f, int src, int mode, double exclude, int slot) {
+  const int nb = red_blocks(c.g);
+  k_reduce_f32<<<nb, 256, 0, c.stream>>>(c.g, f, c.p, c.ss, src, mode, exclude, c.partials);
+  return launch_sum_partials(c, nb, 1, &c.ss->red[slot]);
+}
+
+hipError_t f32_totals(const LaunchCtx& c, const DevF32& f, int src, double* out2) {
+  int nb = red_blocks(c.g);
+  if (nb > kMaxPartials / 2) nb = kMaxPartials / 2;
+  k_totals_f32<<<nb, 256, 0, c.stream>>>(c.g, f, src, c.partials);
+  return launch_sum_partials(c, nb, 2, out2);
+}
+
+}  // namespace pn

@@ -12,6 +12,8 @@ struct LaunchCtx {
   DevParams p;
   StepScalars* ss;      // device scalars
   double* partials;     // reduction scratch, >= kMaxPartials doubles
+  double* cflp;         // CFL maxima per block or per wave, ncflp doubles
+  int ncflp;
   hipStream_t stream;
 };
 
@@ -25,11 +27,13 @@ hipError_t fp64_sweep_z(const LaunchCtx& c, int src);
 // common small kernels (pn_kernels_common.hip)
 hipError_t launch_index_shift(const LaunchCtx& c);
 hipError_t launch_rotate(const LaunchCtx& c);
-hipError_t launch_cfl(const LaunchCtx& c, int src);              // clears, reduces, finalises ss->dt
+hipError_t launch_cfl(const LaunchCtx& c, int src);              // partials into cflp, then ss->dt
 hipError_t launch_reduce(const LaunchCtx& c, int src, int mode, double exclude, int slot);  // -> ss->red[slot]
 hipError_t launch_totals(const LaunchCtx& c, int src, double* out2);  // out2: device, mass and angmom
 hipError_t launch_planet_half_kick(const LaunchCtx& c);
 hipError_t launch_planet_after_sweep(const LaunchCtx& c, int src);  // simtime, drift, forces, half kick
 hipError_t launch_time_advance(const LaunchCtx& c);                 // simtime += dt, ncycle++ (no planet)
+hipError_t launch_dt_finalize(const LaunchCtx& c, int n);           // dt = min(1.1 dt, courant / max of n cflp)
+hipError_t launch_sum_partials(const LaunchCtx& c, int nb, int stride, double* out);
 
 }  // namespace pn

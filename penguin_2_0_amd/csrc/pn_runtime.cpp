@@ -1,7 +1,9 @@
 // pn_runtime.cpp -- the C ABI: contexts, hydr_ring upload/download, the step operators and
-// the device-resident step loop (optionally replayed as a HIP graph).
+// the device-resident step loop (optionally replayed as a HIP graph), for the plain FP64
+// path and the bulk-FP32 path.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -10,6 +12,7 @@
 #include <vector>
 
 #include "penguin.h"
+#include "pn_f32.hpp"
 #include "pn_host.hpp"
 #include "pn_launch.hpp"
 
@@ -22,19 +25,23 @@ struct pn_ctx {
   std::string err;
   DevGrid dg{};
   DevParams dp{};
+  DevF32 df{};                      // FP32 state and tables (FP32 contexts)
+  int var_x = 3, var_y = 3;         // FP32 sweep variants (index into kF32Variants)
+  bool tuned = false;
   StepScalars* ss = nullptr;        // device
   double* partials = nullptr;       // device
+  double* cflp = nullptr;           // device, CFL partial maxima
+  int ncflp = 0;
   double* out2 = nullptr;           // device, totals
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   int cur = 0;                      // buffer holding the state
   bool loaded = false;
-  // host copies of the per-ring scalars the device never changes (for packing)
-  std::vector<double> h_x, h_dx, h_xc, h_xvol, h_z, h_dz, h_zc, h_zvol;
+  // host copies of the per-ring values the device never changes (for packing)
+  std::vector<double> h_x, h_dx, h_xc, h_xvol, h_z, h_dz, h_zc, h_zvol, h_vbg, h_rbg, h_cs2;
   std::vector<double> h_y, h_dy, h_yc, h_yvol;
   // graph of one step (2-D) or two steps (3-D, the buffers alternate)
   hipGraphExec_t gexec = nullptr;
-  int graph_steps = 0;
   // host copies of the fixed inputs, so a snapshot can be re-entered (restart)
   std::vector<char> h_lft, h_rgh, h_udr, h_top;
   std::vector<pn_symdisk> h_val;
@@ -58,6 +65,12 @@ int fail(pn_ctx* c, int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return fail((c), PN_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
 
+#define PN_LAUNCH(c, what, call)                                                          \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) return fail((c), PN_ERR_HIP, "%s: %s", what, hipGetErrorString(e_)); \
+  } while (0)
+
 template <typename T>
 int dalloc(pn_ctx* c, T** p, size_t n) {
   void* q = nullptr;
@@ -74,13 +87,21 @@ LaunchCtx lctx(pn_ctx* c) {
   L.p = c->dp;
   L.ss = c->ss;
   L.partials = c->partials;
+  L.cflp = c->cflp;
+  L.ncflp = c->ncflp;
   L.stream = c->stream;
   return L;
 }
 
 int nwin(int len) { return (len + kReal - 1) / kReal; }
 
-int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p) {
+bool is_f32(const pn_ctx* c) { return c->precision == PN_FP32; }
+
+// the next step's CFL maximum can be taken in the last sweep's epilogue when FrRot cannot
+// change inside a step, i.e. unless the planet moves under the disk's torque
+bool fused_cfl(const pn_ctx* c) { return is_f32(c) && c->P.plnt_flag < 2; }
+
+int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision) {
   if (g->imax < 1 || g->jmax < 2 * kNpad || g->kmax < 1) return fail(c, PN_ERR_ARG, "bad grid %d x %d x %d", g->imax, g->jmax, g->kmax);
   if (p->ndim != 2 && p->ndim != 3) return fail(c, PN_ERR_ARG, "ndim must be 2 or 3");
   if ((p->ndim == 2) != (g->kmax == 1)) return fail(c, PN_ERR_ARG, "ndim %d with kmax %d", p->ndim, g->kmax);
@@ -94,11 +115,14 @@ int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p) {
   if (p->fargo_flag < 0 || p->fargo_flag > 2) return fail(c, PN_ERR_ARG, "fargo_flag %d", p->fargo_flag);
   if (p->plnt_flag < 0 || p->plnt_flag > 2) return fail(c, PN_ERR_ARG, "plnt_flag %d", p->plnt_flag);
   if (p->plnt_flag == 2 && p->ndim == 3) return fail(c, PN_ERR_UNSUPPORTED, "moving planet needs the 2-D force reductions");
+  if (precision == PN_FP32 && (p->ndim != 2 || p->eos != 0))
+    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for the 2-D isothermal disk");
+  if (precision == PN_FP32 && p->visc_flag)
+    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built without viscosity");
   return PN_OK;
 }
 
-// synchronous: the source is a caller's stack record
-int upload_scalars(pn_ctx* c, const StepScalars& h) {
+int upload_scalars(pn_ctx* c, const StepScalars& h) {   // synchronous: h is a caller's stack record
   PN_HIP(c, hipMemcpyAsync(c->ss, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
   PN_HIP(c, hipStreamSynchronize(c->stream));
   return PN_OK;
@@ -115,50 +139,75 @@ int sync(pn_ctx* c) {
   return PN_OK;
 }
 
-// sweeps of one step; the state returns to c->cur in 2-D and moves to the other buffer in 3-D
-int launch_sweeps(pn_ctx* c, int& cur) {
+int launch_cfl_any(pn_ctx* c, int src) {   // partial maxima, then dt
   LaunchCtx L = lctx(c);
-  hipError_t e;
-  if (c->precision == PN_FP64) {
-    e = fp64_sweep_x(L, cur);
-    if (e != hipSuccess) return fail(c, PN_ERR_HIP, "sweep_x: %s", hipGetErrorString(e));
-    e = fp64_sweep_y(L, cur ^ 1);
-    if (e != hipSuccess) return fail(c, PN_ERR_HIP, "sweep_y: %s", hipGetErrorString(e));
+  if (is_f32(c)) {
+    int n = 0;
+    PN_LAUNCH(c, "cfl", f32_cfl(L, c->df, src, &n));
+    PN_LAUNCH(c, "dt", launch_dt_finalize(L, n));
+  } else {
+    PN_LAUNCH(c, "cfl", launch_cfl(L, src));
+  }
+  return PN_OK;
+}
+
+// sweeps of one step; the state returns to cur in 2-D and moves to the other buffer in 3-D
+int launch_sweeps(pn_ctx* c, int& cur, int do_cfl) {
+  LaunchCtx L = lctx(c);
+  if (is_f32(c)) {
+    PN_LAUNCH(c, "prep", f32_prep(L, c->df));
+    PN_LAUNCH(c, "sweep_x", f32_sweep_x(L, c->df, c->var_x, cur));
+    PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, cur ^ 1, do_cfl));
+  } else {
+    PN_LAUNCH(c, "sweep_x", fp64_sweep_x(L, cur));
+    PN_LAUNCH(c, "sweep_y", fp64_sweep_y(L, cur ^ 1));
     if (c->P.ndim == 3) {
-      e = fp64_sweep_z(L, cur);
-      if (e != hipSuccess) return fail(c, PN_ERR_HIP, "sweep_z: %s", hipGetErrorString(e));
+      PN_LAUNCH(c, "sweep_z", fp64_sweep_z(L, cur));
       cur ^= 1;
     }
-  } else {
-    return fail(c, PN_ERR_UNSUPPORTED, "precision %d not built", c->precision);
   }
-  if (c->P.fargo_flag > 0) {
-    e = launch_rotate(L);
-    if (e != hipSuccess) return fail(c, PN_ERR_HIP, "rotate: %s", hipGetErrorString(e));
-  }
+  if (c->P.fargo_flag > 0) PN_LAUNCH(c, "rotate", launch_rotate(L));
   return PN_OK;
 }
 
 // one full step of main.cu:345-387 on the device, reading and writing c->ss
 int launch_step(pn_ctx* c, int& cur) {
   LaunchCtx L = lctx(c);
-  hipError_t e = launch_cfl(L, cur);
-  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "cfl: %s", hipGetErrorString(e));
-  e = launch_index_shift(L);
-  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "index_shift: %s", hipGetErrorString(e));
+  const bool fused = fused_cfl(c);
+  int rc;
+  if (!fused && (rc = launch_cfl_any(c, cur)) != PN_OK) return rc;
+  PN_LAUNCH(c, "index_shift", launch_index_shift(L));
+  if (c->P.plnt_flag > 0) PN_LAUNCH(c, "planet kick", launch_planet_half_kick(L));
+  if ((rc = launch_sweeps(c, cur, fused ? 1 : 0)) != PN_OK) return rc;
   if (c->P.plnt_flag > 0) {
-    e = launch_planet_half_kick(L);
-    if (e != hipSuccess) return fail(c, PN_ERR_HIP, "planet kick: %s", hipGetErrorString(e));
-  }
-  int rc = launch_sweeps(c, cur);
-  if (rc != PN_OK) return rc;
-  if (c->P.plnt_flag > 0) {
-    e = launch_planet_after_sweep(L, cur);
-    if (e != hipSuccess) return fail(c, PN_ERR_HIP, "planet update: %s", hipGetErrorString(e));
+    if (is_f32(c) && c->P.plnt_flag == 2) {
+      // the moving planet's drift and the FP32 force reductions, then forces and kick
+      return fail(c, PN_ERR_UNSUPPORTED, "moving planet on the FP32 path");
+    }
+    PN_LAUNCH(c, "planet update", launch_planet_after_sweep(L, cur));
   } else {
-    e = launch_time_advance(L);
-    if (e != hipSuccess) return fail(c, PN_ERR_HIP, "time advance: %s", hipGetErrorString(e));
+    PN_LAUNCH(c, "time advance", launch_time_advance(L));
   }
+  if (fused) PN_LAUNCH(c, "dt", launch_dt_finalize(L, f32_cfl_partials_y(c->dg, c->var_y)));
+  return PN_OK;
+}
+
+// mean time of nrep back-to-back launches, measured with HIP events on the context's stream
+template <class F>
+int event_time(pn_ctx* c, int nrep, double* ms, F&& launch) {
+  hipEvent_t e0, e1;
+  PN_HIP(c, hipEventCreate(&e0));
+  PN_HIP(c, hipEventCreate(&e1));
+  hipError_t e = hipEventRecord(e0, c->stream);
+  for (int k = 0; k < nrep && e == hipSuccess; ++k) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "kernel timing: %s", hipGetErrorString(e));
+  *ms = static_cast<double>(t) / nrep;
   return PN_OK;
 }
 
@@ -172,10 +221,10 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
   if (!g || !p || !out) return PN_ERR_ARG;
   *out = nullptr;
   pn_ctx* c = new pn_ctx();
-  int rc = check_params(c, g, p);
-  if (rc == PN_OK && precision != PN_FP64 && precision != PN_FP32) rc = fail(c, PN_ERR_ARG, "precision %d", precision);
+  int rc = (precision != PN_FP64 && precision != PN_FP32) ? fail(c, PN_ERR_ARG, "precision %d", precision)
+                                                           : check_params(c, g, p, precision);
   if (rc != PN_OK) {
-    *out = c;   // caller can read the message, then pn_destroy
+    *out = c;   // the caller can read the message, then pn_destroy
     return rc;
   }
   c->g = *g;
@@ -200,33 +249,48 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
   const long nslot = static_cast<long>(d.sr) * d.sk;
   const size_t ncell = static_cast<size_t>(nslot) * g->jmax;
   int r2 = PN_OK;
-  for (int b = 0; b < 2 && r2 == PN_OK; ++b)
-    for (int f = 0; f < kNField && r2 == PN_OK; ++f) r2 = dalloc(c, &d.f[b][f], ncell);
+  if (precision == PN_FP64) {
+    for (int b = 0; b < 2 && r2 == PN_OK; ++b)
+      for (int f = 0; f < kNField && r2 == PN_OK; ++f) r2 = dalloc(c, &d.f[b][f], ncell);
+  } else {
+    for (int b = 0; b < 2 && r2 == PN_OK; ++b)
+      for (int f = 0; f < 3 && r2 == PN_OK; ++f) r2 = dalloc(c, &c->df.f[b][f], ncell);
+  }
   double *x, *dx, *xc, *xvol, *z, *dz, *zc, *zvol, *radc, *cs2, *nu, *y, *dy, *yc, *yvol;
   pn_symdisk* val;
-  if (r2 == PN_OK) r2 = dalloc(c, &x, nslot);
-  if (r2 == PN_OK) r2 = dalloc(c, &dx, nslot);
-  if (r2 == PN_OK) r2 = dalloc(c, &xc, nslot);
-  if (r2 == PN_OK) r2 = dalloc(c, &xvol, nslot);
-  if (r2 == PN_OK) r2 = dalloc(c, &z, nslot);
-  if (r2 == PN_OK) r2 = dalloc(c, &dz, nslot);
-  if (r2 == PN_OK) r2 = dalloc(c, &zc, nslot);
-  if (r2 == PN_OK) r2 = dalloc(c, &zvol, nslot);
-  if (r2 == PN_OK) r2 = dalloc(c, &radc, nslot);
-  if (r2 == PN_OK) r2 = dalloc(c, &cs2, nslot);
-  if (r2 == PN_OK) r2 = dalloc(c, &nu, nslot);
-  if (r2 == PN_OK) r2 = dalloc(c, &d.rot_v, nslot);
-  if (r2 == PN_OK) r2 = dalloc(c, &d.res_v, nslot);
+  double** slot_tabs[] = {&x, &dx, &xc, &xvol, &z, &dz, &zc, &zvol, &radc, &cs2, &nu, &d.rot_v, &d.res_v};
+  for (double** t : slot_tabs)
+    if (r2 == PN_OK) r2 = dalloc(c, t, nslot);
   if (r2 == PN_OK) r2 = dalloc(c, &d.rot_j, nslot);
   if (r2 == PN_OK) r2 = dalloc(c, &d.inc_j, nslot);
-  if (r2 == PN_OK) r2 = dalloc(c, &y, g->jmax);
-  if (r2 == PN_OK) r2 = dalloc(c, &dy, g->jmax);
-  if (r2 == PN_OK) r2 = dalloc(c, &yc, g->jmax);
-  if (r2 == PN_OK) r2 = dalloc(c, &yvol, g->jmax);
+  double** col_tabs[] = {&y, &dy, &yc, &yvol};
+  for (double** t : col_tabs)
+    if (r2 == PN_OK) r2 = dalloc(c, t, g->jmax);
   if (r2 == PN_OK) r2 = dalloc(c, &val, static_cast<size_t>(g->imax) * g->kmax);
   if (r2 == PN_OK) r2 = dalloc(c, &c->ss, 1);
   if (r2 == PN_OK) r2 = dalloc(c, &c->partials, kMaxPartials);
   if (r2 == PN_OK) r2 = dalloc(c, &c->out2, 2);
+  // CFL partials: one per block of the grid-stride CFL kernels (at most 2048) or one per
+  // azimuthal-sweep wave of the fused FP32 CFL (largest with 2 cells per lane)
+  c->ncflp = 2048;
+  if (precision == PN_FP32) c->ncflp = std::max(c->ncflp, f32_windows(g->jmax, 2) * g->imax);
+  if (r2 == PN_OK) r2 = dalloc(c, &c->cflp, c->ncflp);
+  if (precision == PN_FP32) {
+    float *fdy, *fyvol;
+    double *rbg, *vbg;
+    if (r2 == PN_OK) r2 = dalloc(c, &c->df.tab, static_cast<size_t>(nslot) * kSlotQ);
+    if (r2 == PN_OK) r2 = dalloc(c, &rbg, nslot);
+    if (r2 == PN_OK) r2 = dalloc(c, &vbg, nslot);
+    if (r2 == PN_OK) r2 = dalloc(c, &fdy, g->jmax);
+    if (r2 == PN_OK) r2 = dalloc(c, &fyvol, g->jmax);
+    if (r2 == PN_OK) r2 = dalloc(c, &c->df.trig, g->jmax);
+    if (r2 == PN_OK) {
+      c->df.rbg = rbg;
+      c->df.vbg = vbg;
+      c->df.dy = fdy;
+      c->df.yvol = fyvol;
+    }
+  }
   if (r2 != PN_OK) {
     *out = c;
     return r2;
@@ -256,7 +320,6 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
   q.has_w = (p->ndim == 3) ? 1 : 0;
   StepScalars s0{};
   rc = upload_scalars(c, s0);
-  if (rc == PN_OK) rc = sync(c);
   *out = c;
   return rc;
 }
@@ -274,7 +337,7 @@ int pn_destroy(pn_ctx* c) {
 int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* lft, const void* rgh, size_t lr_bytes,
                   const void* udr, const void* top, size_t ut_bytes, const pn_symdisk* val, size_t nval) {
   if (!c) return PN_ERR_ARG;
-  if (c->dg.f[0][0] == nullptr) return fail(c, PN_ERR_STATE, "context was not created");
+  if (c->ss == nullptr) return fail(c, PN_ERR_STATE, "context was not created");
   const int I = c->g.imax, J = c->g.jmax, K = c->g.kmax;
   const size_t rb = pn_ring_bytes(J);
   const size_t nring = static_cast<size_t>(I) * K;
@@ -289,7 +352,7 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
   std::vector<double> F[kNField];
   for (int f = 0; f < kNField; ++f) F[f].assign(static_cast<size_t>(nslot) * J, 0.0);
   std::vector<double> sx(nslot, 0.0), sdx(nslot, 1.0), sxc(nslot, 1.0), sxvol(nslot, 1.0), sz(nslot, 0.0),
-      sdz(nslot, 1.0), szc(nslot, 0.0), szvol(nslot, 1.0), srot_v(nslot, 0.0), sres_v(nslot, 0.0);
+      sdz(nslot, 1.0), szc(nslot, 0.0), szvol(nslot, 1.0), srot_v(nslot, 0.0), sres_v(nslot, 0.0), svbg(nslot, 0.0);
   std::vector<int32_t> srot_j(nslot, 0), sinc_j(nslot, 0);
   // field values are copied as bytes so that signed zeros and NaN payloads survive
   auto take = [&](const void* base, long idx, int slot, bool interior) -> int {
@@ -302,6 +365,7 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
     sx[slot] = *R.x; sdx[slot] = *R.dx; sxc[slot] = *R.xc; sxvol[slot] = *R.xvol;
     sz[slot] = *R.z; sdz[slot] = *R.dz; szc[slot] = *R.zc; szvol[slot] = *R.zvol;
     srot_v[slot] = *R.rot_v;
+    svbg[slot] = R.v[0];   // ghost rings are uniform; interior rings get val[].v below
     if (interior) {
       sres_v[slot] = *R.res_v;
       srot_j[slot] = *R.rot_j;
@@ -321,7 +385,9 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
       const long m = i + static_cast<long>(I) * k;
       RingView R = ring_view(const_cast<void*>(rings), J, m);
       if (*R.i != i || *R.k != k) return fail(c, PN_ERR_LAYOUT, "ring %ld carries (i,k) = (%d,%d), expected (%d,%d)", m, *R.i, *R.k, i, k);
-      if ((rc = take(rings, m, slot_of(d, i, k), true)) != PN_OK) return rc;
+      const int s = slot_of(d, i, k);
+      if ((rc = take(rings, m, s, true)) != PN_OK) return rc;
+      svbg[s] = val[m].v;
     }
   // ghost rings keep rot_j = 0, inc_j = 0, res_v = 0 (never rotated, never shifted)
   for (int k = 0; k < K; ++k)
@@ -344,6 +410,7 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
         has_w = 1;
         break;
       }
+  if (is_f32(c) && has_w) return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path carries no w (2-D disk with w = 0)");
   c->dp.has_w = has_w;
   // hoisted per-ring closures, evaluated on the host in FP64 exactly as the reference's
   // device code evaluates them per cell (ppm.cu:81-82, 139; disk_profile.cu:39-42, 149-152)
@@ -365,11 +432,75 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
   c->h_yvol.assign(R0.yvol, R0.yvol + J);
   c->h_x = sx; c->h_dx = sdx; c->h_xc = sxc; c->h_xvol = sxvol;
   c->h_z = sz; c->h_dz = sdz; c->h_zc = szc; c->h_zvol = szvol;
+  c->h_vbg = svbg;
+  c->h_cs2 = scs2;
 
   hipStream_t st = c->stream;
-  const size_t fb = static_cast<size_t>(nslot) * J * sizeof(double);
-  for (int b = 0; b < 2; ++b)
-    for (int f = 0; f < kNField; ++f) PN_HIP(c, hipMemcpyAsync(d.f[b][f], F[f].data(), fb, hipMemcpyHostToDevice, st));
+  const size_t cells = static_cast<size_t>(nslot) * J;
+  if (!is_f32(c)) {
+    for (int b = 0; b < 2; ++b)
+      for (int f = 0; f < kNField; ++f)
+        PN_HIP(c, hipMemcpyAsync(d.f[b][f], F[f].data(), cells * sizeof(double), hipMemcpyHostToDevice, st));
+  } else {
+    // backgrounds: the SymDisk record of an interior ring, a ghost ring's own (uniform) values
+    std::vector<double> rbg(nslot);
+    for (long s = 0; s < nslot; ++s) rbg[s] = F[FR][static_cast<size_t>(s) * J];
+    for (int k = 0; k < K; ++k)
+      for (int i = 0; i < I; ++i) rbg[slot_of(d, i, k)] = val[i + static_cast<long>(I) * k].r;
+    c->h_rbg = rbg;
+    // Sigma - Sigma_b, v_r and v_phi - v_bg, rounded once
+    std::vector<float> G[3];
+    for (int f = 0; f < 3; ++f) G[f].resize(cells);
+    for (long s = 0; s < nslot; ++s)
+      for (int j = 0; j < J; ++j) {
+        const size_t a = static_cast<size_t>(s) * J + j;
+        G[0][a] = static_cast<float>(F[FR][a] - rbg[s]);
+        G[1][a] = static_cast<float>(F[FU][a]);
+        G[2][a] = static_cast<float>(F[FV][a] - svbg[s]);
+      }
+    for (int b = 0; b < 2; ++b)
+      for (int f = 0; f < 3; ++f)
+        PN_HIP(c, hipMemcpyAsync(c->df.f[b][f], G[f].data(), cells * sizeof(float), hipMemcpyHostToDevice, st));
+    // the per-slot table, every entry evaluated in FP64 and rounded once (2-D: slots are
+    // consecutive rings, r_cyl = rad)
+    std::vector<float> tab(static_cast<size_t>(nslot) * kSlotQ * 4, 0.0f);
+    std::vector<double> pb(nslot), Lb(nslot);
+    for (long s = 0; s < nslot; ++s) {
+      pb[s] = scs2[s] * rbg[s];
+      Lb[s] = sradc[s] * svbg[s];
+    }
+    for (long s = 0; s < nslot; ++s) {
+      const bool hn = s + 1 < nslot, hp = s > 0;
+      const double rad = sxc[s], xa0 = sx[s];
+      const double V0 = hn ? 0.5 * (sx[s] + sx[s + 1]) : sx[s] + 0.5 * sdx[s];
+      const double dpbm = hp ? pb[s] - pb[s - 1] : 0.0;
+      const double abg = dpbm * V0 / (rbg[s] * sxvol[s]);
+      const double fb = -1.0 / rad / rad + Lb[s] * Lb[s] / rad / rad / rad;
+      const double Lf = hp ? 0.5 * (Lb[s] + Lb[s - 1]) : Lb[s];
+      const double ffb = -1.0 / xa0 / xa0 + Lf * Lf / xa0 / xa0 / xa0;
+      const double q[kSlotQ * 4] = {
+          rad, xa0, sdx[s], 1.0 / rad,
+          rbg[s], hn ? rbg[s + 1] - rbg[s] : 0.0, pb[s], hn ? pb[s + 1] - pb[s] : 0.0,
+          scs2[s], std::sqrt(scs2[s]), Lb[s], hn ? Lb[s + 1] - Lb[s] : 0.0,
+          fb - abg, abg, dpbm, ffb,
+          Lf, 1.0 / (xa0 * xa0 * xa0), sres_v[s] / sradc[s], sxvol[s]};
+      for (int e = 0; e < kSlotQ * 4; ++e) tab[static_cast<size_t>(s) * kSlotQ * 4 + e] = static_cast<float>(q[e]);
+    }
+    std::vector<float> t_dy(J), t_yvol(J);
+    for (int j = 0; j < J; ++j) {
+      t_dy[j] = static_cast<float>(c->h_dy[j]);
+      t_yvol[j] = static_cast<float>(c->h_yvol[j]);
+    }
+    PN_HIP(c, hipMemcpyAsync(c->df.tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    PN_HIP(c, hipMemcpyAsync(const_cast<float*>(c->df.dy), t_dy.data(), J * sizeof(float), hipMemcpyHostToDevice, st));
+    PN_HIP(c, hipMemcpyAsync(const_cast<float*>(c->df.yvol), t_yvol.data(), J * sizeof(float), hipMemcpyHostToDevice, st));
+    PN_HIP(c, hipMemcpyAsync(const_cast<double*>(c->df.rbg), c->h_rbg.data(), nslot * sizeof(double), hipMemcpyHostToDevice, st));
+    PN_HIP(c, hipMemcpyAsync(const_cast<double*>(c->df.vbg), svbg.data(), nslot * sizeof(double), hipMemcpyHostToDevice, st));
+    c->df.nwin_x = f32_windows(I, kF32Variants[c->var_x][0]);
+    c->df.nwin_y = f32_windows(J, kF32Variants[c->var_y][0]);
+    // the host vectors above must outlive the copies
+    PN_HIP(c, hipStreamSynchronize(st));
+  }
   auto up = [&](const void* dst, const void* src, size_t n) {
     return hipMemcpyAsync(const_cast<void*>(dst), src, n, hipMemcpyHostToDevice, st);
   };
@@ -422,10 +553,18 @@ int pn_store_rings(pn_ctx* c, void* rings, size_t rings_bytes) {
   if (!rings || rings_bytes != rb * I * K) return fail(c, PN_ERR_ARG, "rings: %zu bytes, expected %zu", rings_bytes, rb * I * K);
   const DevGrid& d = c->dg;
   const long nslot = static_cast<long>(d.sr) * d.sk;
+  const size_t cells = static_cast<size_t>(nslot) * J;
   std::vector<double> F[kNField];
-  for (int f = 0; f < kNField; ++f) {
-    F[f].resize(static_cast<size_t>(nslot) * J);
-    PN_HIP(c, hipMemcpyAsync(F[f].data(), d.f[c->cur][f], F[f].size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  for (int f = 0; f < kNField; ++f) F[f].assign(cells, 0.0);
+  std::vector<float> G[3];
+  if (!is_f32(c)) {
+    for (int f = 0; f < kNField; ++f)
+      PN_HIP(c, hipMemcpyAsync(F[f].data(), d.f[c->cur][f], cells * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  } else {
+    for (int f = 0; f < 3; ++f) {
+      G[f].resize(cells);
+      PN_HIP(c, hipMemcpyAsync(G[f].data(), c->df.f[c->cur][f], cells * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
   }
   std::vector<double> rot_v(nslot), res_v(nslot);
   std::vector<int32_t> rot_j(nslot), inc_j(nslot);
@@ -435,6 +574,16 @@ int pn_store_rings(pn_ctx* c, void* rings, size_t rings_bytes) {
   PN_HIP(c, hipMemcpyAsync(inc_j.data(), d.inc_j, nslot * 4, hipMemcpyDeviceToHost, c->stream));
   int rc = sync(c);
   if (rc != PN_OK) return rc;
+  if (is_f32(c))
+    for (long s = 0; s < nslot; ++s)
+      for (int j = 0; j < J; ++j) {
+        const size_t a = static_cast<size_t>(s) * J + j;
+        const double r = c->h_rbg[s] + static_cast<double>(G[0][a]);
+        F[FR][a] = r;
+        F[FP][a] = r * c->h_cs2[s];   // isothermal closure (ppm.cu:139)
+        F[FU][a] = static_cast<double>(G[1][a]);
+        F[FV][a] = c->h_vbg[s] + static_cast<double>(G[2][a]);
+      }
   std::memset(rings, 0, rb * I * K);
   for (int k = 0; k < K; ++k)
     for (int i = 0; i < I; ++i) {
@@ -472,8 +621,7 @@ int pn_get_dt(pn_ctx* c, double dt_prev, double FrRot, double* dt) {
   h.FrRot = FrRot;
   h.bad = 0;
   if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
-  hipError_t e = launch_cfl(lctx(c), c->cur);
-  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "cfl: %s", hipGetErrorString(e));
+  if ((rc = launch_cfl_any(c, c->cur)) != PN_OK) return rc;
   if ((rc = download_scalars(c, &h)) != PN_OK) return rc;
   *dt = h.dt;
   if (h.bad) return fail(c, PN_ERR_NUMERIC, "non-finite or non-positive dt %g", h.dt);
@@ -490,8 +638,7 @@ int pn_index_shift(pn_ctx* c, double dt, double FrRot) {
   h.dt = dt;
   h.FrRot = FrRot;
   if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
-  hipError_t e = launch_index_shift(lctx(c));
-  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "index_shift: %s", hipGetErrorString(e));
+  PN_LAUNCH(c, "index_shift", launch_index_shift(lctx(c)));
   return sync(c);
 }
 
@@ -507,7 +654,7 @@ int pn_sweep2d(pn_ctx* c, double dt, const pn_body* planet, double FrRot) {
   h.planet = *planet;
   if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
   int cur = c->cur;
-  if ((rc = launch_sweeps(c, cur)) != PN_OK) return rc;
+  if ((rc = launch_sweeps(c, cur, 0)) != PN_OK) return rc;
   if ((rc = sync(c)) != PN_OK) return rc;
   c->cur = cur;
   return PN_OK;
@@ -518,7 +665,6 @@ int pn_sweep_axis(pn_ctx* c, int axis, double dt, const pn_body* planet, double 
   if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
   if (!(dt > 0.0)) return fail(c, PN_ERR_ARG, "dt %g", dt);
   if (axis < 0 || axis > 2 || (axis == 2 && c->P.ndim != 3)) return fail(c, PN_ERR_ARG, "axis %d", axis);
-  if (c->precision != PN_FP64) return fail(c, PN_ERR_UNSUPPORTED, "single sweeps run on the FP64 path");
   StepScalars h{};
   int rc = download_scalars(c, &h);
   if (rc != PN_OK) return rc;
@@ -527,8 +673,14 @@ int pn_sweep_axis(pn_ctx* c, int axis, double dt, const pn_body* planet, double 
   h.planet = *planet;
   if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
   LaunchCtx L = lctx(c);
-  hipError_t e = (axis == 0) ? fp64_sweep_x(L, c->cur) : (axis == 1) ? fp64_sweep_y(L, c->cur) : fp64_sweep_z(L, c->cur);
-  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "sweep %d: %s", axis, hipGetErrorString(e));
+  if (is_f32(c)) {
+    PN_LAUNCH(c, "prep", f32_prep(L, c->df));
+    if (axis == 0) PN_LAUNCH(c, "sweep_x", f32_sweep_x(L, c->df, c->var_x, c->cur));
+    else PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, c->cur, 0));
+  } else {
+    hipError_t e = (axis == 0) ? fp64_sweep_x(L, c->cur) : (axis == 1) ? fp64_sweep_y(L, c->cur) : fp64_sweep_z(L, c->cur);
+    if (e != hipSuccess) return fail(c, PN_ERR_HIP, "sweep %d: %s", axis, hipGetErrorString(e));
+  }
   if ((rc = sync(c)) != PN_OK) return rc;
   c->cur ^= 1;
   return PN_OK;
@@ -538,36 +690,92 @@ int pn_time_kernels(pn_ctx* c, int nrep, double* ms6) {
   if (!c || !ms6 || nrep < 1) return PN_ERR_ARG;
   if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
   for (int q = 0; q < 6; ++q) ms6[q] = 0.0;
-  hipEvent_t e0, e1;
-  PN_HIP(c, hipEventCreate(&e0));
-  PN_HIP(c, hipEventCreate(&e1));
   LaunchCtx L = lctx(c);
   int rc = PN_OK;
   for (int q = 0; q < 6 && rc == PN_OK; ++q) {
     if (q == 4 && c->P.ndim != 3) continue;
     if (q == 5 && c->P.fargo_flag == 0) continue;
-    if (q >= 2 && q <= 4 && c->precision != PN_FP64) continue;
-    hipError_t e = hipEventRecord(e0, c->stream);
-    for (int k = 0; k < nrep && e == hipSuccess; ++k) {
+    rc = event_time(c, nrep, &ms6[q], [&]() -> hipError_t {
+      hipError_t e = hipSuccess;
       switch (q) {
-        case 0: e = launch_cfl(L, c->cur); break;
+        case 0:
+          if (is_f32(c)) {
+            int n = 0;
+            e = f32_cfl(L, c->df, c->cur, &n);
+            if (e == hipSuccess) e = launch_dt_finalize(L, n);
+          } else {
+            e = launch_cfl(L, c->cur);
+          }
+          break;
         case 1: e = launch_index_shift(L); break;
-        case 2: e = fp64_sweep_x(L, c->cur); c->cur ^= 1; break;
-        case 3: e = fp64_sweep_y(L, c->cur); c->cur ^= 1; break;
+        case 2:
+          e = is_f32(c) ? f32_sweep_x(L, c->df, c->var_x, c->cur) : fp64_sweep_x(L, c->cur);
+          c->cur ^= 1;
+          break;
+        case 3:
+          e = is_f32(c) ? f32_sweep_y(L, c->df, c->var_y, c->cur, 1) : fp64_sweep_y(L, c->cur);
+          c->cur ^= 1;
+          break;
         case 4: e = fp64_sweep_z(L, c->cur); c->cur ^= 1; break;
         case 5: e = launch_rotate(L); break;
       }
-    }
-    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    float ms = 0.0f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (e != hipSuccess) rc = fail(c, PN_ERR_HIP, "timing kernel %d: %s", q, hipGetErrorString(e));
-    ms6[q] = static_cast<double>(ms) / nrep;
+      return e;
+    });
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
   return rc;
+}
+
+int pn_tune(pn_ctx* c, int nrep, int* var_x, int* var_y) {
+  if (!c || nrep < 1) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  if (!is_f32(c)) {
+    if (var_x) *var_x = 0;
+    if (var_y) *var_y = 0;
+    return PN_OK;
+  }
+  LaunchCtx L = lctx(c);
+  PN_LAUNCH(c, "prep", f32_prep(L, c->df));
+  // every candidate reads the current buffer and writes the other one, so the state is
+  // left as it was
+  double best_x = 1e30, best_y = 1e30;
+  int bx = c->var_x, by = c->var_y;
+  for (int v = 0; v < kNumF32Variants; ++v) {
+    double tx = 0.0, ty = 0.0;
+    int rc = event_time(c, nrep, &tx, [&]() { return f32_sweep_x(L, c->df, v, c->cur); });
+    if (rc == PN_OK) rc = event_time(c, nrep, &ty, [&]() { return f32_sweep_y(L, c->df, v, c->cur, 0); });
+    if (rc != PN_OK) return rc;
+    if (tx < best_x) { best_x = tx; bx = v; }
+    if (ty < best_y) { best_y = ty; by = v; }
+  }
+  c->var_x = bx;
+  c->var_y = by;
+  c->df.nwin_x = f32_windows(c->g.imax, kF32Variants[bx][0]);
+  c->df.nwin_y = f32_windows(c->g.jmax, kF32Variants[by][0]);
+  c->tuned = true;
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  if (var_x) *var_x = bx;
+  if (var_y) *var_y = by;
+  return PN_OK;
+}
+
+int pn_set_variants(pn_ctx* c, int var_x, int var_y) {
+  if (!c) return PN_ERR_ARG;
+  if (!is_f32(c)) return (var_x == 0 && var_y == 0) ? PN_OK : fail(c, PN_ERR_ARG, "the FP64 path has one variant");
+  if (var_x < 0 || var_x >= kNumF32Variants || var_y < 0 || var_y >= kNumF32Variants)
+    return fail(c, PN_ERR_ARG, "variants (%d, %d) outside [0, %d)", var_x, var_y, kNumF32Variants);
+  c->var_x = var_x;
+  c->var_y = var_y;
+  c->df.nwin_x = f32_windows(c->g.imax, kF32Variants[var_x][0]);
+  c->df.nwin_y = f32_windows(c->g.jmax, kF32Variants[var_y][0]);
+  c->tuned = true;
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  return PN_OK;
 }
 
 int pn_reduce(pn_ctx* c, const pn_body* planet, double exclude, int mode, double* out) {
@@ -579,8 +787,8 @@ int pn_reduce(pn_ctx* c, const pn_body* planet, double exclude, int mode, double
   if (rc != PN_OK) return rc;
   h.planet = *planet;
   if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
-  hipError_t e = launch_reduce(lctx(c), c->cur, mode, exclude * c->P.sc_h, 3);
-  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "reduce: %s", hipGetErrorString(e));
+  if (is_f32(c)) PN_LAUNCH(c, "reduce", f32_reduce(lctx(c), c->df, c->cur, mode, exclude * c->P.sc_h, 3));
+  else PN_LAUNCH(c, "reduce", launch_reduce(lctx(c), c->cur, mode, exclude * c->P.sc_h, 3));
   if ((rc = download_scalars(c, &h)) != PN_OK) return rc;
   *out = h.red[3];
   return PN_OK;
@@ -589,8 +797,8 @@ int pn_reduce(pn_ctx* c, const pn_body* planet, double exclude, int mode, double
 int pn_totals(pn_ctx* c, double* mass, double* angmom) {
   if (!c || !mass || !angmom) return PN_ERR_ARG;
   if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
-  hipError_t e = launch_totals(lctx(c), c->cur, c->out2);
-  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "totals: %s", hipGetErrorString(e));
+  if (is_f32(c)) PN_LAUNCH(c, "totals", f32_totals(lctx(c), c->df, c->cur, c->out2));
+  else PN_LAUNCH(c, "totals", launch_totals(lctx(c), c->cur, c->out2));
   double h[2];
   PN_HIP(c, hipMemcpyAsync(h, c->out2, sizeof h, hipMemcpyDeviceToHost, c->stream));
   int rc = sync(c);
@@ -635,14 +843,18 @@ int pn_run_begin(pn_ctx* c, pn_run_state* st) {
 int pn_run(pn_ctx* c, int nsteps, int graph, pn_run_state* st) {
   if (!c || !st || nsteps < 0) return PN_ERR_ARG;
   if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  int rc;
+  if (is_f32(c) && !c->tuned && (rc = pn_tune(c, 3, nullptr, nullptr)) != PN_OK) return rc;
   StepScalars h{};
   h.dt = st->dt;
   h.FrRot = st->FrRot;
   h.simtime = st->simtime;
   h.planet = st->planet;
   h.ncycle = st->ncycle;
-  int rc = upload_scalars(c, h);
-  if (rc != PN_OK) return rc;
+  if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
+  const bool fused = fused_cfl(c);
+  // with the CFL taken in the sweep epilogue, dt of the first step comes from the state
+  if (fused && nsteps > 0 && (rc = launch_cfl_any(c, c->cur)) != PN_OK) return rc;
   const int per_graph = (c->P.ndim == 3) ? 2 : 1;
   int done = 0;
   if (graph && nsteps >= per_graph) {
@@ -650,6 +862,7 @@ int pn_run(pn_ctx* c, int nsteps, int graph, pn_run_state* st) {
       hipGraph_t gr;
       PN_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       int cur = c->cur;
+      rc = PN_OK;
       for (int s = 0; s < per_graph && rc == PN_OK; ++s) rc = launch_step(c, cur);
       hipError_t e = hipStreamEndCapture(c->stream, &gr);
       if (rc != PN_OK) return rc;
@@ -658,7 +871,6 @@ int pn_run(pn_ctx* c, int nsteps, int graph, pn_run_state* st) {
       e = hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0);
       hipGraphDestroy(gr);
       if (e != hipSuccess) return fail(c, PN_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-      c->graph_steps = per_graph;
     }
     while (nsteps - done >= per_graph) {
       PN_HIP(c, hipGraphLaunch(c->gexec, c->stream));
@@ -670,7 +882,8 @@ int pn_run(pn_ctx* c, int nsteps, int graph, pn_run_state* st) {
     if ((rc = launch_step(c, cur)) != PN_OK) return rc;
   c->cur = cur;
   if ((rc = download_scalars(c, &h)) != PN_OK) return rc;
-  st->dt = h.dt;
+  // fused: the record already holds the next step's dt; report the last one used
+  st->dt = (fused && nsteps > 0) ? h.dt_prev : h.dt;
   st->FrRot = h.FrRot;
   st->simtime = h.simtime;
   st->planet = h.planet;

@@ -1,0 +1,481 @@
+// ppm_wave_f32.hpp -- the PPMLR window update in perturbation form, FP32 bulk (2-D
+// isothermal disk).
+//
+// Method and stage order are those of ppm_wave.hpp (ppmlr.cu:2-56); one wavefront owns a
+// window, K cells per lane.  The arithmetic is arranged so that FP32 keeps the accuracy
+// that matters for a disk whose state is a smooth background plus small perturbations:
+//   * density, pressure and angular momentum are carried as deviations from per-ring
+//     backgrounds (rho_b, p_b = c_s^2 rho_b, L_b = r v_bg).  A parabola is built in each
+//     cell's own frame: neighbour differences are (deviation difference) + (background
+//     difference, an FP64 value rounded once), so no FP32 number holds a large value next
+//     to a small change of it;
+//   * the Riemann solve works on relative jumps eps = p / p_b(left cell) - 1 with accurate
+//     log1p / expm1, and returns p* as such a relative jump;
+//   * the radial momentum balance -- star gravity, centrifugal l^2/r^3 and the pressure
+//     jump across the cell, which nearly cancel -- is split into its background residual
+//     (evaluated once per ring in FP64) and terms that are each small, so every FP32
+//     rounding is relative to a small quantity;
+//   * coordinates in offset form (edge displacements dlt from the Eulerian edges);
+//   * mass and angular momentum remapped in offset form, the background part of each flux
+//     difference taken exactly (compensated flux differencing);
+//   * the planet's force from the half-angle form R^2 = (r-r_p)^2 + 4 r r_p sin^2(dphi/2),
+//     half-angle sin / cos from a per-column FP64 table rotated by short Taylor series.
+// The azimuthal sweep advects v_phi - v_bg (an exact Galilean shift of the sweep) and moves
+// its grid at the per-ring residual rate (v_bg - res_v)/r - FrRot; that uniform part of the
+// grid motion is kept apart from the velocity-dependent part so the cell volumes and the
+// density fluxes never difference it.
+#pragma once
+
+#include "pn_f32.hpp"
+#include "ppm_wave.hpp"
+
+namespace pn {
+
+struct PlanetF {
+  float m, x, rs2, inv_x2, vy;
+};
+
+// sin / cos of (A + e)/2 from sin / cos of A/2 and a small e (|e| < ~0.5)
+__device__ __forceinline__ void rot_half(float s, float c, float e, float& so, float& co) {
+  const float h = 0.5f * e, h2 = h * h;
+  const float sh = h * (1.0f - h2 * (1.0f / 6.0f) * (1.0f - h2 * (1.0f / 20.0f) * (1.0f - h2 * (1.0f / 42.0f))));
+  const float ch = 1.0f - h2 * 0.5f * (1.0f - h2 * (1.0f / 12.0f) * (1.0f - h2 * (1.0f / 30.0f)));
+  so = s * ch + c * sh;
+  co = c * ch - s * sh;
+}
+
+// planet gravity with the indirect term (forces.cu:112-159) from the half angle to the planet
+__device__ __forceinline__ float planet_fr(const PlanetF& p, float rad, float s, float c) {
+  const float s2 = s * s;
+  const float cosd = 1.0f - 2.0f * s2;
+  const float dr = rad - p.x;
+  const float R2 = dr * dr + 4.0f * rad * p.x * s2 + p.rs2;
+  const float ri = __frsqrt_rn(R2);
+  return -p.m * (dr + 2.0f * p.x * s2) * (ri * ri * ri) - p.m * cosd * p.inv_x2;
+}
+__device__ __forceinline__ float planet_fphi(const PlanetF& p, float rad, float s, float c) {
+  const float s2 = s * s;
+  const float sind = 2.0f * s * c;
+  const float dr = rad - p.x;
+  const float R2 = dr * dr + 4.0f * rad * p.x * s2 + p.rs2;
+  const float ri = __frsqrt_rn(R2);
+  return -p.m * p.x * sind * (ri * ri * ri) + p.m * sind * p.inv_x2;
+}
+
+// log(1 + x): 2 atanh(x / (2 + x)) as an odd series for |x| < 1/4, else the library log
+__device__ __forceinline__ float log1p_acc(float x) {
+  if (fabsf(x) < 0.25f) {
+    const float z = x / (2.0f + x), z2 = z * z;
+    return 2.0f * z * (1.0f + z2 * (1.0f / 3.0f + z2 * (1.0f / 5.0f + z2 * (1.0f / 7.0f + z2 * (1.0f / 9.0f)))));
+  }
+  return __logf(1.0f + x);
+}
+// exp(y) - 1: Taylor series to y^7 for |y| < 1/4, else the library exp
+__device__ __forceinline__ float expm1_acc(float y) {
+  if (fabsf(y) < 0.25f)
+    return y * (1.0f + y * (0.5f + y * (1.0f / 6.0f + y * (1.0f / 24.0f + y * (1.0f / 120.0f + y * (1.0f / 720.0f + y * (1.0f / 5040.0f)))))));
+  return __expf(y) - 1.0f;
+}
+
+// parabola of X = X_b + x in each cell's frame (parabola.cu:168-231): val = x (the cell's
+// deviation), dbg = X_b(c+1) - X_b(c).  Outputs are relative to the cell's own X_b.  With
+// dbg == 0 this is the plain parabola of ppm_wave.hpp.
+template <int K>
+__device__ __forceinline__ Parab<float, K> parabola_off(int nmin, int nmaxp, const C<float, K>& flat,
+                                                        const C<float, K>& val, const C<float, K>& dbg,
+                                                        const Para<float, K>& P) {
+  C<float, K> vm = prev1(val), vp = next1(val), dbm = prev1(dbg);
+  C<float, K> slope, dar;
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    const float a = val[h];
+    const float da_r = (vp[h] - a) + dbg[h];
+    const float da_l = (a - vm[h]) + dbm[h];
+    float t1 = P.c3[h] * da_r + P.c4[h] * da_l;
+    if (t1 < 0.0f) t1 = -min3(-t1, 2.0f * fabsf(da_l), 2.0f * fabsf(da_r));
+    else t1 = min3(t1, 2.0f * fabsf(da_l), 2.0f * fabsf(da_r));
+    if (da_l * da_r < 0.0f) t1 = 0.0f;
+    slope[h] = t1;
+    dar[h] = da_r;
+  }
+  C<float, K> slp = next1(slope);
+  C<float, K> ar;
+#pragma unroll
+  for (int h = 0; h < K; ++h) ar[h] = val[h] + P.c0[h] * dar[h] + P.c1[h] * slp[h] + P.c2[h] * slope[h];
+  C<float, K> alv = prev1(ar);
+  Parab<float, K> o;
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    const int c = cell_of(K, h);
+    const float a = val[h];
+    float t1 = flat[h] * a;
+    float t2 = 1.0f - flat[h];
+    float arr = t1 + t2 * ar[h];
+    float la = t1 + t2 * (alv[h] - dbm[h]);   // the left neighbour's right edge, in this frame
+    t1 = arr - la;
+    t2 = t1 * t1;
+    t1 = t1 * (6.0f * (a - 0.5f * (arr + la)));
+    if ((arr - a) * (a - la) <= 0.0f) {
+      arr = a;
+      la = a;
+    }
+    if (t2 < t1) la = 3.0f * a - 2.0f * arr;
+    if (t2 < -t1) arr = 3.0f * a - 2.0f * la;
+    const bool in = (c >= nmin + 2 && c < nmaxp - 2);
+    o.da[h] = in ? arr - la : 0.0f;
+    o.a6[h] = in ? 6.0f * (a - 0.5f * (arr + la)) : 0.0f;
+    o.al[h] = in ? la : 0.0f;
+  }
+  return o;
+}
+
+template <int K>
+struct WinP {
+  // carried state: deviations from the cell's backgrounds
+  C<float, K> r;   // rho - rho_b
+  C<float, K> u;   // sweep-normal velocity (radial: v_r; azimuthal: v_phi - v_bg)
+  C<float, K> v;   // radial: l - L_b (l = r v_phi); azimuthal: v_r
+  // backgrounds (azimuthal sweep: uniform in the window, differences zero)
+  C<float, K> rb, drb, pb, dpb, Lb, dLb, cs, cs2;
+  // geometry
+  C<float, K> rad, xa0, dx0, dvol0, vrot, ts, tc;
+};
+
+// AX 0 radial (cylindrical), AX 1 azimuthal.  tab / sl: the per-slot table and the slot of
+// each cell (radial sweep only; the radial balance terms are read where they are used).
+// wres: the azimuthal grid's uniform angular rate.  On return cells [6, nmax-6) hold the
+// update.
+template <int K, int AX>
+__device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const float4* __restrict__ tab, const int* sl, int nmax,
+                                           float dt, float wres, const PlanetF& pl, bool planet, bool flat_on,
+                                           float sqrt_gam) {
+  const float hdt = 0.5f * dt;
+  const float fourthd = 1.3333333333333333f;
+  const float smallr = 1.0e-15f;
+
+  // ---- flatten (flatten.cu:7-40), on absolute pressure (relative-jump thresholds)
+  C<float, K> flat = splat<float, K>(0.0f);
+  if (flat_on) {
+    C<float, K> pabs;
+#pragma unroll
+    for (int h = 0; h < K; ++h) pabs[h] = W.pb[h] + W.r[h] * W.cs2[h];
+    C<float, K> pm1 = prev1(pabs), pp1 = next1(pabs), pm2 = prev2(pabs), pp2 = next2(pabs);
+    C<float, K> um1 = prev1(W.u), up1 = next1(W.u);
+    C<float, K> steep;
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      const int c = cell_of(K, h);
+      float s = 0.0f;
+      if (c >= 2 && c < nmax - 2) {
+        const float dp1 = pp1[h] - pm1[h];
+        float dp2 = pp2[h] - pm2[h];
+        if (fabsf(dp2) < 1.0e-15f) dp2 = 1.0e-15f;
+        const float ddp = fabsf(dp1) / fminf(pp1[h], pm1[h]);
+        bool shock = ddp > 0.33f;
+        if (um1[h] < up1[h]) shock = false;
+        if (shock) s = fmaxf(0.0f, (dp1 / dp2 - 0.75f) * 5.0f);
+      }
+      steep[h] = s;
+    }
+    C<float, K> sm = prev1(steep), sp = next1(steep);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      const int c = cell_of(K, h);
+      flat[h] = (c >= 2 && c < nmax - 2) ? fminf(0.5f, max3(sm[h], steep[h], sp[h])) : 0.0f;
+    }
+  }
+
+  // ---- states (states.cu:3-73)
+  Para<float, K> P0 = paraset(0, nmax, W.dx0);
+  C<float, K> fhalf;   // hdt * force at the left edge
+  if constexpr (AX == 0) {
+    C<float, K> vm = prev1(W.v);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      const float4 q3 = tab[sl[h] * kSlotQ + 3], q4 = tab[sl[h] * kSlotQ + 4];
+      // l at the edge = L_f + dlf; f = f_face + dlf (2 L_f + dlf) / xa0^3 (+ planet)
+      const float dlf = 0.5f * (W.v[h] + vm[h]);
+      float f = q3.w + dlf * (2.0f * q4.x + dlf) * q4.y;
+      if (planet) {
+        float s, c;
+        rot_half(W.ts[h], W.tc[h], hdt * (W.vrot[h] - pl.vy), s, c);
+        f += planet_fr(pl, W.xa0[h], s, c);
+      }
+      fhalf[h] = hdt * f;
+    }
+  } else {
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      float f = 0.0f;
+      if (planet) {
+        float s, c;
+        rot_half(W.ts[h], W.tc[h], -0.5f * W.dx0[h] + hdt * (W.vrot[h] - pl.vy), s, c);
+        f = planet_fphi(pl, W.rad[h], s, c);
+      }
+      fhalf[h] = hdt * f;
+    }
+  }
+  C<float, K> csn, csfac;
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    // sqrt(gam p / rho) = sqrt(gam) c_s for the isothermal closure
+    float s = hdt * sqrt_gam * W.cs[h] / W.dx0[h];
+    if constexpr (AX == 1) s = s / W.rad[h];
+    csn[h] = s;
+    csfac[h] = 1.0f - fourthd * s;
+  }
+  // edge averages of the traced parabolae (states.cu:45-68), relative to the cell's frame
+  auto edges = [&](const Parab<float, K>& q, C<float, K>& re, C<float, K>& le) {
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      re[h] = q.al[h] + q.da[h] - csn[h] * (q.da[h] - csfac[h] * q.a6[h]);
+      le[h] = q.al[h] + csn[h] * (q.da[h] + csfac[h] * q.a6[h]);
+    }
+  };
+  // interface c: left state = right edge of cell c-1, right state = left edge of cell c, as
+  // relative jumps against the backgrounds of cell c-1
+  C<float, K> epl, epr, erl, err, ul, ur;
+  {
+    C<float, K> pp, re, le;
+#pragma unroll
+    for (int h = 0; h < K; ++h) pp[h] = W.r[h] * W.cs2[h];
+    edges(parabola_off(0, nmax, flat, pp, W.dpb, P0), re, le);
+    C<float, K> q;
+#pragma unroll
+    for (int h = 0; h < K; ++h) q[h] = re[h] / W.pb[h];
+    C<float, K> qm = prev1(q), pbm = prev1(W.pb), dpbm = prev1(W.dpb);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      epl[h] = qm[h];
+      epr[h] = (le[h] + dpbm[h]) / pbm[h];
+    }
+  }
+  {
+    C<float, K> re, le;
+    edges(parabola_off(0, nmax, flat, W.r, W.drb, P0), re, le);
+    C<float, K> q;
+#pragma unroll
+    for (int h = 0; h < K; ++h) q[h] = re[h] / W.rb[h];
+    C<float, K> qm = prev1(q), rbm = prev1(W.rb), drbm = prev1(W.drb);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      erl[h] = qm[h];
+      err[h] = (le[h] + drbm[h]) / rbm[h];
+    }
+  }
+  {
+    C<float, K> re, le;
+    edges(parabola(0, nmax, flat, W.u, P0), re, le);
+    C<float, K> rem = prev1(re);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      ul[h] = rem[h] + fhalf[h];
+      ur[h] = le[h] + fhalf[h];
+    }
+  }
+
+  // ---- riemann, isothermal two-rarefaction closed form (riemann.cu:119-126):
+  //   t1 = (sqrt(pl/rl) + sqrt(pr/rr))/2, t2 = -(ur-ul)/(2 t1), p* = sqrt(pl pr) e^t2,
+  //   u* = ul - t1 log(p*/pl); here with pl = p_b(1+epl), ..., p* = p_b(1+em)
+  C<float, K> umid, em;
+  {
+    C<float, K> csm = prev1(W.cs);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      const float lpl = log1p_acc(epl[h]), lpr = log1p_acc(epr[h]);
+      const float lrl = log1p_acc(erl[h]), lrr = log1p_acc(err[h]);
+      const float t1 = 0.5f * csm[h] * (__expf(0.5f * (lpl - lrl)) + __expf(0.5f * (lpr - lrr)));
+      const float t2 = -0.5f * (ur[h] - ul[h]) / t1;
+      em[h] = expm1_acc(0.5f * (lpl + lpr) + t2);
+      umid[h] = ul[h] - t1 * (0.5f * (lpr - lpl) + t2);
+    }
+  }
+
+  // ---- evolve (evolve.cu:3-166): edges as displacements from the Eulerian edges.  The
+  // reference moves edges [3, nmax-2) and cells [3, nmax-3).
+  C<float, K> smv, dlt;
+  const float Wd = (AX == 1) ? dt * wres : 0.0f;
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    const int c = cell_of(K, h);
+    const bool mv = (c >= 3 && c < nmax - 2);
+    float s;
+    if constexpr (AX == 0) s = dt * umid[h];
+    else s = dt * umid[h] / W.rad[h];
+    smv[h] = mv ? s : 0.0f;
+    dlt[h] = mv ? s + Wd : 0.0f;
+  }
+  C<float, K> Dl, dltx, sx;
+  {
+    C<float, K> sp = next1(smv);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      const int c = cell_of(K, h);
+      const bool in = (c >= 3 && c < nmax - 3);
+      Dl[h] = in ? sp[h] - smv[h] : 0.0f;   // dxl - dx0, free of the uniform motion
+      dltx[h] = in ? dlt[h] : 0.0f;          // the left edge of a moved cell
+      sx[h] = in ? smv[h] : 0.0f;
+    }
+  }
+  C<float, K> rL;   // Lagrangian density deviation
+  {
+    C<float, K> emp = next1(em);
+    if constexpr (AX == 0) {
+      C<float, K> pbm = prev1(W.pb), dltxp = next1(dltx);
+#pragma unroll
+      for (int h = 0; h < K; ++h) {
+        const int c = cell_of(K, h);
+        const bool in = (c >= 3 && c < nmax - 3);
+        const float4 q0 = tab[sl[h] * kSlotQ + 0], q3 = tab[sl[h] * kSlotQ + 3];
+        const float ixc = q0.w;
+        const float ixc3 = ixc * ixc * ixc;
+        const float rho = W.rb[h] + W.r[h];
+        const float dl = W.v[h];
+        // force at the Eulerian centre (f1) and at the moved centre (f2) less the
+        // background's: gravity -1/r^2 and centrifugal l^2/r^3 with l = L_b + dl
+        const float q1 = dl * (2.0f * W.Lb[h] + dl) * ixc3;
+        const float dxc = dltx[h] + 0.5f * Dl[h];             // moved centre - centre
+        const float e = dxc * ixc;
+        const float g2 = -e * (2.0f - e * (3.0f - 4.0f * e));    // (1+e)^-2 - 1
+        const float g3 = -e * (3.0f - e * (6.0f - 10.0f * e));   // (1+e)^-3 - 1
+        const float B = W.Lb[h] * W.Lb[h] * ixc3;
+        float fdiff = -(ixc * ixc) * g2 + (B + q1) * g3;
+        if (planet) {
+          fdiff += planet_fr(pl, W.rad[h], W.ts[h], W.tc[h]);
+          float s, cc;
+          rot_half(W.ts[h], W.tc[h], dt * (W.vrot[h] - pl.vy), s, cc);
+          fdiff += planet_fr(pl, W.rad[h] + dxc, s, cc);
+        }
+        // pressure: (p*(c+1) - p*(c)) area / (rho dvol0), area = V0 + dV, and
+        //   p*(c+1) - p*(c) = dp_bm + dp,  dp = p_b(c) em(c+1) - p_b(c-1) em(c)
+        const float dp = W.pb[h] * emp[h] - pbm[h] * em[h];
+        const float dV = 0.25f * (dltx[h] + dltxp[h]);
+        const float acc = q3.x + q1 + 0.5f * fdiff + q3.y * (W.r[h] / rho) -
+                          (q3.z * dV + dp * (W.rad[h] + dV)) / (rho * W.dvol0[h]);
+        // Lagrangian volume change, offset form of dvol = dxl (xa + dxl/2)
+        const float d = dltx[h], x0 = W.xa0[h], w0 = W.dx0[h];
+        const float ddv = w0 * (d + 0.5f * Dl[h]) + Dl[h] * (x0 + d + 0.5f * w0 + 0.5f * Dl[h]);
+        const float dvol = W.dvol0[h] + ddv;
+        rL[h] = in ? (W.r[h] * W.dvol0[h] - W.rb[h] * ddv) / dvol : W.r[h];
+        if (in) W.u[h] = W.u[h] + dt * acc;
+      }
+    } else {
+#pragma unroll
+      for (int h = 0; h < K; ++h) {
+        const int c = cell_of(K, h);
+        const bool in = (c >= 3 && c < nmax - 3);
+        float f = 0.0f;
+        if (planet) {
+          f = planet_fphi(pl, W.rad[h], W.ts[h], W.tc[h]);
+          const float off = dltx[h] + 0.5f * Dl[h] + dt * (W.vrot[h] - pl.vy);
+          float s, cc;
+          rot_half(W.ts[h], W.tc[h], off, s, cc);
+          f += planet_fphi(pl, W.rad[h], s, cc);
+        }
+        const float rho = W.rb[h] + W.r[h];
+        const float dpm = W.pb[h] * (emp[h] - em[h]);   // p_b is uniform along the ring
+        const float un = W.u[h] + (0.5f * dt * f - dt * dpm / (rho * W.dvol0[h]));
+        const float ddv = Dl[h] * W.rad[h];
+        const float dvol = W.dvol0[h] + ddv;
+        rL[h] = in ? (W.r[h] * W.dvol0[h] - W.rb[h] * ddv) / dvol : W.r[h];
+        if (in) W.u[h] = un;
+      }
+    }
+  }
+
+  // ---- remap (remap.cu:182-394)
+  C<float, K> dxl;
+#pragma unroll
+  for (int h = 0; h < K; ++h) dxl[h] = W.dx0[h] + Dl[h];
+  Para<float, K> P1 = paraset(3, nmax - 3, dxl);
+  C<float, K> dxf, dxf2, lvol, delC;
+  C<float, K> dxm = prev1(dxl);
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    const float d = dltx[h];
+    if constexpr (AX == 0) lvol[h] = d * (W.xa0[h] + 0.5f * d);
+    else lvol[h] = d * W.rad[h];
+    delC[h] = (d >= 0.0f) ? 1.0f : 0.0f;
+    float e;
+    if (d >= 0.0f) {
+      e = 0.5f * d / dxm[h];
+      dxf2[h] = 1.0f - fourthd * e;
+    } else {
+      e = 0.5f * d / dxl[h];
+      dxf2[h] = 1.0f + fourthd * e;
+    }
+    dxf[h] = e;
+  }
+  // average over the swept region of the left edge (remap.cu:242-258), relative to the
+  // cell's own background (dbg: background difference to the next cell)
+  auto face = [&](const Parab<float, K>& Q, const C<float, K>& dbg, C<float, K>& out) {
+    C<float, K> alm = prev1(Q.al), dam = prev1(Q.da), a6m = prev1(Q.a6), dbm = prev1(dbg);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      if (delC[h] != 0.0f) out[h] = (alm[h] + dam[h] - dxf[h] * (dam[h] - dxf2[h] * a6m[h])) - dbm[h];
+      else out[h] = Q.al[h] - dxf[h] * (Q.da[h] + dxf2[h] * Q.a6[h]);
+    }
+  };
+  const C<float, K> zero = splat<float, K>(0.0f);
+  C<float, K> mass, invm, fr;
+  {
+    Parab<float, K> Q = parabola_off(3, nmax - 3, flat, rL, W.drb, P1);
+    C<float, K> fv, frd;
+    face(Q, W.drb, fv);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      if constexpr (AX == 0) {
+        frd[h] = (W.rb[h] + fv[h]) * lvol[h];
+        fr[h] = frd[h];
+      } else {
+        // rho_b times the uniform displacement is the same at every edge: it carries no
+        // mass difference, so the density update leaves it out
+        frd[h] = W.rb[h] * (W.rad[h] * sx[h]) + fv[h] * lvol[h];
+        fr[h] = frd[h] + W.rb[h] * (W.rad[h] * (dltx[h] - sx[h]));
+      }
+    }
+    C<float, K> frp = next1(frd);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      const int c = cell_of(K, h);
+      const float rn = W.r[h] + (frd[h] - frp[h]) / W.dvol0[h];
+      mass[h] = (W.rb[h] + W.r[h]) * W.dvol0[h];
+      invm[h] = 1.0f / (fmaxf(W.rb[h] + rn, smallr) * W.dvol0[h]);
+      W.r[h] = (c >= 6 && c < nmax - 6) ? rn : rL[h];
+    }
+  }
+  // X <- (X m + F_left - F_right) / m_new
+  auto remap_plain = [&](C<float, K>& X) {
+    Parab<float, K> Q = parabola(3, nmax - 3, flat, X, P1);
+    C<float, K> fl;
+    face(Q, zero, fl);
+#pragma unroll
+    for (int h = 0; h < K; ++h) fl[h] = fl[h] * fr[h];
+    C<float, K> flp = next1(fl);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      const int c = cell_of(K, h);
+      if (c >= 6 && c < nmax - 6) X[h] = (X[h] * mass[h] + (fl[h] - flp[h])) * invm[h];
+    }
+  };
+  remap_plain(W.u);
+  if constexpr (AX == 0) {
+    // l - L_b: fluxes relative to the cell's own L_b; the right edge's flux comes from the
+    // next cell's frame, L_b(c+1) = L_b(c) + dL_b
+    Parab<float, K> Q = parabola_off(3, nmax - 3, flat, W.v, W.dLb, P1);
+    C<float, K> fl;
+    face(Q, W.dLb, fl);
+#pragma unroll
+    for (int h = 0; h < K; ++h) fl[h] = fl[h] * fr[h];
+    C<float, K> flp = next1(fl), frp = next1(fr);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      const int c = cell_of(K, h);
+      if (c >= 6 && c < nmax - 6) W.v[h] = (W.v[h] * mass[h] + (fl[h] - (flp[h] + W.dLb[h] * frp[h]))) * invm[h];
+    }
+  } else {
+    remap_plain(W.v);
+  }
+}
+
+}  // namespace pn
