@@ -1,14 +1,22 @@
 """Throughput of the hydro step on one MI355X, as one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--precision fp64|fp32]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--precision fp32|fp64]
 
 A step is main.cu:345-387 -- CFL dt, FARGO split, planet half kick, r sweep, phi sweep,
 integer phi roll, planet update -- advanced on the device with dt and the planet kept
 there (a captured HIP graph per step).  value = zone-updates/s = cells * K / wall time of
-the K timed steps, with the state resident in HBM.  The roofline entry prices the dominant
-sweep kernel, timed with HIP events on the context's stream, against HBM bandwidth.  The
-CPU baseline times the FP64 oracle on the host cores on a bounded prefix of the same
-workload.
+the K timed steps of the measured path, with the state resident in HBM.
+
+With the default FP32 path the plain FP64 path is run beside it on the same initial state
+for the same W + K steps: its throughput gives the speedup, and its final state is the
+reference for the relative L1 / Linf errors of Sigma, v_r and v_phi (compared in the lab
+frame) and for the mass and angular-momentum drift.
+
+The roofline entry prices the dominant sweep kernel of the measured path -- its algorithmic
+bytes per launch (one read and one write of the 3 state fields of every cell) over its
+mean launch time, measured with HIP events on the context's stream -- against HBM
+bandwidth.  The CPU baseline times the FP64 oracle on the host cores on a bounded prefix of
+the same workload.
 """
 from __future__ import annotations
 
@@ -18,6 +26,8 @@ import os
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -48,14 +58,44 @@ def cpu_baseline(w: configs.Workload, budget_s: float = 12.0) -> dict:
                       f"{n} steps in {dt:.2f} s"}
 
 
+def timed_run(w: configs.Workload, prec: int, steps: int, warmup: int) -> dict:
+    disk = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    ctx = abi.Context(w.grid, w.params, prec)
+    ctx.load(disk)
+    st = ctx.run_begin()
+    m0, l0 = ctx.totals()
+    ctx.run(warmup, st, graph=True)
+    t0 = time.perf_counter()
+    ctx.run(steps, st, graph=True)   # returns after the stream has drained
+    wall = time.perf_counter() - t0
+    m1, l1 = ctx.totals()
+    state = ctx.store()
+    km = ctx.time_kernels(5)         # after the snapshot: timing advances the state
+    ctx.close()
+    return {"wall": wall, "value": w.cells * steps / wall, "state": state, "st": st, "km": km,
+            "dmass": (m1 - m0) / m0, "dangmom": (l1 - l0) / l0}
+
+
+def lab(state: np.ndarray, f: str) -> np.ndarray:
+    """Field f with each ring's data rolled to lab columns (data j sits at j + rot_j)."""
+    a = state[f]
+    return np.stack([np.roll(a[i], int(state["rot_j"][i])) for i in range(a.shape[0])])
+
+
+def rel_err(a: np.ndarray, b: np.ndarray) -> tuple[float, float]:
+    d = np.abs(a - b)
+    return float(d.sum() / np.abs(b).sum()), float(d.max() / np.abs(b).max())
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-ref", action="store_true", help="skip the FP64 leg of an FP32 run")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported (the disk is not "
@@ -65,19 +105,11 @@ def main(argv=None) -> int:
 
     w = getattr(configs, a.config)()
     prec = abi.PN_FP64 if a.precision == "fp64" else abi.PN_FP32
-    disk = abi.init_disk(w.grid, w.params, w.seed, w.noise)
-    ctx = abi.Context(w.grid, w.params, prec)
-    ctx.load(disk)
-    st = ctx.run_begin()
-    ctx.run(a.warmup, st, graph=True)
-    t0 = time.perf_counter()
-    ctx.run(a.steps, st, graph=True)   # returns after the stream has drained
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    value = w.cells * a.steps / wall
+    run = timed_run(w, prec, a.steps, a.warmup)
+    value = run["value"]
+    km = run["km"]
 
     # dominant kernel, timed on the context's stream
-    km = ctx.time_kernels(5)
     dom = max(("sweep_x", "sweep_y"), key=lambda k: km[k])
     wbytes = 8 if prec == abi.PN_FP64 else 4
     fields = 3   # Sigma, v_r, v_phi: the minimal 2-D isothermal state
@@ -91,17 +123,31 @@ def main(argv=None) -> int:
             traffic = None
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "kernel": dom, "kernel_ms": km[dom], "kernels_ms": km}
+                "kernel": dom, "kernel_ms": km[dom], "bytes_per_launch": bytes_per_launch, "kernels_ms": km}
     out = {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if prec == abi.PN_FP64 else "f32",
+        "warmup": a.warmup, "ms_per_step": run["wall"] / a.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == abi.PN_FP64 else "f32",
         "data": "synthetic (seeded disk initial condition of the workload)",
-        "config": {"workload": w.name, "cells": w.cells, "path": "plain FP64" if prec == abi.PN_FP64 else "FP32 bulk",
-                   "graph": True},
+        "config": {"workload": w.name, "cells": w.cells,
+                   "path": "plain FP64" if prec == abi.PN_FP64 else "FP32 bulk (single disk)", "graph": True},
         "roofline": roofline,
+        "conservation": {"mass_drift": run["dmass"], "angmom_drift": run["dangmom"]},
     }
-    ctx.close()
+    if prec == abi.PN_FP32 and not a.no_ref:
+        ref = timed_run(w, abi.PN_FP64, a.steps, a.warmup)
+        errs = {f: rel_err(lab(run["state"], f), lab(ref["state"], f)) for f in ("r", "u", "v")}
+        out["fp64_plain"] = {"value": ref["value"], "ms_per_step": ref["wall"] / a.steps * 1e3,
+                             "speedup": value / ref["value"], "kernels_ms": ref["km"],
+                             "mass_drift": ref["dmass"], "angmom_drift": ref["dangmom"]}
+        out["accuracy_vs_fp64"] = {
+            "after_steps": a.warmup + a.steps,
+            "l1_rho": errs["r"][0], "linf_rho": errs["r"][1],
+            "l1_vr": errs["u"][0], "linf_vr": errs["u"][1],
+            "l1_vphi": errs["v"][0], "linf_vphi": errs["v"][1],
+            "rot_j_equal": bool(np.array_equal(run["state"]["rot_j"], ref["state"]["rot_j"])),
+            "dt_rel_diff": abs(run["st"].dt - ref["st"].dt) / ref["st"].dt,
+        }
     if rank == 0 and not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
     if rank == 0:

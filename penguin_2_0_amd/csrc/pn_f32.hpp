@@ -5,23 +5,34 @@
 
 namespace pn {
 
-// per-slot constants of the FP32 sweeps: five float4 per slot (FP64 values rounded once)
-//   q0  rad (cell centre), xa0 (left edge), dx0 (width), 1/rad
-//   q1  rho_b, rho_b(s+1) - rho_b(s), p_b, p_b(s+1) - p_b(s)
-//   q2  c_s^2, c_s, L_b, L_b(s+1) - L_b(s)
-//   q3  res_bg, a_bg, dp_bm = p_b(s) - p_b(s-1), f_face
-//   q4  L_f = (L_b(s) + L_b(s-1))/2, 1/xa0^3, vrot = res_v / r (rewritten each step), dvol0
+// per-slot constants of the FP32 sweeps, entry-major: entry e of slot s at tab[e*stride + s]
+// (FP64 values rounded once; rho_b is Sigma_b, p_b = c_s^2 Sigma_b, L_b = r v_bg):
+//   rad (cell centre), xa0 (left edge), dx0 (width), 1/rad,
+//   rho_b, rho_b(s+1) - rho_b(s), p_b, p_b(s+1) - p_b(s),
+//   c_s^2, c_s, L_b, L_b(s+1) - L_b(s),
+//   res_bg, a_bg, dp_bm = p_b(s) - p_b(s-1), f_face,
+//   L_f = (L_b(s) + L_b(s-1))/2, 1/xa0^3, vrot = res_v / r (rewritten each step), dvol0
 // with, in FP64: a_bg = dp_bm V0 / (rho_b dvol0), V0 = (xa0(s) + xa0(s+1))/2, the pressure
 // acceleration of the background; res_bg = -1/rad^2 + L_b^2/rad^3 - a_bg, the background's
 // net radial acceleration; f_face = -1/xa0^2 + L_f^2/xa0^3, the force at the left edge.
-// rho_b is Sigma_b, p_b = c_s^2 Sigma_b, L_b = r v_bg.
-constexpr int kSlotQ = 5;
+// The stride leaves room past the last slot for a radial window's overhang (copies of the
+// last slot, as a clamped read would give).
+enum TabEntry {
+  kTabRad, kTabXa0, kTabDx0, kTabIxc,
+  kTabRb, kTabDrb, kTabPb, kTabDpb,
+  kTabCs2, kTabCs, kTabLb, kTabDLb,
+  kTabResBg, kTabAbg, kTabDpbm, kTabFface,
+  kTabLf, kTabIxa3, kTabVrot, kTabDvol0,
+  kTabEntries
+};
+constexpr int kTabOverhang = 256;   // >= the longest radial window
 
 // FP32 state as deviations from per-slot backgrounds, and the per-slot constants.
 // Fields: 0 Sigma - Sigma_b(slot), 1 v_r, 2 v_phi - v_bg(slot).
 struct DevF32 {
   float* f[2][3];
-  float4* tab;                 // kSlotQ (= 5) float4 per slot, layout in ppm_wave_f32.hpp
+  float* tab;                  // kTabEntries x tstride, entry-major
+  int tstride;                 // nslot + kTabOverhang, a multiple of 4
   const double *rbg, *vbg;     // Sigma_b and v_bg per slot, FP64
   const float *dy, *yvol;      // per column
   float2* trig;                // per column, per step: sin / cos of half the angle to the planet
@@ -29,7 +40,11 @@ struct DevF32 {
 };
 
 constexpr int kNumF32Variants = 4;
-extern const int kF32Variants[kNumF32Variants][2];   // (cells per lane K, waves per workgroup)
+// radial sweep: (cells per lane K, waves per workgroup W, columns per wave Q); a workgroup
+// stages a window of W*Q columns through LDS
+extern const int kF32VariantsX[kNumF32Variants][3];
+// azimuthal sweep: (cells per lane K, waves per workgroup W)
+extern const int kF32VariantsY[kNumF32Variants][2];
 
 int f32_windows(int len, int K);
 // per-step tables: planet trig per column, vrot per slot (after Index_Shift)

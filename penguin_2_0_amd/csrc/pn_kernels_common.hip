@@ -181,19 +181,35 @@ __global__ __launch_bounds__(256) void k_totals(DevGrid g, DevParams prm, int sr
   }
 }
 
-// serial Neumaier sum of nb partials (stride st) into out
-__global__ void k_sum_partials(const double* partials, int nb, int st, double* out) {
-  if (threadIdx.x != 0) return;
+__device__ __forceinline__ void neumaier_add(double& s, double& c, double x) {
+  const double t = s + x;
+  if (fabs(s) >= fabs(x)) c += (s - t) + x;
+  else c += (x - t) + s;
+  s = t;
+}
+
+constexpr int kSumThreads = 256;
+
+// Neumaier sum of nb partials (stride st) into out, in a fixed order: thread t sums
+// partials t, t + 256, ...; thread 0 then adds the 256 running sums and their
+// compensations in thread order
+__global__ __launch_bounds__(kSumThreads) void k_sum_partials(const double* partials, int nb, int st, double* out) {
+  __shared__ double sh_s[kSumThreads], sh_c[kSumThreads];
   for (int q = 0; q < st; ++q) {
     double s = 0.0, c = 0.0;
-    for (int b = 0; b < nb; ++b) {
-      const double x = partials[(long)b * st + q];
-      const double t = s + x;
-      if (fabs(s) >= fabs(x)) c += (s - t) + x;
-      else c += (x - t) + s;
-      s = t;
+    for (int b = threadIdx.x; b < nb; b += kSumThreads) neumaier_add(s, c, partials[(long)b * st + q]);
+    sh_s[threadIdx.x] = s;
+    sh_c[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double S = 0.0, Cc = 0.0;
+      for (int t = 0; t < kSumThreads; ++t) {
+        neumaier_add(S, Cc, sh_s[t]);
+        Cc += sh_c[t];
+      }
+      out[q] = S + Cc;
     }
-    out[q] = s + c;
+    __syncthreads();
   }
 }
 
@@ -202,17 +218,6 @@ __device__ double cs2_of(const DevParams& prm, double r) { return prm.sc_h * prm
 __device__ double set_M_p(const DevParams& prm, double t) {   // planet.cu:28-34
   if (t < prm.t_growth) return fmax(prm.M_p * sin(kPi * t / (2.0 * prm.t_growth)), 0.01 * kEarthMass);
   return prm.M_p;
-}
-
-// kick of kick_drift_kick (planet.cu:80-90) with dt = dt_step / 2
-__global__ void k_planet_half_kick(DevParams prm, StepScalars* ss) {
-  if (threadIdx.x != 0) return;
-  const double dt = 0.5 * ss->dt;
-  pn_body& b = ss->planet;
-  b.vx += dt * b.fx;
-  b.vy += ss->FrRot + dt * b.fy / b.x / b.x;
-  ss->FrRot = (prm.frrot == 1) ? b.vy : 0.0;
-  b.vy -= ss->FrRot;
 }
 
 // after the sweeps: time advance, drift, forces and the closing half kick
@@ -255,10 +260,134 @@ __global__ void k_planet_forces_kick(DevParams prm, StepScalars* ss, int do_kick
   }
 }
 
-__global__ void k_time_advance(StepScalars* ss) {
+constexpr int kStepThreads = 1024;
+
+// max over n partials by the whole (single) workgroup; the result is valid in thread 0
+__device__ double block_max_partials(const double* __restrict__ cflp, int n, double* sh) {
+  double m = 0.0;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) m = fmax(m, cflp[t]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, sh[w]);
+  return m;
+}
+
+// thread 0: dt = min(1.1 dt_prev, courant / max)  (timestep.cu:194, 224-230)
+__device__ void finalize_dt(const DevParams& prm, StepScalars* ss, double mx) {
+  ss->cfl_max = mx;
+  double dt = fmin(1.1 * ss->dt, prm.courant / mx);
+  if (!(dt > 0.0) || isinf(dt)) ss->bad = 1;
+  ss->dt_prev = ss->dt;
+  ss->dt = dt;
+}
+
+// Index_Shift of one ring (orbital_advection.cu:12-43)
+__device__ void index_shift_ring(DevGrid& g, const DevParams& prm, int n, double dt, double FrRot) {
+  const int i = n % g.imax, k = n / g.imax;
+  const int s = slot_of(g, i, k);
+  const double xc = g.xc[s];
+  const double widthy = xc * g.dy[0] / dt;
+  double rot_v;
+  if (prm.fargo == 1) rot_v = g.val[i + g.imax * k].v - FrRot * xc;
+  else if (prm.fargo == 2) rot_v = (1.0 - FrRot) * xc;
+  else rot_v = 0.0;
+  const double jp = g.rot_v[s] / widthy;
+  if (jp >= 0.0) g.inc_j[s] = (int)(jp + 0.5);
+  else g.inc_j[s] = (int)(jp - 0.5);
+  g.res_v[s] = floor(jp + 0.5) * widthy;
+  g.rot_v[s] = rot_v;
+}
+
+// Everything of a step before the sweeps, in one workgroup (main.cu:345-370): dt from ncfl
+// CFL partials (ncfl 0: ss->dt is already this step's), Index_Shift of every ring, and the
+// planet's opening half kick.  For FP32 contexts also the per-step tables: sin / cos of half
+// the angle from the planet to each column centre (trig) and vrot = res_v / r_cyl per slot
+// (vrot_f[s * vrot_stride]).
+__global__ __launch_bounds__(kStepThreads) void k_pre_step(DevGrid g, DevParams prm, StepScalars* ss,
+                                                           const double* __restrict__ cflp, int ncfl, float2* trig,
+                                                           float* vrot_f, int vrot_stride) {
+  __shared__ double sh[kStepThreads / 64];
+  __shared__ double s_dt, s_fr;
+  if (ncfl > 0) {
+    const double mx = block_max_partials(cflp, ncfl, sh);
+    if (threadIdx.x == 0) finalize_dt(prm, ss, mx);
+  }
+  if (threadIdx.x == 0) {
+    s_dt = ss->dt;
+    s_fr = ss->FrRot;
+  }
+  __syncthreads();
+  const double dt = s_dt, FrRot = s_fr;
+  const int nring = g.imax * g.kmax;
+  for (int n = threadIdx.x; n < nring; n += blockDim.x) {
+    index_shift_ring(g, prm, n, dt, FrRot);
+    if (vrot_f) {
+      const int s = slot_of(g, n % g.imax, n / g.imax);
+      vrot_f[(long)s * vrot_stride] = (float)(g.res_v[s] / g.radc[s]);
+    }
+  }
+  if (trig) {
+    const double py = ss->planet.y;   // the half kick changes velocities only
+    for (int j = threadIdx.x; j < g.jmax; j += blockDim.x) {
+      const double a = 0.5 * (g.yc[j] - py);
+      trig[j] = make_float2((float)sin(a), (float)cos(a));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && prm.plnt > 0) {
+    const double hdt = 0.5 * dt;
+    pn_body& b = ss->planet;
+    b.vx += hdt * b.fx;
+    b.vy += FrRot + hdt * b.fy / b.x / b.x;
+    ss->FrRot = (prm.frrot == 1) ? b.vy : 0.0;
+    b.vy -= ss->FrRot;
+  }
+}
+
+// Everything of a step after the sweeps for a planet that does not feel the disk
+// (plnt_flag < 2), in one workgroup: the integer roll of every ring (ppm.cu:309-322), time
+// advance, drift, forces and closing half kick (main.cu:378-387, planet.cu:54-93), then,
+// with ncfl > 0, the next step's dt from the CFL partials the last sweep wrote.
+__global__ __launch_bounds__(kStepThreads) void k_post_step(DevGrid g, DevParams prm, StepScalars* ss,
+                                                            const double* __restrict__ cflp, int ncfl) {
+  __shared__ double sh[kStepThreads / 64];
+  if (prm.fargo > 0) {
+    const int nring = g.imax * g.kmax;
+    for (int n = threadIdx.x; n < nring; n += blockDim.x) {
+      const int s = slot_of(g, n % g.imax, n / g.imax);
+      int j = g.rot_j[s] + g.inc_j[s];
+      if (j < 0) j += g.jmax;
+      if (j >= g.jmax) j -= g.jmax;
+      g.rot_j[s] = j;
+    }
+  }
+  double mx = 0.0;
+  if (ncfl > 0) mx = block_max_partials(cflp, ncfl, sh);
   if (threadIdx.x != 0) return;
-  ss->simtime += ss->dt;
+  const double dt = ss->dt;
+  ss->simtime += dt;
   ss->ncycle += 1;
+  if (prm.plnt > 0) {
+    pn_body& b = ss->planet;
+    const double old_rad = b.x;
+    b.x += dt * b.vx;
+    b.y += dt * b.vy;
+    b.vy = ((b.vy + ss->FrRot) * old_rad * old_rad / b.x / b.x) - ss->FrRot;
+    if (b.y < 0.0) b.y += kTwoPi;
+    if (b.y >= kTwoPi) b.y -= kTwoPi;
+    if (b.x != old_rad) b.rs = prm.rs_fac * sqrt(cs2_of(prm, b.x) * b.x * b.x * b.x);
+    b.m = set_M_p(prm, ss->simtime);
+    b.fx = -1.0 / b.x / b.x + b.x * ((b.vy + ss->FrRot) * (b.vy + ss->FrRot));
+    b.fy = 0.0;
+    const double hdt = 0.5 * dt;
+    b.vx += hdt * b.fx;
+    b.vy += ss->FrRot + hdt * b.fy / b.x / b.x;
+    ss->FrRot = (prm.frrot == 1) ? b.vy : 0.0;
+    b.vy -= ss->FrRot;
+  }
+  if (ncfl > 0) finalize_dt(prm, ss, mx);
 }
 
 int red_blocks(const DevGrid& g) {
@@ -294,7 +423,7 @@ hipError_t launch_cfl(const LaunchCtx& c, int src) {
 hipError_t launch_reduce(const LaunchCtx& c, int src, int mode, double exclude, int slot) {
   int nb = red_blocks(c.g);
   k_reduce<<<nb, 256, 0, c.stream>>>(c.g, c.p, c.ss, src, mode, exclude, c.partials);
-  k_sum_partials<<<1, 64, 0, c.stream>>>(c.partials, nb, 1, &c.ss->red[slot]);
+  k_sum_partials<<<1, kSumThreads, 0, c.stream>>>(c.partials, nb, 1, &c.ss->red[slot]);
   return hipGetLastError();
 }
 
@@ -302,7 +431,27 @@ hipError_t launch_totals(const LaunchCtx& c, int src, double* out2) {
   int nb = red_blocks(c.g);
   if (nb > kMaxPartials / 2) nb = kMaxPartials / 2;
   k_totals<<<nb, 256, 0, c.stream>>>(c.g, c.p, src, c.partials);
-  k_sum_partials<<<1, 64, 0, c.stream>>>(c.partials, nb, 2, out2);
+  k_sum_partials<<<1, kSumThreads, 0, c.stream>>>(c.partials, nb, 2, out2);
+  return hipGetLastError();
+}
+
+hipError_t launch_pre_step(const LaunchCtx& c, int ncfl, float2* trig, float* vrot_f, int vrot_stride) {
+  if (ncfl < 0 || ncfl > c.ncflp) return hipErrorInvalidValue;
+  k_pre_step<<<1, kStepThreads, 0, c.stream>>>(c.g, c.p, c.ss, c.cflp, ncfl, trig, vrot_f, vrot_stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_post_step(const LaunchCtx& c, int ncfl) {
+  if (ncfl < 0 || ncfl > c.ncflp || c.p.plnt == 2) return hipErrorInvalidValue;
+  k_post_step<<<1, kStepThreads, 0, c.stream>>>(c.g, c.p, c.ss, c.cflp, ncfl);
+  return hipGetLastError();
+}
+
+hipError_t launch_cfl_partials(const LaunchCtx& c, int src, int* n) {
+  const int nb = red_blocks(c.g);
+  if (nb > c.ncflp) return hipErrorInvalidValue;
+  k_cfl<<<nb, 256, 0, c.stream>>>(c.g, c.p, c.ss, c.cflp, src);
+  *n = nb;
   return hipGetLastError();
 }
 
@@ -313,17 +462,7 @@ hipError_t launch_dt_finalize(const LaunchCtx& c, int n) {
 }
 
 hipError_t launch_sum_partials(const LaunchCtx& c, int nb, int stride, double* out) {
-  k_sum_partials<<<1, 64, 0, c.stream>>>(c.partials, nb, stride, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_time_advance(const LaunchCtx& c) {
-  k_time_advance<<<1, 64, 0, c.stream>>>(c.ss);
-  return hipGetLastError();
-}
-
-hipError_t launch_planet_half_kick(const LaunchCtx& c) {
-  k_planet_half_kick<<<1, 64, 0, c.stream>>>(c.p, c.ss);
+  k_sum_partials<<<1, kSumThreads, 0, c.stream>>>(c.partials, nb, stride, out);
   return hipGetLastError();
 }
 

@@ -59,74 +59,126 @@ __global__ void k_prep(DevGrid g, DevF32 f, const StepScalars* __restrict__ ss, 
     const double a = 0.5 * (g.yc[t] - ss->planet.y);
     f.trig[t] = make_float2((float)sin(a), (float)cos(a));
   }
-  if (t < nslot) reinterpret_cast<float*>(f.tab)[t * 4 * kSlotQ + 18] = (float)(g.res_v[t] / g.radc[t]);
+  if (t < nslot) f.tab[(long)kTabVrot * f.tstride + t] = (float)(g.res_v[t] / g.radc[t]);
+}
+
+// K consecutive floats from LDS (vector read; p is K-float aligned)
+template <int K>
+__device__ __forceinline__ C<float, K> lds_k(const float* p) {
+  C<float, K> o;
+  if constexpr (K == 2) {
+    const float2 v = *reinterpret_cast<const float2*>(p);
+    o[0] = v.x;
+    o[1] = v.y;
+  } else if constexpr (K == 4) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+  } else {
+#pragma unroll
+    for (int h = 0; h < K; ++h) o[h] = p[h];
+  }
+  return o;
 }
 
 // ---------------------------------------------------------------------------------------
-template <int K, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_sweep_x_f32(DevGrid g, DevF32 f, DevParams prm,
-                                                          const StepScalars* __restrict__ ss, int src) {
-  constexpr int REAL = 64 * K - 2 * kNpad;
-  const int lane = threadIdx.x & 63;
-  const int gw = xcd_swizzle(blockIdx.x, gridDim.x) * WPB + (threadIdx.x >> 6);
+// Radial sweep.  A workgroup of W waves takes one radial window (64 K rings with halos) and
+// NC = W Q consecutive lab columns.  The window's NC columns of the 3 fields are staged in
+// LDS with loads that are contiguous along phi (each ring's NC cells are adjacent in HBM);
+// every wave then runs Q columns as radial pencils out of LDS, writes its results back in
+// place, and the workgroup stores the window's interior rings contiguously again.
+template <int K, int W, int Q>
+__global__ __launch_bounds__(64 * W) void k_sweep_x_f32(DevGrid g, DevF32 f, DevParams prm,
+                                                        const StepScalars* __restrict__ ss, int src) {
+  constexpr int WIN = 64 * K, REAL = WIN - 2 * kNpad, NC = W * Q, LD = WIN + 2;
+  __shared__ float tile[3][NC][LD];   // [field][column][cell]; LD even keeps K-float reads aligned
+  __shared__ int srot[WIN];
   const int jmax = g.jmax;
-  const int j0 = gw % jmax;
-  const int l = gw / jmax;
-  if (l >= f.nwin_x) return;   // wave-uniform; no barriers in this kernel
+  const int ngrp = (jmax + NC - 1) / NC;
+  const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int grp = bid % ngrp, l = bid / ngrp;
+  if (l >= f.nwin_x) return;   // workgroup-uniform, before any barrier
+  const int jb = grp * NC;
+  const int lim = win_len_k<K>(g.imax, f.nwin_x, l);
+  const int s0 = slot_of(g, REAL * l - kNpad, 0);   // slot of window cell 0
+  for (int c = threadIdx.x; c < WIN; c += 64 * W) srot[c] = g.rot_j[s0 + (c < lim ? c : lim - 1)];
+  __syncthreads();
+  {
+    const float* __restrict__ fr = f.f[src][0];
+    const float* __restrict__ fu = f.f[src][1];
+    const float* __restrict__ fv = f.f[src][2];
+    for (int e = threadIdx.x; e < WIN * NC; e += 64 * W) {
+      const int q = e % NC, c = e / NC;
+      const int jl = jb + q;
+      if (jl < jmax) {
+        const long a = (long)(s0 + (c < lim ? c : lim - 1)) * jmax + wrapj(jl - srot[c], jmax);
+        tile[0][q][c] = fr[a];
+        tile[1][q][c] = fu[a];
+        tile[2][q][c] = fv[a];
+      }
+    }
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const float dt = (float)ss->dt;
   const PlanetF pl = planet_f(ss);
-  const int lim = win_len_k<K>(g.imax, f.nwin_x, l);
-  const float* __restrict__ fr = f.f[src][0];
-  const float* __restrict__ fu = f.f[src][1];
-  const float* __restrict__ fv = f.f[src][2];
-  const float4* __restrict__ tab = f.tab;
-  const float2 tr = f.trig[j0];
-
-  WinP<K> W;
-  int sl[K];
-  long addr[K];
+  const float sqrt_gam = sqrtf((float)prm.gam);
+  const TabLane<K> tx{f.tab, f.tstride, s0 + K * lane};   // overhang cells read copies of the last slot
+  for (int qq = 0; qq < Q; ++qq) {
+    const int q = wv * Q + qq;
+    const int j0 = jb + q;
+    if (j0 >= jmax) break;   // wave-uniform
+    WinP<K> Wp;
+    Wp.rad = tx.get(kTabRad);
+    Wp.xa0 = tx.get(kTabXa0);
+    Wp.dx0 = tx.get(kTabDx0);
+    Wp.dvol0 = tx.get(kTabDvol0);
+    Wp.rb = tx.get(kTabRb);
+    Wp.drb = tx.get(kTabDrb);
+    Wp.pb = tx.get(kTabPb);
+    Wp.dpb = tx.get(kTabDpb);
+    Wp.cs2 = tx.get(kTabCs2);
+    Wp.cs = tx.get(kTabCs);
+    Wp.Lb = tx.get(kTabLb);
+    Wp.dLb = tx.get(kTabDLb);
+    Wp.vrot = tx.get(kTabVrot);
+    const float2 tr = f.trig[j0];
+    Wp.ts = splat<float, K>(tr.x);
+    Wp.tc = splat<float, K>(tr.y);
+    Wp.r = lds_k<K>(&tile[0][q][K * lane]);
+    Wp.u = lds_k<K>(&tile[1][q][K * lane]);
+    const C<float, K> vp = lds_k<K>(&tile[2][q][K * lane]);
 #pragma unroll
-  for (int h = 0; h < K; ++h) {
-    const int c = K * lane + h;
-    const int i_ring = REAL * l - kNpad + (c < lim ? c : lim - 1);
-    const int s = slot_of(g, i_ring, 0);
-    sl[h] = s;
-    const int j = wrapj(j0 - g.rot_j[s], jmax);
-    const long a = (long)s * jmax + j;
-    addr[h] = a;
-    const float4 q0 = tab[s * kSlotQ], q1 = tab[s * kSlotQ + 1], q2 = tab[s * kSlotQ + 2], q4 = tab[s * kSlotQ + 4];
-    W.rad[h] = q0.x;
-    W.xa0[h] = q0.y;
-    W.dx0[h] = q0.z;
-    W.rb[h] = q1.x;
-    W.drb[h] = q1.y;
-    W.pb[h] = q1.z;
-    W.dpb[h] = q1.w;
-    W.cs2[h] = q2.x;
-    W.cs[h] = q2.y;
-    W.Lb[h] = q2.z;
-    W.dLb[h] = q2.w;
-    W.vrot[h] = q4.z;
-    W.dvol0[h] = q4.w;
-    W.ts[h] = tr.x;
-    W.tc[h] = tr.y;
-    W.r[h] = fr[a];
-    W.u[h] = fu[a];
-    W.v[h] = q0.x * fv[a];   // l - L_b = r (v_phi - v_bg); r_cyl = r in 2-D
+    for (int h = 0; h < K; ++h) Wp.v[h] = Wp.rad[h] * vp[h];   // l - L_b = r (v_phi - v_bg); r_cyl = r in 2-D
+    ppmlr_pert<K, 0>(Wp, tx, lim, dt, 0.0f, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
+    const C<float, K> ixc = tx.get(kTabIxc);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      const int c = K * lane + h;
+      if (c >= kNpad && c < lim - kNpad) {   // only this wave touches column q
+        tile[0][q][c] = Wp.r[h];
+        tile[1][q][c] = Wp.u[h];
+        tile[2][q][c] = Wp.v[h] * ixc[h];
+      }
+    }
   }
-  ppmlr_pert<K, 0>(W, tab, sl, lim, dt, 0.0f, pl, prm.plnt > 0, prm.flat != 0, sqrtf((float)prm.gam));
-
-  float* __restrict__ dr = f.f[src ^ 1][0];
-  float* __restrict__ du = f.f[src ^ 1][1];
-  float* __restrict__ dv = f.f[src ^ 1][2];
-#pragma unroll
-  for (int h = 0; h < K; ++h) {
-    const int c = K * lane + h;
-    if (c >= kNpad && c < lim - kNpad) {
-      const long a = addr[h];
-      dr[a] = W.r[h];
-      du[a] = W.u[h];
-      dv[a] = W.v[h] * tab[sl[h] * kSlotQ].w;
+  __syncthreads();
+  {
+    float* __restrict__ dr = f.f[src ^ 1][0];
+    float* __restrict__ du = f.f[src ^ 1][1];
+    float* __restrict__ dv = f.f[src ^ 1][2];
+    for (int e = threadIdx.x; e < WIN * NC; e += 64 * W) {
+      const int q = e % NC, c = e / NC;
+      const int jl = jb + q;
+      if (jl < jmax && c >= kNpad && c < lim - kNpad) {
+        const long a = (long)(s0 + c) * jmax + wrapj(jl - srot[c], jmax);
+        dr[a] = tile[0][q][c];
+        du[a] = tile[1][q][c];
+        dv[a] = tile[2][q][c];
+      }
     }
   }
 }
@@ -155,9 +207,10 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, D
   // sees: v_bg - rot_v - FrRot r (timestep.cu:62)
   const float wres = (float)((vbg - g.res_v[s]) / rcd - FrRot);
   const float cfl_off = (float)(vbg - g.rot_v[s] - FrRot * rcd);
-  const float4 q0 = f.tab[s * kSlotQ], q1 = f.tab[s * kSlotQ + 1], q2 = f.tab[s * kSlotQ + 2],
-               q4 = f.tab[s * kSlotQ + 4];
-  const float rc = q0.x;
+  const float* __restrict__ T = f.tab + s;
+  const long ts = f.tstride;
+  const float rc = T[kTabRad * ts], dxr = T[kTabDx0 * ts], rb = T[kTabRb * ts], pb = T[kTabPb * ts],
+              cs2 = T[kTabCs2 * ts], cs = T[kTabCs * ts], vrot = T[kTabVrot * ts];
   const long base = (long)s * jmax;
   const float* __restrict__ fr = f.f[src][0] + base;
   const float* __restrict__ fu = f.f[src][1] + base;
@@ -180,13 +233,13 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, D
     W.dvol0[h] = f.yvol[j0] * rc;
     W.xa0[h] = 0.0f;
     W.rad[h] = rc;
-    W.vrot[h] = q4.z;
-    W.rb[h] = q1.x;
+    W.vrot[h] = vrot;
+    W.rb[h] = rb;
     W.drb[h] = 0.0f;
-    W.pb[h] = q1.z;
+    W.pb[h] = pb;
     W.dpb[h] = 0.0f;
-    W.cs2[h] = q2.x;
-    W.cs[h] = q2.y;
+    W.cs2[h] = cs2;
+    W.cs[h] = cs;
     W.Lb[h] = 0.0f;
     W.dLb[h] = 0.0f;
     dyc[h] = f.dy[j];
@@ -195,13 +248,13 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, D
     W.v[h] = fu[j];   // v_r
   }
   const float sqrt_gam = sqrtf((float)prm.gam);
-  ppmlr_pert<K, 1>(W, nullptr, nullptr, lim, dt, wres, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
+  ppmlr_pert<K, 1>(W, TabLane<K>{nullptr, 0, 0}, lim, dt, wres, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
 
   float* __restrict__ dr = f.f[src ^ 1][0] + base;
   float* __restrict__ du = f.f[src ^ 1][1] + base;
   float* __restrict__ dv = f.f[src ^ 1][2] + base;
-  const float csnd = sqrt_gam * q2.y;
-  const float idx = 1.0f / q0.z;
+  const float csnd = sqrt_gam * cs;
+  const float idx = 1.0f / dxr;
   float m = 0.0f;
 #pragma unroll
   for (int h = 0; h < K; ++h) {
@@ -243,11 +296,12 @@ __global__ __launch_bounds__(256) void k_cfl_f32(DevGrid g, DevF32 f, DevParams 
     const int n = (int)(t / g.jmax);
     const int s = slot_of(g, n, 0);
     const long a = (long)s * g.jmax + j;
-    const float4 q0 = f.tab[s * kSlotQ], q2 = f.tab[s * kSlotQ + 2];
-    const float csnd = sqrt_gam * q2.y;
+    const float* T = f.tab + s;
+    const long ts = f.tstride;
+    const float csnd = sqrt_gam * T[kTabCs * ts];
     const float cfl_off = (float)(f.vbg[s] - g.rot_v[s] - FrRot * g.radc[s]);
-    const float t0 = (fabsf(f.f[src][1][a]) + csnd) / q0.z;
-    const float t1 = (fabsf(f.f[src][2][a] + cfl_off) + csnd) / (f.dy[j] * q0.x);
+    const float t0 = (fabsf(f.f[src][1][a]) + csnd) / T[kTabDx0 * ts];
+    const float t1 = (fabsf(f.f[src][2][a] + cfl_off) + csnd) / (f.dy[j] * T[kTabRad * ts]);
     m = fmaxf(m, fmaxf(t0, t1));
   }
   m = block_max256(m, sh);
@@ -336,13 +390,15 @@ int red_blocks(const DevGrid& g) {
   return (int)(b < 1 ? 1 : b);
 }
 
-template <int K, int WPB>
+template <int K, int W, int Q>
 hipError_t sweep_x_k(const LaunchCtx& c, const DevF32& f0, int src) {
   DevF32 f = f0;
   f.nwin_x = f32_windows(c.g.imax, K);
-  const long waves = (long)c.g.jmax * f.nwin_x;
-  dim3 grid((unsigned)((waves + WPB - 1) / WPB)), block(64 * WPB);
-  k_sweep_x_f32<K, WPB><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+  // the last window reads table entries up to 64 K - 1 slots past its first slot
+  if (f.tstride < c.g.sr * c.g.sk + 64 * K) return hipErrorInvalidValue;
+  const long ngrp = (c.g.jmax + W * Q - 1) / (W * Q);
+  dim3 grid((unsigned)(ngrp * f.nwin_x)), block(64 * W);
+  k_sweep_x_f32<K, W, Q><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
   return hipGetLastError();
 }
 
@@ -364,10 +420,11 @@ int f32_windows(int len, int K) {
   return (len + real - 1) / real;
 }
 
-const int kF32Variants[kNumF32Variants][2] = {{2, 2}, {2, 4}, {4, 2}, {4, 4}};
+const int kF32VariantsX[kNumF32Variants][3] = {{2, 4, 2}, {2, 8, 1}, {2, 16, 1}, {2, 4, 4}};
+const int kF32VariantsY[kNumF32Variants][2] = {{2, 2}, {2, 4}, {4, 2}, {4, 4}};
 
 int f32_cfl_partials_y(const DevGrid& g, int variant) {
-  return f32_windows(g.jmax, kF32Variants[variant][0]) * g.imax;
+  return f32_windows(g.jmax, kF32VariantsY[variant][0]) * g.imax;
 }
 
 hipError_t f32_prep(const LaunchCtx& c, const DevF32& f) {
@@ -379,10 +436,10 @@ hipError_t f32_prep(const LaunchCtx& c, const DevF32& f) {
 
 hipError_t f32_sweep_x(const LaunchCtx& c, const DevF32& f, int variant, int src) {
   switch (variant) {
-    case 0: return sweep_x_k<2, 2>(c, f, src);
-    case 1: return sweep_x_k<2, 4>(c, f, src);
-    case 2: return sweep_x_k<4, 2>(c, f, src);
-    case 3: return sweep_x_k<4, 4>(c, f, src);
+    case 0: return sweep_x_k<2, 4, 2>(c, f, src);
+    case 1: return sweep_x_k<2, 8, 1>(c, f, src);
+    case 2: return sweep_x_k<2, 16, 1>(c, f, src);
+    case 3: return sweep_x_k<2, 4, 4>(c, f, src);
   }
   return hipErrorInvalidValue;
 }

@@ -30,10 +30,16 @@ hipError_t launch_rotate(const LaunchCtx& c);
 hipError_t launch_cfl(const LaunchCtx& c, int src);              // partials into cflp, then ss->dt
 hipError_t launch_reduce(const LaunchCtx& c, int src, int mode, double exclude, int slot);  // -> ss->red[slot]
 hipError_t launch_totals(const LaunchCtx& c, int src, double* out2);  // out2: device, mass and angmom
-hipError_t launch_planet_half_kick(const LaunchCtx& c);
 hipError_t launch_planet_after_sweep(const LaunchCtx& c, int src);  // simtime, drift, forces, half kick
-hipError_t launch_time_advance(const LaunchCtx& c);                 // simtime += dt, ncycle++ (no planet)
 hipError_t launch_dt_finalize(const LaunchCtx& c, int n);           // dt = min(1.1 dt, courant / max of n cflp)
 hipError_t launch_sum_partials(const LaunchCtx& c, int nb, int stride, double* out);
+// CFL maxima per block into cflp (no dt); *n receives the partial count
+hipError_t launch_cfl_partials(const LaunchCtx& c, int src, int* n);
+// one workgroup before the sweeps: dt from ncfl partials (0: keep ss->dt), Index_Shift,
+// opening half kick; trig / vrot_f (FP32 tables) may be null
+hipError_t launch_pre_step(const LaunchCtx& c, int ncfl, float2* trig, float* vrot_f, int vrot_stride);
+// one workgroup after the sweeps (plnt_flag < 2): roll, time and planet update, then dt
+// from ncfl partials when ncfl > 0
+hipError_t launch_post_step(const LaunchCtx& c, int ncfl);
 
 }  // namespace pn

@@ -26,7 +26,7 @@ struct pn_ctx {
   DevGrid dg{};
   DevParams dp{};
   DevF32 df{};                      // FP32 state and tables (FP32 contexts)
-  int var_x = 3, var_y = 3;         // FP32 sweep variants (index into kF32Variants)
+  int var_x = 0, var_y = 3;         // FP32 sweep variants (indices into kF32VariantsX / Y)
   bool tuned = false;
   StepScalars* ss = nullptr;        // device
   double* partials = nullptr;       // device
@@ -151,11 +151,12 @@ int launch_cfl_any(pn_ctx* c, int src) {   // partial maxima, then dt
   return PN_OK;
 }
 
-// sweeps of one step; the state returns to cur in 2-D and moves to the other buffer in 3-D
-int launch_sweeps(pn_ctx* c, int& cur, int do_cfl) {
+// sweeps of one step; the state returns to cur in 2-D and moves to the other buffer in 3-D.
+// prep: refresh the FP32 per-step tables first; roll: the integer roll after the sweeps.
+int launch_sweeps(pn_ctx* c, int& cur, int do_cfl, bool prep, bool roll) {
   LaunchCtx L = lctx(c);
   if (is_f32(c)) {
-    PN_LAUNCH(c, "prep", f32_prep(L, c->df));
+    if (prep) PN_LAUNCH(c, "prep", f32_prep(L, c->df));
     PN_LAUNCH(c, "sweep_x", f32_sweep_x(L, c->df, c->var_x, cur));
     PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, cur ^ 1, do_cfl));
   } else {
@@ -166,29 +167,29 @@ int launch_sweeps(pn_ctx* c, int& cur, int do_cfl) {
       cur ^= 1;
     }
   }
-  if (c->P.fargo_flag > 0) PN_LAUNCH(c, "rotate", launch_rotate(L));
+  if (roll && c->P.fargo_flag > 0) PN_LAUNCH(c, "rotate", launch_rotate(L));
   return PN_OK;
 }
 
-// one full step of main.cu:345-387 on the device, reading and writing c->ss
+// one full step of main.cu:345-387 on the device, reading and writing c->ss:
+//   [CFL partials] pre-step (dt, Index_Shift, half kick, FP32 tables) -> sweeps -> post-step
+// (roll, planet, time, and the next dt when the CFL is fused into the last sweep)
 int launch_step(pn_ctx* c, int& cur) {
   LaunchCtx L = lctx(c);
   const bool fused = fused_cfl(c);
-  int rc;
-  if (!fused && (rc = launch_cfl_any(c, cur)) != PN_OK) return rc;
-  PN_LAUNCH(c, "index_shift", launch_index_shift(L));
-  if (c->P.plnt_flag > 0) PN_LAUNCH(c, "planet kick", launch_planet_half_kick(L));
-  if ((rc = launch_sweeps(c, cur, fused ? 1 : 0)) != PN_OK) return rc;
-  if (c->P.plnt_flag > 0) {
-    if (is_f32(c) && c->P.plnt_flag == 2) {
-      // the moving planet's drift and the FP32 force reductions, then forces and kick
-      return fail(c, PN_ERR_UNSUPPORTED, "moving planet on the FP32 path");
-    }
-    PN_LAUNCH(c, "planet update", launch_planet_after_sweep(L, cur));
-  } else {
-    PN_LAUNCH(c, "time advance", launch_time_advance(L));
+  if (is_f32(c) && c->P.plnt_flag == 2) return fail(c, PN_ERR_UNSUPPORTED, "moving planet on the FP32 path");
+  int ncfl = 0, rc;
+  if (!fused) {
+    if (is_f32(c)) PN_LAUNCH(c, "cfl", f32_cfl(L, c->df, cur, &ncfl));
+    else PN_LAUNCH(c, "cfl", launch_cfl_partials(L, cur, &ncfl));
   }
-  if (fused) PN_LAUNCH(c, "dt", launch_dt_finalize(L, f32_cfl_partials_y(c->dg, c->var_y)));
+  float2* trig = is_f32(c) ? c->df.trig : nullptr;
+  float* vrot = is_f32(c) ? c->df.tab + static_cast<long>(kTabVrot) * c->df.tstride : nullptr;
+  PN_LAUNCH(c, "pre step", launch_pre_step(L, ncfl, trig, vrot, 1));
+  const bool post = c->P.plnt_flag < 2;
+  if ((rc = launch_sweeps(c, cur, fused ? 1 : 0, false, !post)) != PN_OK) return rc;
+  if (post) PN_LAUNCH(c, "post step", launch_post_step(L, fused ? f32_cfl_partials_y(c->dg, c->var_y) : 0));
+  else PN_LAUNCH(c, "planet update", launch_planet_after_sweep(L, cur));
   return PN_OK;
 }
 
@@ -278,7 +279,8 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
   if (precision == PN_FP32) {
     float *fdy, *fyvol;
     double *rbg, *vbg;
-    if (r2 == PN_OK) r2 = dalloc(c, &c->df.tab, static_cast<size_t>(nslot) * kSlotQ);
+    c->df.tstride = static_cast<int>((nslot + kTabOverhang + 3) / 4 * 4);
+    if (r2 == PN_OK) r2 = dalloc(c, &c->df.tab, static_cast<size_t>(kTabEntries) * c->df.tstride);
     if (r2 == PN_OK) r2 = dalloc(c, &rbg, nslot);
     if (r2 == PN_OK) r2 = dalloc(c, &vbg, nslot);
     if (r2 == PN_OK) r2 = dalloc(c, &fdy, g->jmax);
@@ -461,9 +463,10 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
     for (int b = 0; b < 2; ++b)
       for (int f = 0; f < 3; ++f)
         PN_HIP(c, hipMemcpyAsync(c->df.f[b][f], G[f].data(), cells * sizeof(float), hipMemcpyHostToDevice, st));
-    // the per-slot table, every entry evaluated in FP64 and rounded once (2-D: slots are
-    // consecutive rings, r_cyl = rad)
-    std::vector<float> tab(static_cast<size_t>(nslot) * kSlotQ * 4, 0.0f);
+    // the per-slot table, entry-major, every entry evaluated in FP64 and rounded once (2-D:
+    // slots are consecutive rings, r_cyl = rad); the overhang past the last slot repeats it
+    const long ts = c->df.tstride;
+    std::vector<float> tab(static_cast<size_t>(kTabEntries) * ts, 0.0f);
     std::vector<double> pb(nslot), Lb(nslot);
     for (long s = 0; s < nslot; ++s) {
       pb[s] = scs2[s] * rbg[s];
@@ -478,13 +481,32 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
       const double fb = -1.0 / rad / rad + Lb[s] * Lb[s] / rad / rad / rad;
       const double Lf = hp ? 0.5 * (Lb[s] + Lb[s - 1]) : Lb[s];
       const double ffb = -1.0 / xa0 / xa0 + Lf * Lf / xa0 / xa0 / xa0;
-      const double q[kSlotQ * 4] = {
-          rad, xa0, sdx[s], 1.0 / rad,
-          rbg[s], hn ? rbg[s + 1] - rbg[s] : 0.0, pb[s], hn ? pb[s + 1] - pb[s] : 0.0,
-          scs2[s], std::sqrt(scs2[s]), Lb[s], hn ? Lb[s + 1] - Lb[s] : 0.0,
-          fb - abg, abg, dpbm, ffb,
-          Lf, 1.0 / (xa0 * xa0 * xa0), sres_v[s] / sradc[s], sxvol[s]};
-      for (int e = 0; e < kSlotQ * 4; ++e) tab[static_cast<size_t>(s) * kSlotQ * 4 + e] = static_cast<float>(q[e]);
+      double q[kTabEntries];
+      q[kTabRad] = rad;
+      q[kTabXa0] = xa0;
+      q[kTabDx0] = sdx[s];
+      q[kTabIxc] = 1.0 / rad;
+      q[kTabRb] = rbg[s];
+      q[kTabDrb] = hn ? rbg[s + 1] - rbg[s] : 0.0;
+      q[kTabPb] = pb[s];
+      q[kTabDpb] = hn ? pb[s + 1] - pb[s] : 0.0;
+      q[kTabCs2] = scs2[s];
+      q[kTabCs] = std::sqrt(scs2[s]);
+      q[kTabLb] = Lb[s];
+      q[kTabDLb] = hn ? Lb[s + 1] - Lb[s] : 0.0;
+      q[kTabResBg] = fb - abg;
+      q[kTabAbg] = abg;
+      q[kTabDpbm] = dpbm;
+      q[kTabFface] = ffb;
+      q[kTabLf] = Lf;
+      q[kTabIxa3] = 1.0 / (xa0 * xa0 * xa0);
+      q[kTabVrot] = sres_v[s] / sradc[s];
+      q[kTabDvol0] = sxvol[s];
+      for (int e = 0; e < kTabEntries; ++e) {
+        tab[e * ts + s] = static_cast<float>(q[e]);
+        if (s == nslot - 1)
+          for (long o = nslot; o < ts; ++o) tab[e * ts + o] = static_cast<float>(q[e]);
+      }
     }
     std::vector<float> t_dy(J), t_yvol(J);
     for (int j = 0; j < J; ++j) {
@@ -496,8 +518,8 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
     PN_HIP(c, hipMemcpyAsync(const_cast<float*>(c->df.yvol), t_yvol.data(), J * sizeof(float), hipMemcpyHostToDevice, st));
     PN_HIP(c, hipMemcpyAsync(const_cast<double*>(c->df.rbg), c->h_rbg.data(), nslot * sizeof(double), hipMemcpyHostToDevice, st));
     PN_HIP(c, hipMemcpyAsync(const_cast<double*>(c->df.vbg), svbg.data(), nslot * sizeof(double), hipMemcpyHostToDevice, st));
-    c->df.nwin_x = f32_windows(I, kF32Variants[c->var_x][0]);
-    c->df.nwin_y = f32_windows(J, kF32Variants[c->var_y][0]);
+    c->df.nwin_x = f32_windows(I, kF32VariantsX[c->var_x][0]);
+    c->df.nwin_y = f32_windows(J, kF32VariantsY[c->var_y][0]);
     // the host vectors above must outlive the copies
     PN_HIP(c, hipStreamSynchronize(st));
   }
@@ -654,7 +676,7 @@ int pn_sweep2d(pn_ctx* c, double dt, const pn_body* planet, double FrRot) {
   h.planet = *planet;
   if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
   int cur = c->cur;
-  if ((rc = launch_sweeps(c, cur, 0)) != PN_OK) return rc;
+  if ((rc = launch_sweeps(c, cur, 0, true, true)) != PN_OK) return rc;
   if ((rc = sync(c)) != PN_OK) return rc;
   c->cur = cur;
   return PN_OK;
@@ -749,8 +771,8 @@ int pn_tune(pn_ctx* c, int nrep, int* var_x, int* var_y) {
   }
   c->var_x = bx;
   c->var_y = by;
-  c->df.nwin_x = f32_windows(c->g.imax, kF32Variants[bx][0]);
-  c->df.nwin_y = f32_windows(c->g.jmax, kF32Variants[by][0]);
+  c->df.nwin_x = f32_windows(c->g.imax, kF32VariantsX[bx][0]);
+  c->df.nwin_y = f32_windows(c->g.jmax, kF32VariantsY[by][0]);
   c->tuned = true;
   if (c->gexec) {
     hipGraphExecDestroy(c->gexec);
@@ -768,8 +790,8 @@ int pn_set_variants(pn_ctx* c, int var_x, int var_y) {
     return fail(c, PN_ERR_ARG, "variants (%d, %d) outside [0, %d)", var_x, var_y, kNumF32Variants);
   c->var_x = var_x;
   c->var_y = var_y;
-  c->df.nwin_x = f32_windows(c->g.imax, kF32Variants[var_x][0]);
-  c->df.nwin_y = f32_windows(c->g.jmax, kF32Variants[var_y][0]);
+  c->df.nwin_x = f32_windows(c->g.imax, kF32VariantsX[var_x][0]);
+  c->df.nwin_y = f32_windows(c->g.jmax, kF32VariantsY[var_y][0]);
   c->tuned = true;
   if (c->gexec) {
     hipGraphExecDestroy(c->gexec);

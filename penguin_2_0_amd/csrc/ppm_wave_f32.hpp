@@ -141,14 +141,39 @@ struct WinP {
   C<float, K> rad, xa0, dx0, dvol0, vrot, ts, tc;
 };
 
-// AX 0 radial (cylindrical), AX 1 azimuthal.  tab / sl: the per-slot table and the slot of
-// each cell (radial sweep only; the radial balance terms are read where they are used).
-// wres: the azimuthal grid's uniform angular rate.  On return cells [6, nmax-6) hold the
-// update.
+// the per-slot table entries of this lane's K cells, entry-major (entry e of slot s at
+// p[e * stride + s]); base = the slot of the lane's first cell
+template <int K>
+struct TabLane {
+  const float* __restrict__ p;
+  int stride, base;
+  __device__ __forceinline__ C<float, K> get(int e) const {
+    const float* q = p + (long)e * stride + base;
+    C<float, K> o;
+    if constexpr (K == 2) {
+      const float2 v = *reinterpret_cast<const float2*>(q);
+      o[0] = v.x;
+      o[1] = v.y;
+    } else if constexpr (K == 4) {
+      const float4 v = *reinterpret_cast<const float4*>(q);
+      o[0] = v.x;
+      o[1] = v.y;
+      o[2] = v.z;
+      o[3] = v.w;
+    } else {
+#pragma unroll
+      for (int h = 0; h < K; ++h) o[h] = q[h];
+    }
+    return o;
+  }
+};
+
+// AX 0 radial (cylindrical), AX 1 azimuthal.  tx: the per-slot table of this lane's cells
+// (radial sweep only; the radial balance terms are read where they are used).  wres: the
+// azimuthal grid's uniform angular rate.  On return cells [6, nmax-6) hold the update.
 template <int K, int AX>
-__device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const float4* __restrict__ tab, const int* sl, int nmax,
-                                           float dt, float wres, const PlanetF& pl, bool planet, bool flat_on,
-                                           float sqrt_gam) {
+__device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, int nmax, float dt, float wres,
+                                           const PlanetF& pl, bool planet, bool flat_on, float sqrt_gam) {
   const float hdt = 0.5f * dt;
   const float fourthd = 1.3333333333333333f;
   const float smallr = 1.0e-15f;
@@ -190,12 +215,12 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const float4* __restrict_
   C<float, K> fhalf;   // hdt * force at the left edge
   if constexpr (AX == 0) {
     C<float, K> vm = prev1(W.v);
+    const C<float, K> ffb = tx.get(kTabFface), Lf = tx.get(kTabLf), ixa3 = tx.get(kTabIxa3);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
-      const float4 q3 = tab[sl[h] * kSlotQ + 3], q4 = tab[sl[h] * kSlotQ + 4];
       // l at the edge = L_f + dlf; f = f_face + dlf (2 L_f + dlf) / xa0^3 (+ planet)
       const float dlf = 0.5f * (W.v[h] + vm[h]);
-      float f = q3.w + dlf * (2.0f * q4.x + dlf) * q4.y;
+      float f = ffb[h] + dlf * (2.0f * Lf[h] + dlf) * ixa3[h];
       if (planet) {
         float s, c;
         rot_half(W.ts[h], W.tc[h], hdt * (W.vrot[h] - pl.vy), s, c);
@@ -322,12 +347,13 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const float4* __restrict_
     C<float, K> emp = next1(em);
     if constexpr (AX == 0) {
       C<float, K> pbm = prev1(W.pb), dltxp = next1(dltx);
+      const C<float, K> tixc = tx.get(kTabIxc), res_bg = tx.get(kTabResBg), abg = tx.get(kTabAbg),
+                        dpbm = tx.get(kTabDpbm);
 #pragma unroll
       for (int h = 0; h < K; ++h) {
         const int c = cell_of(K, h);
         const bool in = (c >= 3 && c < nmax - 3);
-        const float4 q0 = tab[sl[h] * kSlotQ + 0], q3 = tab[sl[h] * kSlotQ + 3];
-        const float ixc = q0.w;
+        const float ixc = tixc[h];
         const float ixc3 = ixc * ixc * ixc;
         const float rho = W.rb[h] + W.r[h];
         const float dl = W.v[h];
@@ -350,8 +376,8 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const float4* __restrict_
         //   p*(c+1) - p*(c) = dp_bm + dp,  dp = p_b(c) em(c+1) - p_b(c-1) em(c)
         const float dp = W.pb[h] * emp[h] - pbm[h] * em[h];
         const float dV = 0.25f * (dltx[h] + dltxp[h]);
-        const float acc = q3.x + q1 + 0.5f * fdiff + q3.y * (W.r[h] / rho) -
-                          (q3.z * dV + dp * (W.rad[h] + dV)) / (rho * W.dvol0[h]);
+        const float acc = res_bg[h] + q1 + 0.5f * fdiff + abg[h] * (W.r[h] / rho) -
+                          (dpbm[h] * dV + dp * (W.rad[h] + dV)) / (rho * W.dvol0[h]);
         // Lagrangian volume change, offset form of dvol = dxl (xa + dxl/2)
         const float d = dltx[h], x0 = W.xa0[h], w0 = W.dx0[h];
         const float ddv = w0 * (d + 0.5f * Dl[h]) + Dl[h] * (x0 + d + 0.5f * w0 + 0.5f * Dl[h]);

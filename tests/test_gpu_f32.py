@@ -1,0 +1,150 @@
+"""FP32-bulk path against the plain FP64 path, through the C ABI, on the same seeded inputs.
+
+The FP32 path carries Sigma, v_r and v_phi as FP32 deviations from per-ring FP64
+backgrounds and is not bit-compatible with the FP64 path; the bars are relative L1 / Linf
+errors in the lab frame (each ring's data rolled by its rot_j) with the tolerances written
+next to each check, while integer work (rot_j) and graph replay are exact.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from penguin_2_0_amd import abi, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def lab(state: np.ndarray, f: str) -> np.ndarray:
+    a = state[f]
+    return np.stack([np.roll(a[i], int(state["rot_j"][i])) for i in range(a.shape[0])])
+
+
+def rel(a, b):
+    d = np.abs(a - b)
+    return d.sum() / np.abs(b).sum(), d.max() / np.abs(b).max()
+
+
+def run(w, prec, n, disk=None, variants=None, graph=True):
+    d = disk if disk is not None else abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    with abi.Context(w.grid, w.params, prec) as ctx:
+        ctx.load(d)
+        if variants is not None:
+            ctx.set_variants(*variants)
+        st = ctx.run_begin()
+        ctx.run(n, st, graph=graph)
+        return ctx.store(), st
+
+
+def test_f32_load_store_roundtrip():
+    """Upload rounds each deviation from the background once; nothing else changes."""
+    w = configs.c1()
+    say("case: fp32 roundtrip", w.name)
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    with abi.Context(w.grid, w.params, abi.PN_FP32) as ctx:
+        ctx.load(d)
+        out = ctx.store()
+    val = d.val_array()
+    for f, bg in (("r", val[:, 0:1]), ("v", val[:, 3:4]), ("u", 0.0)):
+        dev = np.abs(d.rings[f] - bg)
+        err = np.abs(out[f] - d.rings[f])
+        # one FP32 rounding of the deviation, plus the FP64 re-addition
+        assert np.all(err <= 2.0 ** -24 * dev + 4e-16 * np.abs(d.rings[f]) + 1e-300), f
+    assert np.array_equal(out["rot_j"], d.rings["rot_j"])
+    # the isothermal closure: P = c_s^2 Sigma with the ring's c_s^2
+    cs2 = d.rings["p"] / d.rings["r"]
+    assert np.allclose(out["p"], out["r"] * cs2, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("w,n,tol", [(configs.small(130, 250), 50, 1e-6), (configs.c1(), 100, 1e-6)],
+                         ids=["disk2d_130x250", "c1"])
+def test_f32_vs_fp64(w, n, tol):
+    say("case: fp32 vs fp64", w.name, n, "steps")
+    a, sa = run(w, abi.PN_FP32, n)
+    b, sb = run(w, abi.PN_FP64, n)
+    assert np.array_equal(a["rot_j"], b["rot_j"])
+    assert abs(sa.dt - sb.dt) <= 1e-6 * sb.dt
+    assert abs(sa.simtime - sb.simtime) <= 1e-6 * sb.simtime
+    for f, t1, tinf in (("r", tol, 1e-4), ("v", tol, 1e-5)):
+        l1, li = rel(lab(a, f), lab(b, f))
+        say(f"  {f}: L1 {l1:.3e} Linf {li:.3e}")
+        assert l1 <= t1 and li <= tinf, (f, l1, li)
+    # v_r starts at zero: its error is judged against the largest |v_r|
+    l1, li = rel(lab(a, "u"), lab(b, "u"))
+    say(f"  u: L1 {l1:.3e} Linf {li:.3e}")
+    assert l1 <= 1e-4 and li <= 1e-2
+
+
+def test_f32_every_variant():
+    """Every (cells per lane, waves, columns) variant of both sweeps is a valid step."""
+    w = configs.small(130, 250)
+    say("case: fp32 variants", w.name)
+    b, _ = run(w, abi.PN_FP64, 20)
+    for vx in range(4):
+        for vy in range(4):
+            a, _ = run(w, abi.PN_FP32, 20, variants=(vx, vy))
+            l1, li = rel(lab(a, "r"), lab(b, "r"))
+            say(f"  variants ({vx}, {vy}): rho L1 {l1:.3e} Linf {li:.3e}")
+            assert np.array_equal(a["rot_j"], b["rot_j"])
+            assert l1 <= 1e-6 and li <= 1e-4
+
+
+def test_f32_graph_equals_eager_bitwise():
+    w = configs.small(130, 250)
+    say("case: fp32 graph vs eager", w.name)
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    outs = []
+    for graph in (False, True):
+        with abi.Context(w.grid, w.params, abi.PN_FP32) as ctx:
+            ctx.load(d)
+            ctx.set_variants(0, 3)
+            st = ctx.run_begin()
+            for _ in range(3):
+                ctx.run(2, st, graph=graph)
+                ctx.totals()
+            outs.append((ctx.store(), st.dt, st.simtime))
+    assert np.array_equal(outs[0][0].view(np.uint8), outs[1][0].view(np.uint8))
+    assert outs[0][1] == outs[1][1] and outs[0][2] == outs[1][2]
+
+
+def test_f32_roll_invariance_bitwise():
+    """The same physical state stored with every ring rolled by m and rot_j lowered by m steps
+    to the same lab-frame bytes."""
+    w = configs.small(48, 144)
+    say("case: fp32 roll invariance", w.name)
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    rng = np.random.default_rng(5)
+    d2 = d.copy()
+    J = w.grid.jmax
+    for n, m in enumerate(rng.integers(0, J, size=len(d2.rings))):
+        for f in ("r", "p", "u", "v", "w"):
+            d2.rings[f][n] = np.roll(d.rings[f][n], -m)
+        d2.rings["rot_j"][n] = (d.rings["rot_j"][n] + m) % J
+    a, _ = run(w, abi.PN_FP32, 3, disk=d, variants=(0, 3))
+    b, _ = run(w, abi.PN_FP32, 3, disk=d2, variants=(0, 3))
+    for f in ("r", "u", "v"):
+        assert np.array_equal(lab(a, f), lab(b, f)), f
+
+
+def test_f32_totals_track_fp64():
+    """Mass and angular momentum of the FP32 state (summed in FP64) follow the FP64 path's."""
+    w = configs.c1()
+    say("case: fp32 totals", w.name)
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    tot = {}
+    for prec in (abi.PN_FP32, abi.PN_FP64):
+        with abi.Context(w.grid, w.params, prec) as ctx:
+            ctx.load(d)
+            m0, l0 = ctx.totals()
+            st = ctx.run_begin()
+            ctx.run(100, st)
+            m1, l1 = ctx.totals()
+            tot[prec] = (m0, l0, m1, l1)
+    a, b = tot[abi.PN_FP32], tot[abi.PN_FP64]
+    say(f"  fp32 {a}\n  fp64 {b}")
+    assert abs(a[0] - b[0]) <= 1e-12 * b[0] and abs(a[1] - b[1]) <= 1e-12 * abs(b[1])
+    assert abs(a[2] - b[2]) <= 1e-9 * b[2] and abs(a[3] - b[3]) <= 1e-9 * abs(b[3])
