@@ -207,6 +207,37 @@ int pn_set_variants(pn_ctx* ctx, int var_x, int var_y);
 /* Mass and angular-momentum totals in FP64 with compensated summation. */
 int pn_totals(pn_ctx* ctx, double* mass, double* angmom);
 
+/* ---- batches of independent disks ---------------------------------------
+ * No counterpart in the reference, which advances one disk per process.  Each disk of a
+ * batch is an ordinary context (reach it with pn_batch_disk for pn_load_rings,
+ * pn_store_rings, pn_totals, snapshots) with its own dt, planet and frame; per step the
+ * batch issues one launch for every disk's pre-step, one persistent launch per sweep over
+ * the work items of all disks, and one launch for every disk's post-step -- the per-disk
+ * operations of main.cu:343-387 exactly as pn_run performs them. */
+typedef struct pn_batch pn_batch;
+
+/* n disks of grids[d] / params[d].  precision must be PN_FP32; every disk 2-D isothermal
+ * with a planet that does not move (plnt_flag < 2).  On failure *out still holds a batch
+ * whose pn_batch_last_error explains it; pn_batch_destroy it. */
+int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int precision, pn_batch** out);
+int pn_batch_destroy(pn_batch* b);
+const char* pn_batch_last_error(const pn_batch* b);
+int pn_batch_size(const pn_batch* b);
+/* The context of disk d (owned by the batch), or NULL. */
+pn_ctx* pn_batch_disk(pn_batch* b, int d);
+/* pn_run_begin for every disk; states: pn_batch_size records. */
+int pn_batch_run_begin(pn_batch* b, pn_run_state* states);
+/* Advance every disk nsteps steps (states in and out as pn_run's); graph != 0 replays a
+ * captured step.  Tunes the sweep shapes on the first call unless pinned. */
+int pn_batch_run(pn_batch* b, int nsteps, int graph, pn_run_state* states);
+/* Time every sweep variant over the whole batch (state unchanged) and keep the fastest;
+ * pin them instead with pn_batch_set_variants (same indices as pn_set_variants). */
+int pn_batch_tune(pn_batch* b, int nrep, int* var_x, int* var_y);
+int pn_batch_set_variants(pn_batch* b, int var_x, int var_y);
+/* Mean ms over nrep launches of {pre-step, r sweep, phi sweep, post-step} of the whole
+ * batch, timed with HIP events on the batch's stream.  The state advances. */
+int pn_batch_time_kernels(pn_batch* b, int nrep, double* ms4);
+
 /* ---- snapshots (output.cu:48-120, global.cu:27-38, main.cu:246-262) ----- */
 
 /* Write the imax*kmax rings of ctx as one raw binary_<label>_<NNNNN> file

@@ -116,6 +116,16 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_totals": (i, [vp, P(d), P(d)]),
         "pn_save_snapshot": (i, [vp, C.c_char_p]),
         "pn_load_snapshot": (i, [vp, C.c_char_p]),
+        "pn_batch_create": (i, [i, P(PnGrid), P(PnParams), i, P(vp)]),
+        "pn_batch_destroy": (i, [vp]),
+        "pn_batch_last_error": (C.c_char_p, [vp]),
+        "pn_batch_size": (i, [vp]),
+        "pn_batch_disk": (vp, [vp, i]),
+        "pn_batch_run_begin": (i, [vp, P(PnRunState)]),
+        "pn_batch_run": (i, [vp, i, i, P(PnRunState)]),
+        "pn_batch_tune": (i, [vp, i, P(i), P(i)]),
+        "pn_batch_set_variants": (i, [vp, i, i]),
+        "pn_batch_time_kernels": (i, [vp, i, P(d)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -191,7 +201,8 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
-            self.lib.pn_destroy(self.h)
+            if getattr(self, "_owned", True):
+                self.lib.pn_destroy(self.h)
             self.h = C.c_void_p()
 
     def __enter__(self):
@@ -272,3 +283,79 @@ class Context:
 
     def load_snapshot(self, path: str):
         self._chk(self.lib.pn_load_snapshot(self.h, str(path).encode()), "pn_load_snapshot")
+
+    @classmethod
+    def borrowed(cls, h: int, grid: PnGrid, params: PnParams, precision: int) -> "Context":
+        """A wrapper over a context another object owns (a batch's disk): close() leaves it."""
+        self = cls.__new__(cls)
+        self.lib = load_library()
+        self.grid, self.params, self.precision = grid, params, precision
+        self.h = C.c_void_p(h)
+        self._owned = False
+        return self
+
+
+class Batch:
+    """Many independent FP32 disks advanced together (pn_batch_*)."""
+
+    def __init__(self, grids: list[PnGrid], params: list[PnParams], precision: int = PN_FP32):
+        self.lib = load_library()
+        n = len(grids)
+        self.grids, self.params = list(grids), list(params)
+        ga = (PnGrid * n)(*grids)
+        pa = (PnParams * n)(*params)
+        h = C.c_void_p()
+        rc = self.lib.pn_batch_create(n, ga, pa, precision, C.byref(h))
+        self.h = h
+        if rc != PN_OK:
+            msg = self.lib.pn_batch_last_error(h).decode() if h.value else ""
+            self.close()
+            raise PnError(rc, msg)
+        self.disks = [Context.borrowed(self.lib.pn_batch_disk(h, d), grids[d], params[d], precision)
+                      for d in range(n)]
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.pn_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc: int, what: str):
+        if rc != PN_OK:
+            raise PnError(rc, f"{what}: {self.lib.pn_batch_last_error(self.h).decode()}")
+
+    def __len__(self) -> int:
+        return len(self.disks)
+
+    def run_begin(self):
+        st = (PnRunState * len(self))()
+        self._chk(self.lib.pn_batch_run_begin(self.h, st), "pn_batch_run_begin")
+        return st
+
+    def run(self, nsteps: int, states, graph: bool = True):
+        self._chk(self.lib.pn_batch_run(self.h, nsteps, 1 if graph else 0, states), "pn_batch_run")
+        return states
+
+    def tune(self, nrep: int = 3) -> tuple[int, int]:
+        vx, vy = C.c_int(), C.c_int()
+        self._chk(self.lib.pn_batch_tune(self.h, nrep, C.byref(vx), C.byref(vy)), "pn_batch_tune")
+        return vx.value, vy.value
+
+    def set_variants(self, var_x: int, var_y: int):
+        self._chk(self.lib.pn_batch_set_variants(self.h, var_x, var_y), "pn_batch_set_variants")
+
+    def time_kernels(self, nrep: int) -> dict:
+        ms = (C.c_double * 4)()
+        self._chk(self.lib.pn_batch_time_kernels(self.h, nrep, ms), "pn_batch_time_kernels")
+        return dict(zip(("pre_step", "sweep_x", "sweep_y", "post_step"), list(ms)))

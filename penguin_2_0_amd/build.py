@@ -42,6 +42,7 @@ UNITS = {
     "pn_kernels_fp32.hip": FAST_F32,
     "pn_kernels_common.hip": ["-ffp-contract=off"],
     "pn_runtime.cpp": ["-ffp-contract=off"],
+    "pn_batch.cpp": ["-ffp-contract=off"],
     "pn_host.cpp": ["-ffp-contract=off"],
 }
 

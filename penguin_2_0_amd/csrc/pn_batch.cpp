@@ -1,0 +1,328 @@
+// pn_batch.cpp -- many independent disks advanced together on one GPU.
+//
+// Each disk is an ordinary FP32 context (its own state, tables, dt, planet and scalars); the
+// batch adds one stream and, per step, four launches for all of them: one workgroup per disk
+// for the pre-step (Index_Shift, half kick, tables), one persistent launch per sweep over
+// every disk's work items (radial: window x column group; azimuthal: workgroup of waves),
+// and one workgroup per disk for the post-step (roll, planet, time, next dt from the CFL
+// partials the azimuthal sweep wrote).  A step is captured as a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "penguin.h"
+#include "pn_ctx.hpp"
+
+using namespace pn;
+using namespace pn::rt;
+
+struct pn_batch {
+  std::vector<pn_ctx*> disks;
+  std::string err;
+  hipStream_t stream = nullptr;
+  int var_x = 0, var_y = 3;   // indices into kF32VariantsX / kF32VariantsY
+  bool tuned = false;
+  int grid_x = 0, grid_y = 0;
+  DiskDev* dd = nullptr;      // device, one record per disk
+  DevF32* dfs = nullptr;      // device
+  int* base_x = nullptr;      // device, ndisk + 1 item offsets
+  int* base_y = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+int bfail(pn_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf;
+  return code;
+}
+
+#define PB_HIP(b, call)                                                                         \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) return bfail((b), PN_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+int ndisk(const pn_batch* b) { return static_cast<int>(b->disks.size()); }
+
+void drop_graph(pn_batch* b) {
+  if (b->gexec) {
+    hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+  }
+}
+
+// the per-disk records and item offsets for the current variants
+int upload_desc(pn_batch* b) {
+  const int n = ndisk(b);
+  std::vector<DiskDev> dd(n);
+  std::vector<DevF32> df(n);
+  std::vector<int> bx(n + 1, 0), by(n + 1, 0);
+  for (int d = 0; d < n; ++d) {
+    pn_ctx* c = b->disks[d];
+    if (!c->loaded) return bfail(b, PN_ERR_STATE, "disk %d has no state loaded", d);
+    const int ny = f32_cfl_partials_y(c->dg, b->var_y);
+    if (ny > c->ncflp) return bfail(b, PN_ERR_ARG, "disk %d: %d CFL partials exceed %d", d, ny, c->ncflp);
+    DiskDev& D = dd[d];
+    D.g = c->dg;
+    D.p = c->dp;
+    D.ss = c->ss;
+    D.cflp = c->cflp;
+    D.ncfl_post = ny;
+    D.trig = c->df.trig;
+    D.vrot = c->df.tab + static_cast<long>(kTabVrot) * c->df.tstride;
+    df[d] = c->df;
+    df[d].nwin_x = f32_windows(c->g.imax, kF32VariantsX[b->var_x][0]);
+    df[d].nwin_y = f32_windows(c->g.jmax, kF32VariantsY[b->var_y][0]);
+    if (c->df.tstride < c->dg.sr * c->dg.sk + 64 * kF32VariantsX[b->var_x][0])
+      return bfail(b, PN_ERR_ARG, "disk %d: table overhang too short", d);
+    bx[d + 1] = bx[d] + f32_items_x(c->dg, b->var_x);
+    by[d + 1] = by[d] + ny;
+  }
+  PB_HIP(b, hipMemcpyAsync(b->dd, dd.data(), n * sizeof(DiskDev), hipMemcpyHostToDevice, b->stream));
+  PB_HIP(b, hipMemcpyAsync(b->dfs, df.data(), n * sizeof(DevF32), hipMemcpyHostToDevice, b->stream));
+  PB_HIP(b, hipMemcpyAsync(b->base_x, bx.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  PB_HIP(b, hipMemcpyAsync(b->base_y, by.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  PB_HIP(b, hipStreamSynchronize(b->stream));
+  // persistent grids: what is resident at once, or fewer when there are fewer items
+  auto fit = [](int resident, int items) { return std::max(8, std::min(resident, (items + 7) / 8 * 8)); };
+  b->grid_x = fit(f32_batch_grid(0, b->var_x), bx[n]);
+  b->grid_y = fit(f32_batch_grid(1, b->var_y), by[n]);
+  if (b->grid_x < 8 || b->grid_y < 8) return bfail(b, PN_ERR_HIP, "occupancy query failed");
+  return PN_OK;
+}
+
+int launch_batch_step(pn_batch* b, int cur) {
+  const int n = ndisk(b);
+  hipError_t e = launch_pre_step_batch(b->stream, b->dd, n);
+  if (e == hipSuccess) e = f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x);
+  if (e == hipSuccess) e = f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur ^ 1, b->grid_y);
+  if (e == hipSuccess) e = launch_post_step_batch(b->stream, b->dd, n);
+  if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "batched step: %s", hipGetErrorString(e));
+  return PN_OK;
+}
+
+int common_cur(pn_batch* b, int* cur) {
+  *cur = b->disks[0]->cur;
+  for (pn_ctx* c : b->disks) {
+    if (!c->loaded) return bfail(b, PN_ERR_STATE, "a disk has no state loaded");
+    if (c->cur != *cur) return bfail(b, PN_ERR_STATE, "disks sit in different buffers");
+  }
+  return PN_OK;
+}
+
+// per-step tables of every disk (trig, vrot) without touching the state
+int prep_all(pn_batch* b) {
+  for (pn_ctx* c : b->disks) {
+    LaunchCtx L = lctx(c);
+    L.stream = b->stream;
+    hipError_t e = f32_prep(L, c->df);
+    if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "prep: %s", hipGetErrorString(e));
+  }
+  return PN_OK;
+}
+
+template <class F>
+int btime(pn_batch* b, int nrep, double* ms, F&& launch) {
+  hipEvent_t e0, e1;
+  PB_HIP(b, hipEventCreate(&e0));
+  PB_HIP(b, hipEventCreate(&e1));
+  hipError_t e = hipEventRecord(e0, b->stream);
+  for (int k = 0; k < nrep && e == hipSuccess; ++k) e = launch();
+  if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "batch timing: %s", hipGetErrorString(e));
+  *ms = static_cast<double>(t) / nrep;
+  return PN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int precision, pn_batch** out) {
+  if (!out) return PN_ERR_ARG;
+  *out = nullptr;
+  pn_batch* b = new pn_batch();
+  *out = b;   // the caller reads the message of a failure, then pn_batch_destroy
+  if (n < 1 || !grids || !params) return bfail(b, PN_ERR_ARG, "batch of %d disks", n);
+  if (precision != PN_FP32) return bfail(b, PN_ERR_UNSUPPORTED, "the batched launch is built for the FP32 path");
+  for (int d = 0; d < n; ++d) {
+    if (params[d].plnt_flag == 2)
+      return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: a moving planet needs its force reductions between sweeps", d);
+    pn_ctx* c = nullptr;
+    const int rc = pn_create(&grids[d], &params[d], precision, &c);
+    if (c) b->disks.push_back(c);
+    if (rc != PN_OK) return bfail(b, rc, "disk %d: %s", d, c ? c->err.c_str() : "pn_create failed");
+  }
+  PB_HIP(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  PB_HIP(b, hipMalloc(reinterpret_cast<void**>(&b->dd), n * sizeof(DiskDev)));
+  PB_HIP(b, hipMalloc(reinterpret_cast<void**>(&b->dfs), n * sizeof(DevF32)));
+  PB_HIP(b, hipMalloc(reinterpret_cast<void**>(&b->base_x), (n + 1) * sizeof(int)));
+  PB_HIP(b, hipMalloc(reinterpret_cast<void**>(&b->base_y), (n + 1) * sizeof(int)));
+  return PN_OK;
+}
+
+int pn_batch_destroy(pn_batch* b) {
+  if (!b) return PN_OK;
+  drop_graph(b);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  for (pn_ctx* c : b->disks) pn_destroy(c);
+  if (b->dd) hipFree(b->dd);
+  if (b->dfs) hipFree(b->dfs);
+  if (b->base_x) hipFree(b->base_x);
+  if (b->base_y) hipFree(b->base_y);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return PN_OK;
+}
+
+const char* pn_batch_last_error(const pn_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+pn_ctx* pn_batch_disk(pn_batch* b, int d) {
+  if (!b || d < 0 || d >= ndisk(b)) return nullptr;
+  return b->disks[d];
+}
+
+int pn_batch_size(const pn_batch* b) { return b ? ndisk(b) : 0; }
+
+int pn_batch_run_begin(pn_batch* b, pn_run_state* states) {
+  if (!b || !states) return PN_ERR_ARG;
+  for (int d = 0; d < ndisk(b); ++d) {
+    const int rc = pn_run_begin(b->disks[d], &states[d]);
+    if (rc != PN_OK) return bfail(b, rc, "disk %d: %s", d, b->disks[d]->err.c_str());
+  }
+  return PN_OK;
+}
+
+int pn_batch_set_variants(pn_batch* b, int var_x, int var_y) {
+  if (!b) return PN_ERR_ARG;
+  if (var_x < 0 || var_x >= kNumF32Variants || var_y < 0 || var_y >= kNumF32Variants)
+    return bfail(b, PN_ERR_ARG, "variants (%d, %d) outside [0, %d)", var_x, var_y, kNumF32Variants);
+  if (var_x != b->var_x || var_y != b->var_y) drop_graph(b);
+  b->var_x = var_x;
+  b->var_y = var_y;
+  b->tuned = true;
+  return PN_OK;
+}
+
+int pn_batch_tune(pn_batch* b, int nrep, int* var_x, int* var_y) {
+  if (!b || nrep < 1) return PN_ERR_ARG;
+  int cur, rc;
+  if ((rc = common_cur(b, &cur)) != PN_OK) return rc;
+  if ((rc = prep_all(b)) != PN_OK) return rc;
+  const int keep_x = b->var_x, keep_y = b->var_y;
+  double best_x = 1e30, best_y = 1e30;
+  int bx = keep_x, by = keep_y;
+  // every candidate reads the current buffer and writes the other one: the state stays
+  for (int v = 0; v < kNumF32Variants; ++v) {
+    b->var_x = v;
+    b->var_y = v;
+    if ((rc = upload_desc(b)) != PN_OK) return rc;
+    double tx = 0.0, ty = 0.0;
+    const int n = ndisk(b);
+    rc = btime(b, nrep, &tx, [&]() {
+      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x);
+    });
+    if (rc == PN_OK)
+      rc = btime(b, nrep, &ty, [&]() {
+        return f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur, b->grid_y);
+      });
+    if (rc != PN_OK) return rc;
+    if (tx < best_x) { best_x = tx; bx = v; }
+    if (ty < best_y) { best_y = ty; by = v; }
+  }
+  b->var_x = bx;
+  b->var_y = by;
+  b->tuned = true;
+  drop_graph(b);
+  if (var_x) *var_x = bx;
+  if (var_y) *var_y = by;
+  return PN_OK;
+}
+
+int pn_batch_run(pn_batch* b, int nsteps, int graph, pn_run_state* states) {
+  if (!b || !states || nsteps < 0) return PN_ERR_ARG;
+  int cur, rc;
+  if ((rc = common_cur(b, &cur)) != PN_OK) return rc;
+  if (!b->tuned && (rc = pn_batch_tune(b, 3, nullptr, nullptr)) != PN_OK) return rc;
+  if ((rc = upload_desc(b)) != PN_OK) return rc;
+  // every disk's scalars, and its first dt from its state (the fused CFL supplies the rest)
+  for (int d = 0; d < ndisk(b); ++d) {
+    pn_ctx* c = b->disks[d];
+    StepScalars h{};
+    h.dt = states[d].dt;
+    h.FrRot = states[d].FrRot;
+    h.simtime = states[d].simtime;
+    h.planet = states[d].planet;
+    h.ncycle = states[d].ncycle;
+    if ((rc = upload_scalars(c, h)) != PN_OK || (nsteps > 0 && (rc = launch_cfl_any(c, cur)) != PN_OK) ||
+        (rc = sync(c)) != PN_OK)
+      return bfail(b, rc, "disk %d: %s", d, c->err.c_str());
+  }
+  int done = 0;
+  if (graph && nsteps > 0) {
+    if (!b->gexec) {
+      hipGraph_t gr;
+      PB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      rc = launch_batch_step(b, cur);
+      hipError_t e = hipStreamEndCapture(b->stream, &gr);
+      if (rc != PN_OK) return rc;
+      if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&b->gexec, gr, nullptr, nullptr, 0);
+      hipGraphDestroy(gr);
+      if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    for (; done < nsteps; ++done) PB_HIP(b, hipGraphLaunch(b->gexec, b->stream));
+  }
+  for (; done < nsteps; ++done)
+    if ((rc = launch_batch_step(b, cur)) != PN_OK) return rc;
+  PB_HIP(b, hipStreamSynchronize(b->stream));
+  for (int d = 0; d < ndisk(b); ++d) {
+    pn_ctx* c = b->disks[d];
+    StepScalars h{};
+    if ((rc = download_scalars(c, &h)) != PN_OK) return bfail(b, rc, "disk %d: %s", d, c->err.c_str());
+    // the record already holds the next step's dt; report the last one used
+    states[d].dt = nsteps > 0 ? h.dt_prev : h.dt;
+    states[d].FrRot = h.FrRot;
+    states[d].simtime = h.simtime;
+    states[d].planet = h.planet;
+    states[d].ncycle = h.ncycle;
+    if (h.bad) return bfail(b, PN_ERR_NUMERIC, "disk %d: non-finite or non-positive dt during the run", d);
+  }
+  return PN_OK;
+}
+
+int pn_batch_time_kernels(pn_batch* b, int nrep, double* ms4) {
+  if (!b || !ms4 || nrep < 1) return PN_ERR_ARG;
+  int cur, rc;
+  if ((rc = common_cur(b, &cur)) != PN_OK) return rc;
+  if ((rc = upload_desc(b)) != PN_OK) return rc;
+  const int n = ndisk(b);
+  rc = btime(b, nrep, &ms4[0], [&]() { return launch_pre_step_batch(b->stream, b->dd, n); });
+  if (rc == PN_OK)
+    rc = btime(b, nrep, &ms4[1], [&]() {
+      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x);
+    });
+  if (rc == PN_OK)
+    rc = btime(b, nrep, &ms4[2], [&]() {
+      return f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur, b->grid_y);
+    });
+  if (rc == PN_OK) rc = btime(b, nrep, &ms4[3], [&]() { return launch_post_step_batch(b->stream, b->dd, n); });
+  return rc;
+}
+
+}  // extern "C"

@@ -61,6 +61,8 @@ struct StepScalars {
   double red[4];      // reduction results (mass, modes 8 / 9)
   int64_t ncycle;
   int bad;            // non-finite dt seen
+  int trig_ok;        // FP32 planet trig table holds the planet azimuth trig_y
+  double trig_y;
 };
 
 __host__ __device__ inline int slot_of(const DevGrid& g, int i, int k) {

@@ -11,10 +11,13 @@ namespace pn {
 //   rho_b, rho_b(s+1) - rho_b(s), p_b, p_b(s+1) - p_b(s),
 //   c_s^2, c_s, L_b, L_b(s+1) - L_b(s),
 //   res_bg, a_bg, dp_bm = p_b(s) - p_b(s-1), f_face,
-//   L_f = (L_b(s) + L_b(s-1))/2, 1/xa0^3, vrot = res_v / r (rewritten each step), dvol0
+//   L_f = (L_b(s) + L_b(s-1))/2, 1/xa0^3, vrot = res_v / r (rewritten each step), dvol0,
+//   nu, (nu rho_b)', Omega_b', Omega_b''
 // with, in FP64: a_bg = dp_bm V0 / (rho_b dvol0), V0 = (xa0(s) + xa0(s+1))/2, the pressure
 // acceleration of the background; res_bg = -1/rad^2 + L_b^2/rad^3 - a_bg, the background's
-// net radial acceleration; f_face = -1/xa0^2 + L_f^2/xa0^3, the force at the left edge.
+// net radial acceleration; f_face = -1/xa0^2 + L_f^2/xa0^3, the force at the left edge;
+// Omega_b = L_b / rad^2, and the derivatives are those of the viscosity's quadratic fit
+// through the left edges of slots s-1, s, s+1 (viscosity.cu:2-19).
 // The stride leaves room past the last slot for a radial window's overhang (copies of the
 // last slot, as a clamped read would give).
 enum TabEntry {
@@ -23,6 +26,7 @@ enum TabEntry {
   kTabCs2, kTabCs, kTabLb, kTabDLb,
   kTabResBg, kTabAbg, kTabDpbm, kTabFface,
   kTabLf, kTabIxa3, kTabVrot, kTabDvol0,
+  kTabNu, kTabDnb, kTabD1Ob, kTabD2Ob,
   kTabEntries
 };
 constexpr int kTabOverhang = 256;   // >= the longest radial window
@@ -50,9 +54,20 @@ int f32_windows(int len, int K);
 // per-step tables: planet trig per column, vrot per slot (after Index_Shift)
 hipError_t f32_prep(const LaunchCtx& c, const DevF32& f);
 hipError_t f32_sweep_x(const LaunchCtx& c, const DevF32& f, int variant, int src);
-// do_cfl: write each wave's CFL maximum of the stored state to c.cflp[wave]
+// do_cfl: write each workgroup's CFL maximum of the stored state to c.cflp[workgroup]
 hipError_t f32_sweep_y(const LaunchCtx& c, const DevF32& f, int variant, int src, int do_cfl);
 int f32_cfl_partials_y(const DevGrid& g, int variant);   // partials the fused CFL writes
+
+// batches of disks: one persistent launch per sweep over every disk's work items.
+// items: workgroup-sized work items of one disk (radial: window x column group; azimuthal:
+// the y partial count).  base: ndisk + 1 item offsets in device memory; dd / dfs: per-disk
+// records in device memory.  grid: f32_batch_grid(axis, variant).
+int f32_items_x(const DevGrid& g, int variant);
+int f32_batch_grid(int axis, int variant);
+hipError_t f32_sweep_x_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
+                             int variant, int src, int grid);
+hipError_t f32_sweep_y_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
+                             int variant, int src, int grid);
 // CFL maxima per block into c.cflp; returns the partial count through *n
 hipError_t f32_cfl(const LaunchCtx& c, const DevF32& f, int src, int* n);
 hipError_t f32_reduce(const LaunchCtx& c, const DevF32& f, int src, int mode, double exclude, int slot);

@@ -305,11 +305,12 @@ __device__ void index_shift_ring(DevGrid& g, const DevParams& prm, int n, double
 // planet's opening half kick.  For FP32 contexts also the per-step tables: sin / cos of half
 // the angle from the planet to each column centre (trig) and vrot = res_v / r_cyl per slot
 // (vrot_f[s * vrot_stride]).
-__global__ __launch_bounds__(kStepThreads) void k_pre_step(DevGrid g, DevParams prm, StepScalars* ss,
-                                                           const double* __restrict__ cflp, int ncfl, float2* trig,
-                                                           float* vrot_f, int vrot_stride) {
+__device__ __forceinline__ void pre_step_body(DevGrid g, const DevParams& prm, StepScalars* ss,
+                                              const double* __restrict__ cflp, int ncfl, float2* trig, float* vrot_f,
+                                              int vrot_stride) {
   __shared__ double sh[kStepThreads / 64];
-  __shared__ double s_dt, s_fr;
+  __shared__ double s_dt, s_fr, s_py;
+  __shared__ int s_trig;
   if (ncfl > 0) {
     const double mx = block_max_partials(cflp, ncfl, sh);
     if (threadIdx.x == 0) finalize_dt(prm, ss, mx);
@@ -317,6 +318,13 @@ __global__ __launch_bounds__(kStepThreads) void k_pre_step(DevGrid g, DevParams 
   if (threadIdx.x == 0) {
     s_dt = ss->dt;
     s_fr = ss->FrRot;
+    s_py = ss->planet.y;   // the half kick below changes velocities only
+    // the trig table depends on the planet's azimuth alone: refresh it when that moved
+    s_trig = (trig != nullptr) && !(ss->trig_ok && ss->trig_y == s_py);
+    if (s_trig) {
+      ss->trig_ok = 1;
+      ss->trig_y = s_py;
+    }
   }
   __syncthreads();
   const double dt = s_dt, FrRot = s_fr;
@@ -328,8 +336,8 @@ __global__ __launch_bounds__(kStepThreads) void k_pre_step(DevGrid g, DevParams 
       vrot_f[(long)s * vrot_stride] = (float)(g.res_v[s] / g.radc[s]);
     }
   }
-  if (trig) {
-    const double py = ss->planet.y;   // the half kick changes velocities only
+  if (s_trig) {
+    const double py = s_py;
     for (int j = threadIdx.x; j < g.jmax; j += blockDim.x) {
       const double a = 0.5 * (g.yc[j] - py);
       trig[j] = make_float2((float)sin(a), (float)cos(a));
@@ -346,12 +354,24 @@ __global__ __launch_bounds__(kStepThreads) void k_pre_step(DevGrid g, DevParams 
   }
 }
 
+__global__ __launch_bounds__(kStepThreads) void k_pre_step(DevGrid g, DevParams prm, StepScalars* ss,
+                                                           const double* __restrict__ cflp, int ncfl, float2* trig,
+                                                           float* vrot_f, int vrot_stride) {
+  pre_step_body(g, prm, ss, cflp, ncfl, trig, vrot_f, vrot_stride);
+}
+
+// one workgroup per disk of a batch (the fused CFL leaves dt already set: ncfl 0)
+__global__ __launch_bounds__(kStepThreads) void k_pre_step_batch(const DiskDev* __restrict__ dd) {
+  const DiskDev& D = dd[blockIdx.x];
+  pre_step_body(D.g, D.p, D.ss, D.cflp, 0, D.trig, D.vrot, 1);
+}
+
 // Everything of a step after the sweeps for a planet that does not feel the disk
 // (plnt_flag < 2), in one workgroup: the integer roll of every ring (ppm.cu:309-322), time
 // advance, drift, forces and closing half kick (main.cu:378-387, planet.cu:54-93), then,
 // with ncfl > 0, the next step's dt from the CFL partials the last sweep wrote.
-__global__ __launch_bounds__(kStepThreads) void k_post_step(DevGrid g, DevParams prm, StepScalars* ss,
-                                                            const double* __restrict__ cflp, int ncfl) {
+__device__ __forceinline__ void post_step_body(DevGrid g, const DevParams& prm, StepScalars* ss,
+                                               const double* __restrict__ cflp, int ncfl) {
   __shared__ double sh[kStepThreads / 64];
   if (prm.fargo > 0) {
     const int nring = g.imax * g.kmax;
@@ -388,6 +408,16 @@ __global__ __launch_bounds__(kStepThreads) void k_post_step(DevGrid g, DevParams
     b.vy -= ss->FrRot;
   }
   if (ncfl > 0) finalize_dt(prm, ss, mx);
+}
+
+__global__ __launch_bounds__(kStepThreads) void k_post_step(DevGrid g, DevParams prm, StepScalars* ss,
+                                                            const double* __restrict__ cflp, int ncfl) {
+  post_step_body(g, prm, ss, cflp, ncfl);
+}
+
+__global__ __launch_bounds__(kStepThreads) void k_post_step_batch(const DiskDev* __restrict__ dd) {
+  const DiskDev& D = dd[blockIdx.x];
+  post_step_body(D.g, D.p, D.ss, D.cflp, D.ncfl_post);
 }
 
 int red_blocks(const DevGrid& g) {
@@ -444,6 +474,18 @@ hipError_t launch_pre_step(const LaunchCtx& c, int ncfl, float2* trig, float* vr
 hipError_t launch_post_step(const LaunchCtx& c, int ncfl) {
   if (ncfl < 0 || ncfl > c.ncflp || c.p.plnt == 2) return hipErrorInvalidValue;
   k_post_step<<<1, kStepThreads, 0, c.stream>>>(c.g, c.p, c.ss, c.cflp, ncfl);
+  return hipGetLastError();
+}
+
+hipError_t launch_pre_step_batch(hipStream_t s, const DiskDev* dd, int ndisk) {
+  if (ndisk < 1) return hipErrorInvalidValue;
+  k_pre_step_batch<<<ndisk, kStepThreads, 0, s>>>(dd);
+  return hipGetLastError();
+}
+
+hipError_t launch_post_step_batch(hipStream_t s, const DiskDev* dd, int ndisk) {
+  if (ndisk < 1) return hipErrorInvalidValue;
+  k_post_step_batch<<<ndisk, kStepThreads, 0, s>>>(dd);
   return hipGetLastError();
 }
 

@@ -8,8 +8,8 @@
 // the time, the backgrounds and residual rates, and the totals.
 //
 // Work decomposition as in the FP64 sweeps, with K cells per lane (window 64 K, halo 6) and
-// W waves per workgroup picked by the autotuner; the CFL maximum of the next step is taken
-// in the azimuthal sweep's epilogue from the values it stores, one partial per wave.
+// the workgroup shape picked by the autotuner; the CFL maximum of the next step is taken in
+// the azimuthal sweep's epilogue from the values it stores, one partial per workgroup.
 #include "pn_f32.hpp"
 #include "ppm_wave_f32.hpp"
 
@@ -89,17 +89,25 @@ __device__ __forceinline__ C<float, K> lds_k(const float* p) {
 // LDS with loads that are contiguous along phi (each ring's NC cells are adjacent in HBM);
 // every wave then runs Q columns as radial pencils out of LDS, writes its results back in
 // place, and the workgroup stores the window's interior rings contiguously again.
+// VISC: 0 none, 1 always, 2 as the disk's parameters say.
 template <int K, int W, int Q>
-__global__ __launch_bounds__(64 * W) void k_sweep_x_f32(DevGrid g, DevF32 f, DevParams prm,
-                                                        const StepScalars* __restrict__ ss, int src) {
-  constexpr int WIN = 64 * K, REAL = WIN - 2 * kNpad, NC = W * Q, LD = WIN + 2;
-  __shared__ float tile[3][NC][LD];   // [field][column][cell]; LD even keeps K-float reads aligned
-  __shared__ int srot[WIN];
+struct XTile {
+  static constexpr int WIN = 64 * K, NC = W * Q, LD = WIN + 2;
+  float tile[3][NC][LD];   // [field][column][cell]; LD even keeps K-float reads aligned
+  int srot[WIN];
+};
+
+// one work item of the radial sweep: window l, column group grp.  Workgroup-uniform; starts
+// with a barrier so that a workgroup can run items back to back on the same LDS.
+template <int K, int W, int Q, int VISC>
+__device__ __forceinline__ void x_item(const DevGrid& g, const DevF32& f, const DevParams& prm,
+                                       const StepScalars* __restrict__ ss, int src, int grp, int l,
+                                       XTile<K, W, Q>& T) {
+  constexpr int WIN = 64 * K, REAL = WIN - 2 * kNpad, NC = W * Q;
+  auto& tile = T.tile;
+  auto& srot = T.srot;
   const int jmax = g.jmax;
-  const int ngrp = (jmax + NC - 1) / NC;
-  const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int grp = bid % ngrp, l = bid / ngrp;
-  if (l >= f.nwin_x) return;   // workgroup-uniform, before any barrier
+  __syncthreads();
   const int jb = grp * NC;
   const int lim = win_len_k<K>(g.imax, f.nwin_x, l);
   const int s0 = slot_of(g, REAL * l - kNpad, 0);   // slot of window cell 0
@@ -154,6 +162,7 @@ __global__ __launch_bounds__(64 * W) void k_sweep_x_f32(DevGrid g, DevF32 f, Dev
 #pragma unroll
     for (int h = 0; h < K; ++h) Wp.v[h] = Wp.rad[h] * vp[h];   // l - L_b = r (v_phi - v_bg); r_cyl = r in 2-D
     ppmlr_pert<K, 0>(Wp, tx, lim, dt, 0.0f, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
+    if (VISC == 1 || (VISC == 2 && prm.visc)) visc_r_pert<K>(Wp, tx, lim, dt);
     const C<float, K> ixc = tx.get(kTabIxc);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
@@ -183,17 +192,56 @@ __global__ __launch_bounds__(64 * W) void k_sweep_x_f32(DevGrid g, DevF32 f, Dev
   }
 }
 
+// one disk: one workgroup per item, XCD-contiguous
+template <int K, int W, int Q, int VISC>
+__global__ __launch_bounds__(64 * W) void k_sweep_x_f32(DevGrid g, DevF32 f, DevParams prm,
+                                                        const StepScalars* __restrict__ ss, int src) {
+  __shared__ XTile<K, W, Q> T;
+  const int ngrp = (g.jmax + W * Q - 1) / (W * Q);
+  const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int grp = bid % ngrp, l = bid / ngrp;
+  if (l >= f.nwin_x) return;   // workgroup-uniform, before any barrier
+  x_item<K, W, Q, VISC>(g, f, prm, ss, src, grp, l, T);
+}
+
+// the disk owning work item it: base[d] <= it < base[d + 1] (ndisk small; base in device memory)
+__device__ __forceinline__ int disk_of(const int* __restrict__ base, int ndisk, int it) {
+  int lo = 0, hi = ndisk - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= it) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// many disks, persistent: workgroup b runs items t = b, b + G, ... (G = gridDim, a multiple
+// of 8), each mapped through the XCD permutation so that one XCD walks a contiguous run of
+// items (neighbouring column groups share L2 lines).  base: ndisk + 1 item offsets.
+template <int K, int W, int Q>
+__global__ __launch_bounds__(64 * W) void k_sweep_x_f32_batch(const DiskDev* __restrict__ dd,
+                                                              const DevF32* __restrict__ dfs,
+                                                              const int* __restrict__ base, int ndisk, int src) {
+  __shared__ XTile<K, W, Q> T;
+  const int nitems = base[ndisk];
+  for (int t = blockIdx.x; t < nitems; t += gridDim.x) {
+    const int it = xcd_swizzle(t, nitems);
+    const int d = disk_of(base, ndisk, it);
+    const DevGrid g = dd[d].g;
+    const int ngrp = (g.jmax + W * Q - 1) / (W * Q);
+    const int local = it - base[d];
+    x_item<K, W, Q, 2>(g, dfs[d], dd[d].p, dd[d].ss, src, local % ngrp, local / ngrp, T);
+  }
+}
+
 // ---------------------------------------------------------------------------------------
-template <int K, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, DevParams prm,
-                                                          const StepScalars* __restrict__ ss, double* cflp, int src,
-                                                          int do_cfl) {
+// Azimuthal sweep of window l of interior ring n by one wavefront; returns the wave's
+// maximum CFL term of the cells it stored.
+template <int K, int VISC>
+__device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, const DevParams& prm,
+                                          const StepScalars* __restrict__ ss, int src, int n, int l) {
   constexpr int REAL = 64 * K - 2 * kNpad;
   const int lane = threadIdx.x & 63;
-  const int gw = xcd_swizzle(blockIdx.x, gridDim.x) * WPB + (threadIdx.x >> 6);
-  const int l = gw % f.nwin_y;
-  const int n = gw / f.nwin_y;
-  if (n >= g.imax) return;   // wave-uniform; no barriers in this kernel
   const int s = slot_of(g, n, 0);
   const int jmax = g.jmax;
   const float dt = (float)ss->dt;
@@ -249,6 +297,7 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, D
   }
   const float sqrt_gam = sqrtf((float)prm.gam);
   ppmlr_pert<K, 1>(W, TabLane<K>{nullptr, 0, 0}, lim, dt, wres, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
+  if (VISC == 1 || (VISC == 2 && prm.visc)) visc_p_pert<K>(W, lim, dt, T[kTabNu * ts]);
 
   float* __restrict__ dr = f.f[src ^ 1][0] + base;
   float* __restrict__ du = f.f[src ^ 1][1] + base;
@@ -270,9 +319,59 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, D
       m = fmaxf(m, fmaxf(t0, t1));
     }
   }
-  if (do_cfl) {
-    m = wave_maxf(m);
-    if (lane == 0) cflp[gw] = (double)m;
+  return wave_maxf(m);
+}
+
+// one wavefront per (ring, window); with do_cfl each workgroup writes the maximum CFL term
+// of its waves to cflp[workgroup]
+template <int K, int WPB, int VISC>
+__global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, DevParams prm,
+                                                          const StepScalars* __restrict__ ss, double* cflp, int src,
+                                                          int do_cfl) {
+  __shared__ float shm[WPB];
+  const int wv = threadIdx.x >> 6;
+  const int gw = xcd_swizzle(blockIdx.x, gridDim.x) * WPB + wv;
+  const int l = gw % f.nwin_y;
+  const int n = gw / f.nwin_y;
+  float m = 0.0f;
+  if (n < g.imax) m = y_pencil<K, VISC>(g, f, prm, ss, src, n, l);   // wave-uniform
+  if (do_cfl) {   // kernel-uniform: every wave reaches the barrier
+    if ((threadIdx.x & 63) == 0) shm[wv] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < WPB; ++w) m = fmaxf(m, shm[w]);
+      cflp[blockIdx.x] = (double)m;
+    }
+  }
+}
+
+// many disks, persistent (as k_sweep_x_f32_batch): item = one workgroup's WPB waves of one
+// disk; each item writes its CFL partial to its disk's cflp[item - base[d]]
+template <int K, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32_batch(const DiskDev* __restrict__ dd,
+                                                              const DevF32* __restrict__ dfs,
+                                                              const int* __restrict__ base, int ndisk, int src) {
+  __shared__ float shm[WPB];
+  const int wv = threadIdx.x >> 6;
+  const int nitems = base[ndisk];
+  for (int t = blockIdx.x; t < nitems; t += gridDim.x) {
+    const int it = xcd_swizzle(t, nitems);
+    const int d = disk_of(base, ndisk, it);
+    const DevGrid g = dd[d].g;
+    const DevF32& f = dfs[d];
+    const int local = it - base[d];
+    const int gw = local * WPB + wv;
+    const int l = gw % f.nwin_y;
+    const int n = gw / f.nwin_y;
+    float m = 0.0f;
+    if (n < g.imax) m = y_pencil<K, 2>(g, f, dd[d].p, dd[d].ss, src, n, l);   // wave-uniform
+    __syncthreads();   // the previous item's partial has been read
+    if ((threadIdx.x & 63) == 0) shm[wv] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < WPB; ++w) m = fmaxf(m, shm[w]);
+      dd[d].cflp[local] = (double)m;
+    }
   }
 }
 
@@ -398,7 +497,8 @@ hipError_t sweep_x_k(const LaunchCtx& c, const DevF32& f0, int src) {
   if (f.tstride < c.g.sr * c.g.sk + 64 * K) return hipErrorInvalidValue;
   const long ngrp = (c.g.jmax + W * Q - 1) / (W * Q);
   dim3 grid((unsigned)(ngrp * f.nwin_x)), block(64 * W);
-  k_sweep_x_f32<K, W, Q><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+  if (c.p.visc) k_sweep_x_f32<K, W, Q, 1><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+  else k_sweep_x_f32<K, W, Q, 0><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
   return hipGetLastError();
 }
 
@@ -407,9 +507,11 @@ hipError_t sweep_y_k(const LaunchCtx& c, const DevF32& f0, int src, int do_cfl) 
   DevF32 f = f0;
   f.nwin_y = f32_windows(c.g.jmax, K);
   const long waves = (long)f.nwin_y * c.g.imax;
-  if (do_cfl && waves > c.ncflp) return hipErrorInvalidValue;
-  dim3 grid((unsigned)((waves + WPB - 1) / WPB)), block(64 * WPB);
-  k_sweep_y_f32<K, WPB><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  const long nblk = (waves + WPB - 1) / WPB;
+  if (do_cfl && nblk > c.ncflp) return hipErrorInvalidValue;
+  dim3 grid((unsigned)nblk), block(64 * WPB);
+  if (c.p.visc) k_sweep_y_f32<K, WPB, 1><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  else k_sweep_y_f32<K, WPB, 0><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
   return hipGetLastError();
 }
 
@@ -424,7 +526,9 @@ const int kF32VariantsX[kNumF32Variants][3] = {{2, 4, 2}, {2, 8, 1}, {2, 16, 1},
 const int kF32VariantsY[kNumF32Variants][2] = {{2, 2}, {2, 4}, {4, 2}, {4, 4}};
 
 int f32_cfl_partials_y(const DevGrid& g, int variant) {
-  return f32_windows(g.jmax, kF32VariantsY[variant][0]) * g.imax;
+  const int waves = f32_windows(g.jmax, kF32VariantsY[variant][0]) * g.imax;
+  const int wpb = kF32VariantsY[variant][1];
+  return (waves + wpb - 1) / wpb;
 }
 
 hipError_t f32_prep(const LaunchCtx& c, const DevF32& f) {
@@ -450,6 +554,83 @@ hipError_t f32_sweep_y(const LaunchCtx& c, const DevF32& f, int variant, int src
     case 1: return sweep_y_k<2, 4>(c, f, src, do_cfl);
     case 2: return sweep_y_k<4, 2>(c, f, src, do_cfl);
     case 3: return sweep_y_k<4, 4>(c, f, src, do_cfl);
+  }
+  return hipErrorInvalidValue;
+}
+
+namespace {
+
+// resident workgroups of kernel k on the whole device, rounded down to a multiple of 8 XCDs
+template <class Kern>
+int persistent_grid(Kern k, int block) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, 0) != hipSuccess) return 0;
+  const int g = cus * (per_cu > 0 ? per_cu : 1);
+  return g >= 8 ? g / 8 * 8 : 8;
+}
+
+template <int K, int W, int Q>
+hipError_t sweep_x_batch_k(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
+                           int src, int grid) {
+  if (grid < 8 || grid % 8) return hipErrorInvalidValue;
+  k_sweep_x_f32_batch<K, W, Q><<<grid, 64 * W, 0, s>>>(dd, dfs, base, ndisk, src);
+  return hipGetLastError();
+}
+
+template <int K, int WPB>
+hipError_t sweep_y_batch_k(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
+                           int src, int grid) {
+  if (grid < 8 || grid % 8) return hipErrorInvalidValue;
+  k_sweep_y_f32_batch<K, WPB><<<grid, 64 * WPB, 0, s>>>(dd, dfs, base, ndisk, src);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int f32_batch_grid(int axis, int variant) {
+  if (axis == 0) {
+    switch (variant) {
+      case 0: return persistent_grid(k_sweep_x_f32_batch<2, 4, 2>, 256);
+      case 1: return persistent_grid(k_sweep_x_f32_batch<2, 8, 1>, 512);
+      case 2: return persistent_grid(k_sweep_x_f32_batch<2, 16, 1>, 1024);
+      case 3: return persistent_grid(k_sweep_x_f32_batch<2, 4, 4>, 256);
+    }
+  } else {
+    switch (variant) {
+      case 0: return persistent_grid(k_sweep_y_f32_batch<2, 2>, 128);
+      case 1: return persistent_grid(k_sweep_y_f32_batch<2, 4>, 256);
+      case 2: return persistent_grid(k_sweep_y_f32_batch<4, 2>, 128);
+      case 3: return persistent_grid(k_sweep_y_f32_batch<4, 4>, 256);
+    }
+  }
+  return 0;
+}
+
+int f32_items_x(const DevGrid& g, int variant) {
+  const int nc = kF32VariantsX[variant][1] * kF32VariantsX[variant][2];
+  return f32_windows(g.imax, kF32VariantsX[variant][0]) * ((g.jmax + nc - 1) / nc);
+}
+
+hipError_t f32_sweep_x_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
+                             int variant, int src, int grid) {
+  switch (variant) {
+    case 0: return sweep_x_batch_k<2, 4, 2>(s, dd, dfs, base, ndisk, src, grid);
+    case 1: return sweep_x_batch_k<2, 8, 1>(s, dd, dfs, base, ndisk, src, grid);
+    case 2: return sweep_x_batch_k<2, 16, 1>(s, dd, dfs, base, ndisk, src, grid);
+    case 3: return sweep_x_batch_k<2, 4, 4>(s, dd, dfs, base, ndisk, src, grid);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t f32_sweep_y_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
+                             int variant, int src, int grid) {
+  switch (variant) {
+    case 0: return sweep_y_batch_k<2, 2>(s, dd, dfs, base, ndisk, src, grid);
+    case 1: return sweep_y_batch_k<2, 4>(s, dd, dfs, base, ndisk, src, grid);
+    case 2: return sweep_y_batch_k<4, 2>(s, dd, dfs, base, ndisk, src, grid);
+    case 3: return sweep_y_batch_k<4, 4>(s, dd, dfs, base, ndisk, src, grid);
   }
   return hipErrorInvalidValue;
 }

@@ -19,6 +19,17 @@ struct LaunchCtx {
 
 constexpr int kMaxPartials = 8192;
 
+// one disk of a batch, as the batched kernels read it (an array of these in device memory)
+struct DiskDev {
+  DevGrid g;
+  DevParams p;
+  StepScalars* ss;
+  double* cflp;      // this disk's CFL partials
+  int ncfl_post;     // partials the post-step reduces into the next dt (fused CFL), or 0
+  float2* trig;      // FP32 per-step tables, or null
+  float* vrot;       // the FP32 table's vrot row (stride 1), or null
+};
+
 // FP64 path (pn_kernels_fp64.hip); each returns hipSuccess or the launch error
 hipError_t fp64_sweep_x(const LaunchCtx& c, int src);
 hipError_t fp64_sweep_y(const LaunchCtx& c, int src);
@@ -41,5 +52,9 @@ hipError_t launch_pre_step(const LaunchCtx& c, int ncfl, float2* trig, float* vr
 // one workgroup after the sweeps (plnt_flag < 2): roll, time and planet update, then dt
 // from ncfl partials when ncfl > 0
 hipError_t launch_post_step(const LaunchCtx& c, int ncfl);
+// the same for every disk of a batch, one workgroup per disk (pre-step with ncfl 0; post-step
+// with each disk's ncfl_post)
+hipError_t launch_pre_step_batch(hipStream_t s, const DiskDev* dd, int ndisk);
+hipError_t launch_post_step_batch(hipStream_t s, const DiskDev* dd, int ndisk);
 
 }  // namespace pn

@@ -15,39 +15,14 @@
 #include "pn_f32.hpp"
 #include "pn_host.hpp"
 #include "pn_launch.hpp"
+#include "pn_ctx.hpp"
 
 using namespace pn;
+using namespace pn::rt;
 
-struct pn_ctx {
-  pn_grid g{};
-  pn_params P{};
-  int precision = PN_FP64;
-  std::string err;
-  DevGrid dg{};
-  DevParams dp{};
-  DevF32 df{};                      // FP32 state and tables (FP32 contexts)
-  int var_x = 0, var_y = 3;         // FP32 sweep variants (indices into kF32VariantsX / Y)
-  bool tuned = false;
-  StepScalars* ss = nullptr;        // device
-  double* partials = nullptr;       // device
-  double* cflp = nullptr;           // device, CFL partial maxima
-  int ncflp = 0;
-  double* out2 = nullptr;           // device, totals
-  hipStream_t stream = nullptr;
-  std::vector<void*> allocs;
-  int cur = 0;                      // buffer holding the state
-  bool loaded = false;
-  // host copies of the per-ring values the device never changes (for packing)
-  std::vector<double> h_x, h_dx, h_xc, h_xvol, h_z, h_dz, h_zc, h_zvol, h_vbg, h_rbg, h_cs2;
-  std::vector<double> h_y, h_dy, h_yc, h_yvol;
-  // graph of one step (2-D) or two steps (3-D, the buffers alternate)
-  hipGraphExec_t gexec = nullptr;
-  // host copies of the fixed inputs, so a snapshot can be re-entered (restart)
-  std::vector<char> h_lft, h_rgh, h_udr, h_top;
-  std::vector<pn_symdisk> h_val;
-};
 
-namespace {
+namespace pn {
+namespace rt {
 
 int fail(pn_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -117,8 +92,6 @@ int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision)
   if (p->plnt_flag == 2 && p->ndim == 3) return fail(c, PN_ERR_UNSUPPORTED, "moving planet needs the 2-D force reductions");
   if (precision == PN_FP32 && (p->ndim != 2 || p->eos != 0))
     return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for the 2-D isothermal disk");
-  if (precision == PN_FP32 && p->visc_flag)
-    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built without viscosity");
   return PN_OK;
 }
 
@@ -193,26 +166,8 @@ int launch_step(pn_ctx* c, int& cur) {
   return PN_OK;
 }
 
-// mean time of nrep back-to-back launches, measured with HIP events on the context's stream
-template <class F>
-int event_time(pn_ctx* c, int nrep, double* ms, F&& launch) {
-  hipEvent_t e0, e1;
-  PN_HIP(c, hipEventCreate(&e0));
-  PN_HIP(c, hipEventCreate(&e1));
-  hipError_t e = hipEventRecord(e0, c->stream);
-  for (int k = 0; k < nrep && e == hipSuccess; ++k) e = launch();
-  if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
-  if (e == hipSuccess) e = hipEventSynchronize(e1);
-  float t = 0.0f;
-  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  if (e != hipSuccess) return fail(c, PN_ERR_HIP, "kernel timing: %s", hipGetErrorString(e));
-  *ms = static_cast<double>(t) / nrep;
-  return PN_OK;
-}
-
-}  // namespace
+}  // namespace rt
+}  // namespace pn
 
 extern "C" {
 
@@ -502,6 +457,26 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
       q[kTabIxa3] = 1.0 / (xa0 * xa0 * xa0);
       q[kTabVrot] = sres_v[s] / sradc[s];
       q[kTabDvol0] = sxvol[s];
+      // the viscosity's quadratic fit (viscosity.cu:2-19) through the left edges of s-1, s,
+      // s+1, applied to the backgrounds nu rho_b and Omega_b = L_b / r^2
+      q[kTabNu] = snu[s];
+      q[kTabDnb] = q[kTabD1Ob] = q[kTabD2Ob] = 0.0;
+      if (hp && hn) {
+        auto fit = [&](double um, double u0, double up, double& first, double& second) {
+          const double x2 = sx[s] - sx[s - 1], x3 = sx[s + 1] - sx[s - 1];
+          const double h2 = (u0 - um) / x2, h3 = (up - um) / x3;
+          const double A = (h3 - h2) / (x3 - x2), B = (x3 * h2 - x2 * h3) / (x3 - x2);
+          second = 2.0 * A;
+          first = second * x2 + B;
+        };
+        auto om = [&](long t) { return Lb[t] / (sradc[t] * sradc[t]); };
+        double d1, d2;
+        fit(snu[s - 1] * rbg[s - 1], snu[s] * rbg[s], snu[s + 1] * rbg[s + 1], d1, d2);
+        q[kTabDnb] = d1;
+        fit(om(s - 1), om(s), om(s + 1), d1, d2);
+        q[kTabD1Ob] = d1;
+        q[kTabD2Ob] = d2;
+      }
       for (int e = 0; e < kTabEntries; ++e) {
         tab[e * ts + s] = static_cast<float>(q[e]);
         if (s == nslot - 1)
@@ -717,7 +692,7 @@ int pn_time_kernels(pn_ctx* c, int nrep, double* ms6) {
   for (int q = 0; q < 6 && rc == PN_OK; ++q) {
     if (q == 4 && c->P.ndim != 3) continue;
     if (q == 5 && c->P.fargo_flag == 0) continue;
-    rc = event_time(c, nrep, &ms6[q], [&]() -> hipError_t {
+    rc = event_time(c, c->stream, nrep, &ms6[q], [&]() -> hipError_t {
       hipError_t e = hipSuccess;
       switch (q) {
         case 0:
@@ -763,8 +738,8 @@ int pn_tune(pn_ctx* c, int nrep, int* var_x, int* var_y) {
   int bx = c->var_x, by = c->var_y;
   for (int v = 0; v < kNumF32Variants; ++v) {
     double tx = 0.0, ty = 0.0;
-    int rc = event_time(c, nrep, &tx, [&]() { return f32_sweep_x(L, c->df, v, c->cur); });
-    if (rc == PN_OK) rc = event_time(c, nrep, &ty, [&]() { return f32_sweep_y(L, c->df, v, c->cur, 0); });
+    int rc = event_time(c, c->stream, nrep, &tx, [&]() { return f32_sweep_x(L, c->df, v, c->cur); });
+    if (rc == PN_OK) rc = event_time(c, c->stream, nrep, &ty, [&]() { return f32_sweep_y(L, c->df, v, c->cur, 0); });
     if (rc != PN_OK) return rc;
     if (tx < best_x) { best_x = tx; bx = v; }
     if (ty < best_y) { best_y = ty; by = v; }

@@ -260,7 +260,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, int
   // interface c: left state = right edge of cell c-1, right state = left edge of cell c, as
   // relative jumps against the backgrounds of cell c-1
   C<float, K> epl, epr, erl, err, ul, ur;
-  {
+  if constexpr (AX == 0) {
     C<float, K> pp, re, le;
 #pragma unroll
     for (int h = 0; h < K; ++h) pp[h] = W.r[h] * W.cs2[h];
@@ -303,7 +303,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, int
   //   t1 = (sqrt(pl/rl) + sqrt(pr/rr))/2, t2 = -(ur-ul)/(2 t1), p* = sqrt(pl pr) e^t2,
   //   u* = ul - t1 log(p*/pl); here with pl = p_b(1+epl), ..., p* = p_b(1+em)
   C<float, K> umid, em;
-  {
+  if constexpr (AX == 0) {
     C<float, K> csm = prev1(W.cs);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
@@ -313,6 +313,18 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, int
       const float t2 = -0.5f * (ur[h] - ul[h]) / t1;
       em[h] = expm1_acc(0.5f * (lpl + lpr) + t2);
       umid[h] = ul[h] - t1 * (0.5f * (lpr - lpl) + t2);
+    }
+  } else {
+    // along a ring c_s^2 is one constant, P = c_s^2 Sigma in every cell, and every step of
+    // the reconstruction is homogeneous of degree one in the values: the pressure states are
+    // c_s^2 times the density states, so eps_p = eps_rho and t1 = c_s exactly
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      const float lrl = log1p_acc(erl[h]), lrr = log1p_acc(err[h]);
+      const float t1 = W.cs[h];
+      const float t2 = -0.5f * (ur[h] - ul[h]) / t1;
+      em[h] = expm1_acc(0.5f * (lrl + lrr) + t2);
+      umid[h] = ul[h] - t1 * (0.5f * (lrr - lrl) + t2);
     }
   }
 
@@ -501,6 +513,74 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, int
     }
   } else {
     remap_plain(W.v);
+  }
+}
+
+// Explicit viscosity after the radial remap (device_viscosity_r, viscosity.cu:370-421) on
+// the perturbation state: n = nu rho and Omega = l / R^2 are background + deviation, and
+// the quadratic-fit derivatives are linear in the values, so each derivative is the
+// background's (from the table, FP64 rounded once) plus the deviation's (FP32, from the
+// cell widths: the left edges of c-1, c, c+1 sit at 0, dx(c-1), dx(c-1) + dx(c)).
+// l_new = l + dt R ddv / rho leaves L_b in place: dl += dt R ddv / rho.
+template <int K>
+__device__ __forceinline__ void visc_r_pert(WinP<K>& W, const TabLane<K>& tx, int lim, float dt) {
+  const C<float, K> nu = tx.get(kTabNu), dnb = tx.get(kTabDnb), d1ob = tx.get(kTabD1Ob), d2ob = tx.get(kTabD2Ob),
+                    ixc = tx.get(kTabIxc);
+  C<float, K> np_, om;
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    np_[h] = nu[h] * W.r[h];
+    om[h] = W.v[h] * (ixc[h] * ixc[h]);
+  }
+  C<float, K> wm = prev1(W.dx0);
+  C<float, K> nm = prev1(np_), npp = next1(np_), um = prev1(W.u), up = next1(W.u), omm = prev1(om), omp = next1(om);
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    const int c = cell_of(K, h);
+    const float x2 = wm[h], x3 = wm[h] + W.dx0[h], R = W.rad[h];
+    const float rho = W.rb[h] + W.r[h], n = nu[h] * rho;
+    float dn, d1, d2, e1, e2;
+    derivs<float>(0.0f, x2, x3, nm[h], np_[h], npp[h], dn, d2);
+    dn += dnb[h];
+    derivs<float>(0.0f, x2, x3, um[h], W.u[h], up[h], d1, d2);
+    const float ddu = 2.0f * n * d2 + 2.0f * dn * d1 + 2.0f * n * d1 / R - 2.0f * n * W.u[h] / R / R;
+    derivs<float>(0.0f, x2, x3, omm[h], om[h], omp[h], e1, e2);
+    e1 += d1ob[h];
+    e2 += d2ob[h];
+    const float ddv = R * n * e2 + R * dn * e1 + 3.0f * n * e1;
+    if (c >= kNpad && c < lim - kNpad) {
+      W.u[h] = W.u[h] + dt * ddu / rho;
+      W.v[h] = W.v[h] + dt * R * ddv / rho;
+    }
+  }
+}
+
+// device_viscosity_p (viscosity.cu:423-472) on the azimuthal sweep's state: sweep u =
+// v_phi - v_bg and sweep v = v_r.  nu and the backgrounds are uniform along the ring, so
+// only deviations enter the derivatives (in the cell's angle coordinates).
+template <int K>
+__device__ __forceinline__ void visc_p_pert(WinP<K>& W, int lim, float dt, float nu) {
+  C<float, K> np_;
+#pragma unroll
+  for (int h = 0; h < K; ++h) np_[h] = nu * W.r[h];
+  C<float, K> wm = prev1(W.dx0);
+  C<float, K> nm = prev1(np_), npp = next1(np_);
+  C<float, K> vm = prev1(W.u), vp = next1(W.u), rm = prev1(W.v), rp = next1(W.v);
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    const int c = cell_of(K, h);
+    const float x2 = wm[h], x3 = wm[h] + W.dx0[h];
+    const float rho = W.rb[h] + W.r[h], n = nu * rho;
+    float dn, d1, d2;
+    derivs<float>(0.0f, x2, x3, nm[h], np_[h], npp[h], dn, d2);
+    derivs<float>(0.0f, x2, x3, vm[h], W.u[h], vp[h], d1, d2);
+    const float ddv = 2.0f * n * d2 + 2.0f * dn * d1;
+    derivs<float>(0.0f, x2, x3, rm[h], W.v[h], rp[h], d1, d2);
+    const float ddu = n * d2 + dn * d1;
+    if (c >= kNpad && c < lim - kNpad) {
+      W.u[h] = W.u[h] + dt * ddv / rho;
+      W.v[h] = W.v[h] + dt * ddu / rho;
+    }
   }
 }
 

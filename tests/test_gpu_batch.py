@@ -1,0 +1,113 @@
+"""Batched FP32 disks (pn_batch_*) against single-disk runs, through the C ABI.
+
+A batch runs the same per-disk arithmetic as a single FP32 context, with the work of all
+disks in one launch per sweep; with the sweep shapes pinned to the same variants every
+disk must come out as its own single-context run, and close to its own FP64 run (relative
+L1 in the lab frame, tolerances written next to each check).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from penguin_2_0_amd import abi, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def lab(state, f):
+    a = state[f]
+    return np.stack([np.roll(a[i], int(state["rot_j"][i])) for i in range(a.shape[0])])
+
+
+def rel(a, b):
+    d = np.abs(a - b)
+    return d.sum() / np.abs(b).sum(), d.max() / np.abs(b).max()
+
+
+def members():
+    """Uneven disks like the parameter sweep: different grids, planet masses, viscosities."""
+    out = []
+    for d, (nr, q, alpha) in enumerate([(48, 1e-5, 1e-4), (64, 3e-4, 1e-2), (96, 1e-3, 3e-3), (130, 1e-4, 1e-3),
+                                        (48, 1e-4, 0.0)]):
+        over = dict(visc_flag=1, ss_alpha=alpha, vis_nu=alpha * 0.05 * 0.05) if alpha > 0 else {}
+        w = configs.small(nr, 3 * nr, M_p=q, **over)
+        w.seed = 100 + d
+        out.append(w)
+    return out
+
+
+def run_batch(ws, n, variants=(0, 3), graph=True):
+    disks = [abi.init_disk(w.grid, w.params, w.seed, w.noise) for w in ws]
+    with abi.Batch([w.grid for w in ws], [w.params for w in ws]) as b:
+        for ctx, d in zip(b.disks, disks):
+            ctx.load(d)
+        b.set_variants(*variants)
+        st = b.run_begin()
+        b.run(n, st, graph=graph)
+        return [ctx.store() for ctx in b.disks], [st[d] for d in range(len(ws))]
+
+
+def run_single(w, prec, n, variants=(0, 3)):
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    with abi.Context(w.grid, w.params, prec) as ctx:
+        ctx.load(d)
+        if prec == abi.PN_FP32:
+            ctx.set_variants(*variants)
+        st = ctx.run_begin()
+        ctx.run(n, st)
+        return ctx.store(), st
+
+
+def test_batch_equals_single_disks():
+    ws = members()
+    n = 30
+    say("case: batch of", len(ws), "disks,", n, "steps")
+    outs, sts = run_batch(ws, n)
+    for w, o, s in zip(ws, outs, sts):
+        ref, rs = run_single(w, abi.PN_FP32, n)
+        say(f"  {w.name}: dt {s.dt!r} vs {rs.dt!r}")
+        assert s.dt == rs.dt and s.simtime == rs.simtime and s.FrRot == rs.FrRot
+        assert np.array_equal(o["rot_j"], ref["rot_j"])
+        for f in ("r", "u", "v"):
+            l1, li = rel(o[f], ref[f])
+            assert li <= 1e-12, (w.name, f, l1, li)
+
+
+def test_batch_vs_fp64():
+    ws = members()
+    n = 100
+    say("case: batch vs fp64,", n, "steps")
+    outs, sts = run_batch(ws, n)
+    for w, o, s in zip(ws, outs, sts):
+        ref, rs = run_single(w, abi.PN_FP64, n)
+        assert np.array_equal(o["rot_j"], ref["rot_j"])
+        l1, li = rel(lab(o, "r"), lab(ref, "r"))
+        say(f"  {w.name}: rho L1 {l1:.3e} Linf {li:.3e}; dt {s.dt:.9e} vs {rs.dt:.9e}")
+        assert l1 <= 1e-6 and li <= 1e-4
+        assert abs(s.dt - rs.dt) <= 1e-6 * rs.dt
+
+
+def test_batch_graph_equals_eager_bitwise():
+    ws = members()[:3]
+    say("case: batch graph vs eager")
+    a, sa = run_batch(ws, 6, graph=False)
+    b, sb = run_batch(ws, 6, graph=True)
+    for x, y, s, t in zip(a, b, sa, sb):
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+        assert s.dt == t.dt and s.simtime == t.simtime
+
+
+@pytest.mark.parametrize("variants", [(1, 0), (2, 1), (3, 2)])
+def test_batch_variants(variants):
+    ws = members()[:3]
+    say("case: batch variants", variants)
+    outs, _ = run_batch(ws, 10, variants=variants)
+    for w, o in zip(ws, outs):
+        ref, _ = run_single(w, abi.PN_FP32, 10, variants=variants)
+        for f in ("r", "u", "v"):
+            assert rel(o[f], ref[f])[1] <= 1e-12, (w.name, f)

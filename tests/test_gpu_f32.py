@@ -79,6 +79,27 @@ def test_f32_vs_fp64(w, n, tol):
     assert l1 <= 1e-4 and li <= 1e-2
 
 
+def visc_disk(nr, nphi, alpha):
+    w = configs.small(nr, nphi, visc_flag=1, ss_alpha=alpha, vis_nu=alpha * 0.05 * 0.05)
+    w.name += f"_visc{alpha:g}"
+    return w
+
+
+@pytest.mark.parametrize("w", [visc_disk(130, 250, 1e-2), visc_disk(256, 768, 1e-3)], ids=lambda w: w.name)
+def test_f32_viscous_vs_fp64(w):
+    """Explicit viscosity in perturbation form tracks the FP64 viscous update."""
+    n = 100
+    say("case: fp32 viscous vs fp64", w.name, n, "steps")
+    a, sa = run(w, abi.PN_FP32, n)
+    b, sb = run(w, abi.PN_FP64, n)
+    assert np.array_equal(a["rot_j"], b["rot_j"])
+    assert abs(sa.dt - sb.dt) <= 1e-6 * sb.dt
+    for f, t1, tinf in (("r", 1e-6, 1e-4), ("v", 1e-6, 1e-5), ("u", 1e-4, 1e-2)):
+        l1, li = rel(lab(a, f), lab(b, f))
+        say(f"  {f}: L1 {l1:.3e} Linf {li:.3e}")
+        assert l1 <= t1 and li <= tinf, (f, l1, li)
+
+
 def test_f32_every_variant():
     """Every (cells per lane, waves, columns) variant of both sweeps is a valid step."""
     w = configs.small(130, 250)

@@ -1,0 +1,125 @@
+"""FP64 HIP path against the FP64 CPU oracle for the flag combinations beyond the plain
+isothermal disk: viscosity, the isentropic (EOS 1) and adiabatic (EOS 2) closures, a planet
+that moves under the disk's torque, FARGO frame mode 2, a non-rotating frame, no
+flattening, a growing planet, and the 3-D r-phi-theta disk with its theta sweep.
+
+Bars as in test_gpu_parity.py: integers (rot_j) bit-exact, reals within the written
+relative L1 / Linf tolerances after each single sweep and after 10 full steps.  The
+isentropic Riemann solver iterates to a relative tolerance of 1e-8 (riemann.cu:73-108), so
+its bars are set from that tolerance, not from ulps.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from penguin_2_0_amd import abi, configs
+
+pytestmark = pytest.mark.gpu
+
+HPI = 1.570796326794896619231321691639751442
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    d = np.abs(a - b)
+    s1, si = np.abs(b).sum(), np.abs(b).max()
+    return (d.sum() / s1 if s1 else d.sum()), (d.max() / si if si else d.max())
+
+
+def disk3d(nr, nphi, nth):
+    p = configs.disk2d(nr, nphi, q=3.0e-5).replace(
+        ndim=3, ngeomx=2, ngeomy=4, ngeomz=5, p_alpha=1.5, p_beta=1.0, zmin=HPI - 0.15, zmax=HPI + 0.15)
+    return configs.Workload(f"disk3d_{nr}x{nphi}x{nth}", abi.PnGrid(nr, nphi, nth), p, seed=11, steps=10)
+
+
+def visc(nr=48, nphi=144):
+    a = 3.0e-3
+    w = configs.small(nr, nphi, visc_flag=1, ss_alpha=a, vis_nu=a * 0.05 * 0.05)
+    w.name += "_visc"
+    return w
+
+
+def named(w, suffix):
+    w.name += suffix
+    return w
+
+
+# (workload, fields compared, single-sweep L1/Linf, 10-step L1/Linf)
+CASES = [
+    (visc(), ("r", "u", "v"), (1e-13, 1e-11), (1e-11, 1e-8)),
+    (visc(130, 250), ("r", "u", "v"), (1e-13, 1e-11), (1e-11, 1e-8)),
+    (named(configs.small(48, 144, eos=2, gam=1.4, init_p_iso=1), "_eos2"), ("r", "p", "u", "v"),
+     (1e-13, 1e-11), (1e-11, 1e-8)),
+    (named(configs.small(48, 144, eos=2, gam=1.4), "_eos2_adiabatic_ic"), ("r", "p", "u", "v"),
+     (1e-13, 1e-11), (1e-11, 1e-8)),
+    (named(configs.small(48, 144, eos=1, gam=1.4), "_eos1"), ("r", "p", "u", "v"), (1e-9, 1e-7), (1e-8, 1e-6)),
+    (named(configs.small(48, 144, plnt_flag=2, M_p=1e-3), "_moving_planet"), ("r", "u", "v"),
+     (1e-13, 1e-11), (1e-11, 1e-8)),
+    (named(configs.small(48, 144, fargo_flag=2), "_fargo2"), ("r", "u", "v"), (1e-13, 1e-11), (1e-11, 1e-8)),
+    (named(configs.small(48, 144, frrot_flag=0), "_inertial"), ("r", "u", "v"), (1e-13, 1e-11), (1e-11, 1e-8)),
+    (named(configs.small(48, 144, flat_flag=0), "_noflat"), ("r", "u", "v"), (1e-13, 1e-11), (1e-11, 1e-8)),
+    (named(configs.small(48, 144, t_growth=5.0), "_growing_planet"), ("r", "u", "v"), (1e-13, 1e-11),
+     (1e-11, 1e-8)),
+    (disk3d(24, 72, 12), ("r", "u", "v", "w"), (1e-13, 1e-11), (1e-11, 1e-8)),
+]
+
+
+def pair(w):
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    o = oracle.OracleDisk(w.grid, w.params, w.seed, w.noise)
+    assert np.array_equal(d.rings.view(np.uint8), o.rings.view(np.uint8))
+    if d.udr is not None:
+        assert np.array_equal(d.udr.view(np.uint8), o.udr.view(np.uint8))
+        assert np.array_equal(d.top.view(np.uint8), o.top.view(np.uint8))
+    ctx = abi.Context(w.grid, w.params, abi.PN_FP64)
+    ctx.load(d)
+    return ctx, o
+
+
+def check(g, o, fields, tol, what):
+    for f in fields:
+        l1, li = rel(g[f], o[f])
+        say(f"  {what} {f}: L1 {l1:.3e}  Linf {li:.3e}")
+        assert l1 <= tol[0] and li <= tol[1], (what, f, l1, li)
+    assert np.array_equal(g["rot_j"], o["rot_j"])
+
+
+@pytest.mark.parametrize("w,fields,tol1,tol10", CASES, ids=[c[0].name for c in CASES])
+def test_sweeps_and_steps(w, fields, tol1, tol10):
+    say("case:", w.name)
+    ctx, o = pair(w)
+    st = o.run_begin()
+    axes = "xyz" if w.params.ndim == 3 else "xy"
+    with ctx:
+        gst = ctx.run_begin()
+        assert gst.FrRot == st.FrRot
+        dt = o.get_dt(st.dt, st.FrRot)
+        dt_g = ctx.get_dt(st.dt, st.FrRot)
+        say(f"  dt oracle {dt!r} gpu {dt_g!r}")
+        assert abs(dt_g - dt) <= 1e-14 * dt
+        o.index_shift(dt, st.FrRot)
+        ctx.index_shift(dt, st.FrRot)
+        for ax in axes:
+            getattr(o, "sweep" + ax)(dt, st.planet, st.FrRot)
+            ctx.sweep_axis(axes.index(ax), dt, st.planet, st.FrRot)
+            check(ctx.store(), o.rings, fields, tol1, f"sweep {ax}")
+    ctx, o = pair(w)
+    st = o.run_begin()
+    o.run(10, st)
+    with ctx:
+        gst = ctx.run_begin()
+        ctx.run(10, gst, graph=True)
+        g = ctx.store()
+    say(f"  10 steps: dt {gst.dt!r} vs {st.dt!r}; planet x {gst.planet.x!r} vs {st.planet.x!r}")
+    assert abs(gst.dt - st.dt) <= 1e-10 * st.dt
+    assert abs(gst.simtime - st.simtime) <= 1e-10 * st.simtime
+    assert abs(gst.planet.x - st.planet.x) <= 1e-12 and abs(gst.planet.y - st.planet.y) <= 1e-12
+    assert abs(gst.FrRot - st.FrRot) <= 1e-12 * max(abs(st.FrRot), 1.0)
+    check(g, o.rings, fields, tol10, "10 steps")
