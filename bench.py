@@ -10,7 +10,9 @@ the K timed steps of the measured path, with the state resident in HBM.
 With the default FP32 path the plain FP64 path is run beside it on the same initial state
 for the same W + K steps: its throughput gives the speedup, and its final state is the
 reference for the relative L1 / Linf errors of Sigma, v_r and v_phi (compared in the lab
-frame) and for the mass and angular-momentum drift.
+frame) and for the mass and angular-momentum drift.  The "batched" object times the
+parameter sweep of 48 uneven viscous disks (N_r 128..512) advanced as one FP32 batch against
+the same disks run one by one on the plain FP64 path, with the worst disk's error.
 
 The roofline entry prices the dominant sweep kernel of the measured path -- its algorithmic
 bytes per launch (one read and one write of the 3 state fields of every cell) over its
@@ -76,6 +78,46 @@ def timed_run(w: configs.Workload, prec: int, steps: int, warmup: int) -> dict:
             "dmass": (m1 - m0) / m0, "dangmom": (l1 - l0) / l0}
 
 
+def batched_run(steps: int, warmup: int) -> dict:
+    """The parameter sweep of 48 uneven viscous disks: the FP32 batch (one launch per sweep
+    for all disks) against every disk's plain FP64 context run on its own, same steps."""
+    ws = configs.c4_members()
+    cells = sum(w.cells for w in ws)
+    disks = [abi.init_disk(w.grid, w.params, w.seed, w.noise) for w in ws]
+    with abi.Batch([w.grid for w in ws], [w.params for w in ws]) as b:
+        for ctx, d in zip(b.disks, disks):
+            ctx.load(d)
+        st = b.run_begin()
+        b.run(warmup, st)
+        t0 = time.perf_counter()
+        b.run(steps, st)
+        wall = time.perf_counter() - t0
+        out32 = [ctx.store() for ctx in b.disks]
+        km = b.time_kernels(5)
+    wall64, worst = 0.0, {"l1_rho": 0.0, "linf_rho": 0.0, "l1_vphi": 0.0, "disk": None}
+    rot_equal = True
+    for w, d, o in zip(ws, disks, out32):
+        with abi.Context(w.grid, w.params, abi.PN_FP64) as ctx:
+            ctx.load(d)
+            s = ctx.run_begin()
+            ctx.run(warmup, s)
+            t0 = time.perf_counter()
+            ctx.run(steps, s)
+            wall64 += time.perf_counter() - t0
+            ref = ctx.store()
+        rot_equal &= bool(np.array_equal(o["rot_j"], ref["rot_j"]))
+        l1, li = rel_err(lab(o, "r"), lab(ref, "r"))
+        lv = rel_err(lab(o, "v"), lab(ref, "v"))[0]
+        if l1 > worst["l1_rho"]:
+            worst.update(l1_rho=l1, linf_rho=li, l1_vphi=lv, disk=w.name)
+    value = cells * steps / wall
+    v64 = cells * steps / wall64
+    return {"workload": f"c4_batch_{len(ws)}_disks", "cells": cells, "steps": steps, "warmup": warmup,
+            "value": value, "unit": "zone-updates/s", "ms_per_step": wall / steps * 1e3, "kernels_ms": km,
+            "fp64_plain_value": v64, "speedup_vs_fp64_plain": value / v64,
+            "accuracy_vs_fp64_worst_disk": dict(worst, after_steps=warmup + steps, rot_j_equal=rot_equal)}
+
+
 def lab(state: np.ndarray, f: str) -> np.ndarray:
     """Field f with each ring's data rolled to lab columns (data j sits at j + rot_j)."""
     a = state[f]
@@ -96,6 +138,8 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 leg of an FP32 run")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched parameter-sweep leg")
+    ap.add_argument("--batch-steps", type=int, default=1000)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported (the disk is not "
@@ -148,6 +192,8 @@ def main(argv=None) -> int:
             "rot_j_equal": bool(np.array_equal(run["state"]["rot_j"], ref["state"]["rot_j"])),
             "dt_rel_diff": abs(run["st"].dt - ref["st"].dt) / ref["st"].dt,
         }
+    if prec == abi.PN_FP32 and not a.no_batch:
+        out["batched"] = batched_run(a.batch_steps, a.warmup)
     if rank == 0 and not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
     if rank == 0:

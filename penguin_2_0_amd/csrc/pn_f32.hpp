@@ -12,7 +12,8 @@ namespace pn {
 //   c_s^2, c_s, L_b, L_b(s+1) - L_b(s),
 //   res_bg, a_bg, dp_bm = p_b(s) - p_b(s-1), f_face,
 //   L_f = (L_b(s) + L_b(s-1))/2, 1/xa0^3, vrot = res_v / r (rewritten each step), dvol0,
-//   nu, (nu rho_b)', Omega_b', Omega_b''
+//   nu, (nu rho_b)', Omega_b', Omega_b'', and c0..c4 of paraset (parabola.cu:236-274) on the
+//   slot widths
 // with, in FP64: a_bg = dp_bm V0 / (rho_b dvol0), V0 = (xa0(s) + xa0(s+1))/2, the pressure
 // acceleration of the background; res_bg = -1/rad^2 + L_b^2/rad^3 - a_bg, the background's
 // net radial acceleration; f_face = -1/xa0^2 + L_f^2/xa0^3, the force at the left edge;
@@ -27,6 +28,7 @@ enum TabEntry {
   kTabResBg, kTabAbg, kTabDpbm, kTabFface,
   kTabLf, kTabIxa3, kTabVrot, kTabDvol0,
   kTabNu, kTabDnb, kTabD1Ob, kTabD2Ob,
+  kTabP0, kTabP1, kTabP2, kTabP3, kTabP4,   // paraset coefficients of the Eulerian radial grid
   kTabEntries
 };
 constexpr int kTabOverhang = 256;   // >= the longest radial window
@@ -41,6 +43,7 @@ struct DevF32 {
   const float *dy, *yvol;      // per column
   float2* trig;                // per column, per step: sin / cos of half the angle to the planet
   int nwin_x, nwin_y;          // windows of the variant in use
+  int uniform_phi;             // every dphi equal: the azimuthal paraset is (1/2, -1/6, 1/6, 1/2, 1/2)
 };
 
 constexpr int kNumF32Variants = 4;

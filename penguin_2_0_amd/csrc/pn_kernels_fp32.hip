@@ -161,7 +161,13 @@ __device__ __forceinline__ void x_item(const DevGrid& g, const DevF32& f, const 
     const C<float, K> vp = lds_k<K>(&tile[2][q][K * lane]);
 #pragma unroll
     for (int h = 0; h < K; ++h) Wp.v[h] = Wp.rad[h] * vp[h];   // l - L_b = r (v_phi - v_bg); r_cyl = r in 2-D
-    ppmlr_pert<K, 0>(Wp, tx, lim, dt, 0.0f, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
+    Para<float, K> P0;
+    P0.c0 = tx.get(kTabP0);
+    P0.c1 = tx.get(kTabP1);
+    P0.c2 = tx.get(kTabP2);
+    P0.c3 = tx.get(kTabP3);
+    P0.c4 = tx.get(kTabP4);
+    ppmlr_pert<K, 0>(Wp, tx, P0, lim, dt, 0.0f, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
     if (VISC == 1 || (VISC == 2 && prm.visc)) visc_r_pert<K>(Wp, tx, lim, dt);
     const C<float, K> ixc = tx.get(kTabIxc);
 #pragma unroll
@@ -296,7 +302,8 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
     W.v[h] = fu[j];   // v_r
   }
   const float sqrt_gam = sqrtf((float)prm.gam);
-  ppmlr_pert<K, 1>(W, TabLane<K>{nullptr, 0, 0}, lim, dt, wres, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
+  const Para<float, K> P0 = f.uniform_phi ? paraset_uniform<K>() : paraset(0, lim, W.dx0);
+  ppmlr_pert<K, 1>(W, TabLane<K>{nullptr, 0, 0}, P0, lim, dt, wres, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
   if (VISC == 1 || (VISC == 2 && prm.visc)) visc_p_pert<K>(W, lim, dt, T[kTabNu * ts]);
 
   float* __restrict__ dr = f.f[src ^ 1][0] + base;
@@ -497,8 +504,9 @@ hipError_t sweep_x_k(const LaunchCtx& c, const DevF32& f0, int src) {
   if (f.tstride < c.g.sr * c.g.sk + 64 * K) return hipErrorInvalidValue;
   const long ngrp = (c.g.jmax + W * Q - 1) / (W * Q);
   dim3 grid((unsigned)(ngrp * f.nwin_x)), block(64 * W);
-  if (c.p.visc) k_sweep_x_f32<K, W, Q, 1><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
-  else k_sweep_x_f32<K, W, Q, 0><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+  // the same body as the batched kernels (viscosity switched by the parameters), so a disk
+  // computes the same bits alone or in a batch
+  k_sweep_x_f32<K, W, Q, 2><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
   return hipGetLastError();
 }
 
@@ -510,8 +518,7 @@ hipError_t sweep_y_k(const LaunchCtx& c, const DevF32& f0, int src, int do_cfl) 
   const long nblk = (waves + WPB - 1) / WPB;
   if (do_cfl && nblk > c.ncflp) return hipErrorInvalidValue;
   dim3 grid((unsigned)nblk), block(64 * WPB);
-  if (c.p.visc) k_sweep_y_f32<K, WPB, 1><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
-  else k_sweep_y_f32<K, WPB, 0><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  k_sweep_y_f32<K, WPB, 2><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
   return hipGetLastError();
 }
 

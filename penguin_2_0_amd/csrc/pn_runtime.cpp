@@ -477,6 +477,21 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
         q[kTabD1Ob] = d1;
         q[kTabD2Ob] = d2;
       }
+      // paraset (parabola.cu:236-274) on the slot widths; neighbours past the grid repeat
+      // the last width (those coefficients never reach a kept parabola)
+      {
+        auto w = [&](long t) { return sdx[std::min(std::max(t, 0L), nslot - 1)]; };
+        auto an = [&](long t) { return w(t) + w(t + 1); };
+        auto bn = [&](long t) { return an(t) + w(t); };
+        auto cc = [&](long t) { return an(t) + w(t + 1); };
+        const double d = 1.0 / (an(s - 1) + an(s + 1));
+        q[kTabP0] = w(s) / an(s) * (1.0 + 2.0 * d * w(s + 1) * (an(s - 1) / bn(s) - an(s + 1) / cc(s)));
+        q[kTabP1] = -d * w(s) * an(s - 1) / bn(s);
+        q[kTabP2] = d * w(s + 1) * an(s + 1) / cc(s);
+        const double e = w(s) / (an(s - 1) + w(s + 1));
+        q[kTabP3] = e * bn(s - 1) / an(s);
+        q[kTabP4] = e * cc(s) / an(s - 1);
+      }
       for (int e = 0; e < kTabEntries; ++e) {
         tab[e * ts + s] = static_cast<float>(q[e]);
         if (s == nslot - 1)
@@ -495,6 +510,9 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
     PN_HIP(c, hipMemcpyAsync(const_cast<double*>(c->df.vbg), svbg.data(), nslot * sizeof(double), hipMemcpyHostToDevice, st));
     c->df.nwin_x = f32_windows(I, kF32VariantsX[c->var_x][0]);
     c->df.nwin_y = f32_windows(J, kF32VariantsY[c->var_y][0]);
+    c->df.uniform_phi = 1;
+    for (int j = 1; j < J; ++j)
+      if (c->h_dy[j] != c->h_dy[0]) c->df.uniform_phi = 0;
     // the host vectors above must outlive the copies
     PN_HIP(c, hipStreamSynchronize(st));
   }

@@ -63,18 +63,29 @@ __device__ __forceinline__ float planet_fphi(const PlanetF& p, float rad, float 
 }
 
 // log(1 + x): 2 atanh(x / (2 + x)) as an odd series for |x| < 1/4, else the library log
+// (both evaluated, one selected: no divergent branch)
 __device__ __forceinline__ float log1p_acc(float x) {
-  if (fabsf(x) < 0.25f) {
-    const float z = x / (2.0f + x), z2 = z * z;
-    return 2.0f * z * (1.0f + z2 * (1.0f / 3.0f + z2 * (1.0f / 5.0f + z2 * (1.0f / 7.0f + z2 * (1.0f / 9.0f)))));
-  }
-  return __logf(1.0f + x);
+  const float z = x / (2.0f + x), z2 = z * z;
+  const float s = 2.0f * z * (1.0f + z2 * (1.0f / 3.0f + z2 * (1.0f / 5.0f + z2 * (1.0f / 7.0f + z2 * (1.0f / 9.0f)))));
+  return fabsf(x) < 0.25f ? s : __logf(1.0f + x);
 }
 // exp(y) - 1: Taylor series to y^7 for |y| < 1/4, else the library exp
 __device__ __forceinline__ float expm1_acc(float y) {
-  if (fabsf(y) < 0.25f)
-    return y * (1.0f + y * (0.5f + y * (1.0f / 6.0f + y * (1.0f / 24.0f + y * (1.0f / 120.0f + y * (1.0f / 720.0f + y * (1.0f / 5040.0f)))))));
-  return __expf(y) - 1.0f;
+  const float s =
+      y * (1.0f + y * (0.5f + y * (1.0f / 6.0f + y * (1.0f / 24.0f + y * (1.0f / 120.0f + y * (1.0f / 720.0f + y * (1.0f / 5040.0f)))))));
+  return fabsf(y) < 0.25f ? s : __expf(y) - 1.0f;
+}
+
+// paraset of a uniform grid (CW84 eq. 1.6 with equal widths): constants
+template <int K>
+__device__ __forceinline__ Para<float, K> paraset_uniform() {
+  Para<float, K> P;
+  P.c0 = splat<float, K>(0.5f);
+  P.c1 = splat<float, K>(-1.0f / 6.0f);
+  P.c2 = splat<float, K>(1.0f / 6.0f);
+  P.c3 = splat<float, K>(0.5f);
+  P.c4 = splat<float, K>(0.5f);
+  return P;
 }
 
 // parabola of X = X_b + x in each cell's frame (parabola.cu:168-231): val = x (the cell's
@@ -169,11 +180,14 @@ struct TabLane {
 };
 
 // AX 0 radial (cylindrical), AX 1 azimuthal.  tx: the per-slot table of this lane's cells
-// (radial sweep only; the radial balance terms are read where they are used).  wres: the
-// azimuthal grid's uniform angular rate.  On return cells [6, nmax-6) hold the update.
+// (radial sweep only; the radial balance terms are read where they are used).  P0: paraset
+// of the Eulerian grid (its values outside the window's paraset ranges never reach a
+// parabola the sweep keeps).  wres: the azimuthal grid's uniform angular rate.  On return
+// cells [6, nmax-6) hold the update.
 template <int K, int AX>
-__device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, int nmax, float dt, float wres,
-                                           const PlanetF& pl, bool planet, bool flat_on, float sqrt_gam) {
+__device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, const Para<float, K>& P0, int nmax,
+                                           float dt, float wres, const PlanetF& pl, bool planet, bool flat_on,
+                                           float sqrt_gam) {
   const float hdt = 0.5f * dt;
   const float fourthd = 1.3333333333333333f;
   const float smallr = 1.0e-15f;
@@ -211,7 +225,6 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, int
   }
 
   // ---- states (states.cu:3-73)
-  Para<float, K> P0 = paraset(0, nmax, W.dx0);
   C<float, K> fhalf;   // hdt * force at the left edge
   if constexpr (AX == 0) {
     C<float, K> vm = prev1(W.v);

@@ -89,7 +89,7 @@ def test_batch_vs_fp64():
         l1, li = rel(lab(o, "r"), lab(ref, "r"))
         say(f"  {w.name}: rho L1 {l1:.3e} Linf {li:.3e}; dt {s.dt:.9e} vs {rs.dt:.9e}")
         assert l1 <= 1e-6 and li <= 1e-4
-        assert abs(s.dt - rs.dt) <= 1e-6 * rs.dt
+        assert abs(s.dt - rs.dt) <= 1e-5 * rs.dt
 
 
 def test_batch_graph_equals_eager_bitwise():

@@ -93,7 +93,8 @@ def test_f32_viscous_vs_fp64(w):
     a, sa = run(w, abi.PN_FP32, n)
     b, sb = run(w, abi.PN_FP64, n)
     assert np.array_equal(a["rot_j"], b["rot_j"])
-    assert abs(sa.dt - sb.dt) <= 1e-6 * sb.dt
+    # dt follows the largest signal speed, i.e. the Linf of the state
+    assert abs(sa.dt - sb.dt) <= 1e-5 * sb.dt
     for f, t1, tinf in (("r", 1e-6, 1e-4), ("v", 1e-6, 1e-5), ("u", 1e-4, 1e-2)):
         l1, li = rel(lab(a, f), lab(b, f))
         say(f"  {f}: L1 {l1:.3e} Linf {li:.3e}")

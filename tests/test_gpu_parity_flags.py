@@ -55,10 +55,12 @@ def named(w, suffix):
 CASES = [
     (visc(), ("r", "u", "v"), (1e-13, 1e-11), (1e-11, 1e-8)),
     (visc(130, 250), ("r", "u", "v"), (1e-13, 1e-11), (1e-11, 1e-8)),
+    # EOS 2 evaluates pow() per interface (riemann.cu:55-71): the device and host pow differ in
+    # the last ulp, which v_r -- zero at start, judged against its own small scale -- shows most
     (named(configs.small(48, 144, eos=2, gam=1.4, init_p_iso=1), "_eos2"), ("r", "p", "u", "v"),
-     (1e-13, 1e-11), (1e-11, 1e-8)),
+     (1e-12, 1e-10), (1e-10, 1e-8)),
     (named(configs.small(48, 144, eos=2, gam=1.4), "_eos2_adiabatic_ic"), ("r", "p", "u", "v"),
-     (1e-13, 1e-11), (1e-11, 1e-8)),
+     (1e-12, 1e-10), (1e-10, 1e-8)),
     (named(configs.small(48, 144, eos=1, gam=1.4), "_eos1"), ("r", "p", "u", "v"), (1e-9, 1e-7), (1e-8, 1e-6)),
     (named(configs.small(48, 144, plnt_flag=2, M_p=1e-3), "_moving_planet"), ("r", "u", "v"),
      (1e-13, 1e-11), (1e-11, 1e-8)),
@@ -67,7 +69,10 @@ CASES = [
     (named(configs.small(48, 144, flat_flag=0), "_noflat"), ("r", "u", "v"), (1e-13, 1e-11), (1e-11, 1e-8)),
     (named(configs.small(48, 144, t_growth=5.0), "_growing_planet"), ("r", "u", "v"), (1e-13, 1e-11),
      (1e-11, 1e-8)),
-    (disk3d(24, 72, 12), ("r", "u", "v", "w"), (1e-13, 1e-11), (1e-11, 1e-8)),
+    # theta volumes are cos(x) - cos(x + d) for the small Lagrangian displacement d
+    # (evolve.cu:110-124, remap.cu:50-54): the last-ulp difference of the device and host cos
+    # is amplified by that cancellation, most visibly in v_theta, which starts at zero
+    (disk3d(24, 72, 12), ("r", "u", "v", "w"), (1e-12, 1e-9), (1e-10, 1e-7)),
 ]
 
 
