@@ -15,7 +15,8 @@ parameter sweep of 48 uneven viscous disks (N_r 128..512) advanced as one FP32 b
 the same disks run one by one on the plain FP64 path, with the worst disk's error.
 
 The roofline entry prices the dominant sweep kernel of the measured path -- its algorithmic
-bytes per launch (one read and one write of the 3 state fields of every cell) over its
+bytes per launch (one read and one write of the minimal state fields of every cell: 3 for
+the 2-D isothermal disk, one more in 3-D, one more for a non-isothermal EOS) over its
 mean launch time, measured with HIP events on the context's stream -- against HBM
 bandwidth.  The CPU baseline times the FP64 oracle on the host cores on a bounded prefix of
 the same workload.
@@ -94,9 +95,9 @@ def batched_run(steps: int, warmup: int) -> dict:
         wall = time.perf_counter() - t0
         out32 = [ctx.store() for ctx in b.disks]
         km = b.time_kernels(5)
-    wall64, worst = 0.0, {"l1_rho": 0.0, "linf_rho": 0.0, "l1_vphi": 0.0, "disk": None}
-    rot_equal = True
-    for w, d, o in zip(ws, disks, out32):
+    wall64, worst, l1s = 0.0, {"l1_rho": 0.0, "linf_rho": 0.0, "l1_vphi": 0.0, "disk": None}, []
+    refs = []
+    for k, (w, d, o) in enumerate(zip(ws, disks, out32)):
         with abi.Context(w.grid, w.params, abi.PN_FP64) as ctx:
             ctx.load(d)
             s = ctx.run_begin()
@@ -105,17 +106,31 @@ def batched_run(steps: int, warmup: int) -> dict:
             ctx.run(steps, s)
             wall64 += time.perf_counter() - t0
             ref = ctx.store()
-        rot_equal &= bool(np.array_equal(o["rot_j"], ref["rot_j"]))
+        refs.append(ref)
         l1, li = rel_err(lab(o, "r"), lab(ref, "r"))
+        l1s.append(l1)
         lv = rel_err(lab(o, "v"), lab(ref, "v"))[0]
-        if l1 > worst["l1_rho"]:
-            worst.update(l1_rho=l1, linf_rho=li, l1_vphi=lv, disk=w.name)
+        if l1 >= worst["l1_rho"]:
+            worst.update(l1_rho=l1, linf_rho=li, l1_vphi=lv, disk=w.name, index=k)
+    # how far the same flow drifts when only its initial state is rounded to FP32 (FP64
+    # arithmetic throughout): the sensitivity floor of that disk
+    k = worst.pop("index")
+    w, d = ws[k], disks[k].copy()
+    for f in ("r", "p", "u", "v"):
+        d.rings[f] = d.rings[f].astype(np.float32).astype(np.float64)
+    with abi.Context(w.grid, w.params, abi.PN_FP64) as ctx:
+        ctx.load(d)
+        s = ctx.run_begin()
+        ctx.run(warmup + steps, s)
+        floor = rel_err(lab(ctx.store(), "r"), lab(refs[k], "r"))[0]
     value = cells * steps / wall
     v64 = cells * steps / wall64
     return {"workload": f"c4_batch_{len(ws)}_disks", "cells": cells, "steps": steps, "warmup": warmup,
             "value": value, "unit": "zone-updates/s", "ms_per_step": wall / steps * 1e3, "kernels_ms": km,
             "fp64_plain_value": v64, "speedup_vs_fp64_plain": value / v64,
-            "accuracy_vs_fp64_worst_disk": dict(worst, after_steps=warmup + steps, rot_j_equal=rot_equal)}
+            "accuracy_vs_fp64": {"after_steps": warmup + steps, "median_l1_rho": float(np.median(l1s)),
+                                 "disks_l1_rho_le_1e-5": int(sum(1 for x in l1s if x <= 1e-5)),
+                                 "worst": dict(worst, fp64_from_fp32_rounded_start_l1_rho=floor)}}
 
 
 def lab(state: np.ndarray, f: str) -> np.ndarray:
@@ -154,9 +169,10 @@ def main(argv=None) -> int:
     km = run["km"]
 
     # dominant kernel, timed on the context's stream
-    dom = max(("sweep_x", "sweep_y"), key=lambda k: km[k])
+    dom = max(("sweep_x", "sweep_y", "sweep_z"), key=lambda k: km[k])
     wbytes = 8 if prec == abi.PN_FP64 else 4
-    fields = 3   # Sigma, v_r, v_phi: the minimal 2-D isothermal state
+    # the minimal state: Sigma (rho), v_r, v_phi; + v_theta in 3-D; + P unless isothermal
+    fields = 3 + (1 if w.params.ndim == 3 else 0) + (1 if w.params.eos != 0 else 0)
     bytes_per_launch = 2 * fields * wbytes * w.cells
     achieved = bytes_per_launch / (km[dom] * 1e-3) / 1e9
     traffic = None

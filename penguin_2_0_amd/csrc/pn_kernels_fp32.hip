@@ -304,7 +304,7 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
   const float sqrt_gam = sqrtf((float)prm.gam);
   const Para<float, K> P0 = f.uniform_phi ? paraset_uniform<K>() : paraset(0, lim, W.dx0);
   ppmlr_pert<K, 1>(W, TabLane<K>{nullptr, 0, 0}, P0, lim, dt, wres, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
-  if (VISC == 1 || (VISC == 2 && prm.visc)) visc_p_pert<K>(W, lim, dt, T[kTabNu * ts]);
+  if (VISC == 1 || (VISC == 2 && prm.visc)) visc_p_pert<K>(W, lim, dt, T[kTabNu * ts], rc);
 
   float* __restrict__ dr = f.f[src ^ 1][0] + base;
   float* __restrict__ du = f.f[src ^ 1][1] + base;

@@ -570,19 +570,23 @@ __device__ __forceinline__ void visc_r_pert(WinP<K>& W, const TabLane<K>& tx, in
 
 // device_viscosity_p (viscosity.cu:423-472) on the azimuthal sweep's state: sweep u =
 // v_phi - v_bg and sweep v = v_r.  nu and the backgrounds are uniform along the ring, so
-// only deviations enter the derivatives (in the cell's angle coordinates).
+// only deviations enter the derivatives.  The coordinates are arc lengths along the ring,
+// the left edges times R = r_cyl (ppm.cu:277-278).
 template <int K>
-__device__ __forceinline__ void visc_p_pert(WinP<K>& W, int lim, float dt, float nu) {
-  C<float, K> np_;
+__device__ __forceinline__ void visc_p_pert(WinP<K>& W, int lim, float dt, float nu, float R) {
+  C<float, K> np_, wx;
 #pragma unroll
-  for (int h = 0; h < K; ++h) np_[h] = nu * W.r[h];
-  C<float, K> wm = prev1(W.dx0);
+  for (int h = 0; h < K; ++h) {
+    np_[h] = nu * W.r[h];
+    wx[h] = W.dx0[h] * R;
+  }
+  C<float, K> wm = prev1(wx);
   C<float, K> nm = prev1(np_), npp = next1(np_);
   C<float, K> vm = prev1(W.u), vp = next1(W.u), rm = prev1(W.v), rp = next1(W.v);
 #pragma unroll
   for (int h = 0; h < K; ++h) {
     const int c = cell_of(K, h);
-    const float x2 = wm[h], x3 = wm[h] + W.dx0[h];
+    const float x2 = wm[h], x3 = wm[h] + wx[h];
     const float rho = W.rb[h] + W.r[h], n = nu * rho;
     float dn, d1, d2;
     derivs<float>(0.0f, x2, x3, nm[h], np_[h], npp[h], dn, d2);

@@ -139,3 +139,29 @@ def test_create_without_device_fails_loudly():
     with pytest.raises(abi.PnError) as e:
         abi.Context(PnGrid(32, 48, 1), configs.disk2d(32, 48))
     assert e.value.code == abi.PN_ERR_HIP
+
+
+def test_batch_argument_errors_before_any_device_call():
+    g = [PnGrid(64, 192, 1), PnGrid(48, 144, 1)]
+    p = [configs.disk2d(64, 192), configs.disk2d(48, 144)]
+    # the batched launch is built for the FP32 path only
+    with pytest.raises(abi.PnError) as e:
+        abi.Batch(g, p, precision=abi.PN_FP64)
+    assert e.value.code == abi.PN_ERR_UNSUPPORTED
+    # a planet that moves needs its force reductions between the sweeps
+    with pytest.raises(abi.PnError) as e:
+        abi.Batch(g, [p[0].replace(plnt_flag=2), p[1]])
+    assert e.value.code == abi.PN_ERR_UNSUPPORTED
+    with pytest.raises(abi.PnError) as e:
+        abi.Batch([], [])
+    assert e.value.code == abi.PN_ERR_ARG
+
+
+@pytest.mark.skipif(_gpu_visible(), reason="checks the no-device error path")
+def test_fp32_and_batch_without_device_fail_loudly():
+    with pytest.raises(abi.PnError) as e:
+        abi.Context(PnGrid(32, 48, 1), configs.disk2d(32, 48), abi.PN_FP32)
+    assert e.value.code == abi.PN_ERR_HIP
+    with pytest.raises(abi.PnError) as e:
+        abi.Batch([PnGrid(32, 48, 1)], [configs.disk2d(32, 48)])
+    assert e.value.code == abi.PN_ERR_HIP
