@@ -161,6 +161,8 @@ int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int pr
   if (n < 1 || !grids || !params) return bfail(b, PN_ERR_ARG, "batch of %d disks", n);
   if (precision != PN_FP32) return bfail(b, PN_ERR_UNSUPPORTED, "the batched launch is built for the FP32 path");
   for (int d = 0; d < n; ++d) {
+    if (params[d].ndim != 2)
+      return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches are built for 2-D disks", d);
     if (params[d].plnt_flag == 2)
       return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: a moving planet needs its force reductions between sweeps", d);
     pn_ctx* c = nullptr;

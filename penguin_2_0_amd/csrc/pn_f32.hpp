@@ -19,6 +19,13 @@ namespace pn {
 // net radial acceleration; f_face = -1/xa0^2 + L_f^2/xa0^3, the force at the left edge;
 // Omega_b = L_b / rad^2, and the derivatives are those of the viscosity's quadratic fit
 // through the left edges of slots s-1, s, s+1 (viscosity.cu:2-19).
+// In 3-D (spherical r, theta) rad is r, L_b = r_cyl v_bg is the angular momentum about the
+// axis, the centrifugal factor of L_b at the left edge is 1/(xa0^3 sin^2 theta) (kTabIxa3),
+// the star's gravity of the background is -1/r^2, and the theta sweep has its own set:
+//   theta left edge, width, volume r (cos - cos), sin / cos of the left and right edges,
+//   background differences to the next theta slot (s + stride in slots), the theta
+//   residual and pressure accelerations, the edge force, L_f, and G = cos / (r sin)^3 of
+//   the centrifugal theta component at the edge and at the centre with two derivatives.
 // The stride leaves room past the last slot for a radial window's overhang (copies of the
 // last slot, as a clamped read would give).
 enum TabEntry {
@@ -29,21 +36,28 @@ enum TabEntry {
   kTabLf, kTabIxa3, kTabVrot, kTabDvol0,
   kTabNu, kTabDnb, kTabD1Ob, kTabD2Ob,
   kTabP0, kTabP1, kTabP2, kTabP3, kTabP4,   // paraset coefficients of the Eulerian radial grid
+  // r_cyl, 1/r_cyl, 1/sin^2(theta), 1/xa0^3, sin / cos theta, 1 - sin theta (all 1, 1/r... in 2-D)
+  kTabRc, kTabIrc, kTabIs2, kTabIxa3w, kTabSth, kTabCth, kTabOms,
+  // theta sweep
+  kTabZa0, kTabDz0, kTabDvolZ, kTabSzl, kTabCzl, kTabSzr, kTabCzr,
+  kTabDrbZ, kTabDpbZ, kTabDLbZ, kTabDpbmZ,
+  kTabResBgZ, kTabAbgZ, kTabFfaceZ, kTabLfZ, kTabGfZ, kTabGcZ, kTabGc1Z, kTabGc2Z,
   kTabEntries
 };
 constexpr int kTabOverhang = 256;   // >= the longest radial window
 
 // FP32 state as deviations from per-slot backgrounds, and the per-slot constants.
-// Fields: 0 Sigma - Sigma_b(slot), 1 v_r, 2 v_phi - v_bg(slot).
+// Fields: 0 Sigma - Sigma_b(slot), 1 v_r, 2 v_phi - v_bg(slot), 3 v_theta (3-D only).
 struct DevF32 {
-  float* f[2][3];
+  float* f[2][4];
   float* tab;                  // kTabEntries x tstride, entry-major
   int tstride;                 // nslot + kTabOverhang, a multiple of 4
   const double *rbg, *vbg;     // Sigma_b and v_bg per slot, FP64
   const float *dy, *yvol;      // per column
   float2* trig;                // per column, per step: sin / cos of half the angle to the planet
-  int nwin_x, nwin_y;          // windows of the variant in use
+  int nwin_x, nwin_y, nwin_z;  // windows of the variant in use
   int uniform_phi;             // every dphi equal: the azimuthal paraset is (1/2, -1/6, 1/6, 1/2, 1/2)
+  int uniform_theta;           // every dtheta equal (3-D)
 };
 
 constexpr int kNumF32Variants = 4;
@@ -58,8 +72,13 @@ int f32_windows(int len, int K);
 hipError_t f32_prep(const LaunchCtx& c, const DevF32& f);
 hipError_t f32_sweep_x(const LaunchCtx& c, const DevF32& f, int variant, int src);
 // do_cfl: write each workgroup's CFL maximum of the stored state to c.cflp[workgroup]
+// (the azimuthal sweep in 2-D, the theta sweep in 3-D: the step's last sweep)
 hipError_t f32_sweep_y(const LaunchCtx& c, const DevF32& f, int variant, int src, int do_cfl);
 int f32_cfl_partials_y(const DevGrid& g, int variant);   // partials the fused CFL writes
+// 3-D: the spherical radial sweep and the theta sweep, both in the radial sweep's variants
+hipError_t f32_sweep_x3(const LaunchCtx& c, const DevF32& f, int variant, int src);
+hipError_t f32_sweep_z(const LaunchCtx& c, const DevF32& f, int variant, int src, int do_cfl);
+int f32_cfl_partials_z(const DevGrid& g, int variant);
 
 // batches of disks: one persistent launch per sweep over every disk's work items.
 // items: workgroup-sized work items of one disk (radial: window x column group; azimuthal:

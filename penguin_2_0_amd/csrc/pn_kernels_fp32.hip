@@ -36,9 +36,10 @@ __device__ __forceinline__ int win_len_k(int len, int nwin, int l) {
   return (l < nwin - 1) ? WIN : (len - REAL * (nwin - 1) + 2 * kNpad);
 }
 
-__device__ __forceinline__ PlanetF planet_f(const StepScalars* ss) {
+// the planet as the FP32 forces see it; the spherical forms use the reduced mass m/(1+m)
+__device__ __forceinline__ PlanetF planet_f(const StepScalars* ss, bool sph = false) {
   PlanetF p;
-  p.m = (float)ss->planet.m;
+  p.m = sph ? (float)(ss->planet.m / (1.0 + ss->planet.m)) : (float)ss->planet.m;
   p.x = (float)ss->planet.x;
   p.rs2 = (float)(ss->planet.rs * ss->planet.rs);
   p.inv_x2 = (float)(1.0 / (ss->planet.x * ss->planet.x));
@@ -84,130 +85,216 @@ __device__ __forceinline__ C<float, K> lds_k(const float* p) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Radial sweep.  A workgroup of W waves takes one radial window (64 K rings with halos) and
-// NC = W Q consecutive lab columns.  The window's NC columns of the 3 fields are staged in
-// LDS with loads that are contiguous along phi (each ring's NC cells are adjacent in HBM);
-// every wave then runs Q columns as radial pencils out of LDS, writes its results back in
-// place, and the workgroup stores the window's interior rings contiguously again.
-// VISC: 0 none, 1 always, 2 as the disk's parameters say.
-template <int K, int W, int Q>
+// Radial and theta sweeps.  A pencil of these sweeps crosses rings (slot step 1 radially,
+// one theta row of slots in theta), so its cells are not contiguous in HBM; the cells of
+// neighbouring lab columns are.  A workgroup of W waves therefore takes one window (64 K
+// cells with halos) of NC = W Q consecutive lab columns, stages the window's NF fields in LDS
+// with loads that are contiguous along phi, runs Q columns per wave as pencils out of LDS,
+// writes its results back in place, and stores the window's interior contiguously again.
+// VISC: 0 none, 1 always, 2 as the disk's parameters say (radial, 2-D).
+template <int K, int W, int Q, int NF>
 struct XTile {
   static constexpr int WIN = 64 * K, NC = W * Q, LD = WIN + 2;
-  float tile[3][NC][LD];   // [field][column][cell]; LD even keeps K-float reads aligned
+  float tile[NF][NC][LD];   // [field][column][cell]; LD even keeps K-float reads aligned
   int srot[WIN];
+  float red[W];
 };
 
-// one work item of the radial sweep: window l, column group grp.  Workgroup-uniform; starts
-// with a barrier so that a workgroup can run items back to back on the same LDS.
-template <int K, int W, int Q, int VISC>
-__device__ __forceinline__ void x_item(const DevGrid& g, const DevF32& f, const DevParams& prm,
-                                       const StepScalars* __restrict__ ss, int src, int grp, int l,
-                                       XTile<K, W, Q>& T) {
+// one work item: window l of the pencils of row `row` (radial: theta row k; theta: ring i),
+// column group grp.  Workgroup-uniform; starts with a barrier so that a workgroup can run
+// items back to back on the same LDS.  With CFL, returns in thread 0 the largest CFL term of
+// the cells it stored (the last sweep of a 3-D step).
+template <int K, int W, int Q, int G, int VISC, bool CFL>
+__device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, const DevParams& prm,
+                                             const StepScalars* __restrict__ ss, int src, int grp, int l, int row,
+                                             XTile<K, W, Q, (G == kGeoRcyl ? 3 : 4)>& T) {
+  constexpr int NF = (G == kGeoRcyl) ? 3 : 4;
   constexpr int WIN = 64 * K, REAL = WIN - 2 * kNpad, NC = W * Q;
+  constexpr bool TH = (G == kGeoTheta);
   auto& tile = T.tile;
   auto& srot = T.srot;
   const int jmax = g.jmax;
   __syncthreads();
   const int jb = grp * NC;
-  const int lim = win_len_k<K>(g.imax, f.nwin_x, l);
-  const int s0 = slot_of(g, REAL * l - kNpad, 0);   // slot of window cell 0
-  for (int c = threadIdx.x; c < WIN; c += 64 * W) srot[c] = g.rot_j[s0 + (c < lim ? c : lim - 1)];
+  const int lim = TH ? win_len_k<K>(g.kmax, f.nwin_z, l) : win_len_k<K>(g.imax, f.nwin_x, l);
+  const int s0 = TH ? slot_of(g, row, REAL * l - kNpad) : slot_of(g, REAL * l - kNpad, row);
+  const int step = TH ? g.sr : 1;   // slots between neighbouring cells of the pencil
+  for (int c = threadIdx.x; c < WIN; c += 64 * W) srot[c] = g.rot_j[s0 + step * (c < lim ? c : lim - 1)];
   __syncthreads();
-  {
-    const float* __restrict__ fr = f.f[src][0];
-    const float* __restrict__ fu = f.f[src][1];
-    const float* __restrict__ fv = f.f[src][2];
-    for (int e = threadIdx.x; e < WIN * NC; e += 64 * W) {
-      const int q = e % NC, c = e / NC;
-      const int jl = jb + q;
-      if (jl < jmax) {
-        const long a = (long)(s0 + (c < lim ? c : lim - 1)) * jmax + wrapj(jl - srot[c], jmax);
-        tile[0][q][c] = fr[a];
-        tile[1][q][c] = fu[a];
-        tile[2][q][c] = fv[a];
-      }
+  for (int e = threadIdx.x; e < WIN * NC; e += 64 * W) {
+    const int q = e % NC, c = e / NC;
+    const int jl = jb + q;
+    if (jl < jmax) {
+      const long a = (long)(s0 + step * (c < lim ? c : lim - 1)) * jmax + wrapj(jl - srot[c], jmax);
+#pragma unroll
+      for (int fi = 0; fi < NF; ++fi) tile[fi][q][c] = f.f[src][fi][a];
     }
   }
   __syncthreads();
 
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const float dt = (float)ss->dt;
-  const PlanetF pl = planet_f(ss);
+  const PlanetF pl = planet_f(ss, G != kGeoRcyl);
   const float sqrt_gam = sqrtf((float)prm.gam);
-  const TabLane<K> tx{f.tab, f.tstride, s0 + K * lane};   // overhang cells read copies of the last slot
+  // radial: overhang cells read copies of the last slot; theta: reads clamped to the window
+  const auto tx = [&] {
+    if constexpr (TH) return TabLaneS<K>{f.tab, f.tstride, s0, g.sr, K * lane, lim - 1};
+    else return TabLane<K>{f.tab, f.tstride, s0 + K * lane};
+  }();
   for (int qq = 0; qq < Q; ++qq) {
     const int q = wv * Q + qq;
     const int j0 = jb + q;
     if (j0 >= jmax) break;   // wave-uniform
     WinP<K> Wp;
     Wp.rad = tx.get(kTabRad);
-    Wp.xa0 = tx.get(kTabXa0);
-    Wp.dx0 = tx.get(kTabDx0);
-    Wp.dvol0 = tx.get(kTabDvol0);
+    if constexpr (TH) {
+      Wp.xa0 = tx.get(kTabZa0);
+      Wp.dx0 = tx.get(kTabDz0);
+      Wp.dvol0 = tx.get(kTabDvolZ);
+      Wp.drb = tx.get(kTabDrbZ);
+      Wp.dpb = tx.get(kTabDpbZ);
+      Wp.dLb = tx.get(kTabDLbZ);
+    } else {
+      Wp.xa0 = tx.get(kTabXa0);
+      Wp.dx0 = tx.get(kTabDx0);
+      Wp.dvol0 = tx.get(kTabDvol0);
+      Wp.drb = tx.get(kTabDrb);
+      Wp.dpb = tx.get(kTabDpb);
+      Wp.dLb = tx.get(kTabDLb);
+    }
     Wp.rb = tx.get(kTabRb);
-    Wp.drb = tx.get(kTabDrb);
     Wp.pb = tx.get(kTabPb);
-    Wp.dpb = tx.get(kTabDpb);
     Wp.cs2 = tx.get(kTabCs2);
     Wp.cs = tx.get(kTabCs);
     Wp.Lb = tx.get(kTabLb);
-    Wp.dLb = tx.get(kTabDLb);
     Wp.vrot = tx.get(kTabVrot);
+    if constexpr (G != kGeoRcyl) {
+      Wp.sth = tx.get(kTabSth);
+      Wp.oms = tx.get(kTabOms);
+    }
     const float2 tr = f.trig[j0];
     Wp.ts = splat<float, K>(tr.x);
     Wp.tc = splat<float, K>(tr.y);
-    Wp.r = lds_k<K>(&tile[0][q][K * lane]);
-    Wp.u = lds_k<K>(&tile[1][q][K * lane]);
-    const C<float, K> vp = lds_k<K>(&tile[2][q][K * lane]);
+    const C<float, K> rc = tx.get(kTabRc);
+    const C<float, K> f0 = lds_k<K>(&tile[0][q][K * lane]), f1 = lds_k<K>(&tile[1][q][K * lane]),
+                      f2 = lds_k<K>(&tile[2][q][K * lane]);
+    C<float, K> f3;
+    if constexpr (NF == 4) f3 = lds_k<K>(&tile[3][q][K * lane]);
+    Wp.r = f0;
 #pragma unroll
-    for (int h = 0; h < K; ++h) Wp.v[h] = Wp.rad[h] * vp[h];   // l - L_b = r (v_phi - v_bg); r_cyl = r in 2-D
+    for (int h = 0; h < K; ++h) {
+      // l - L_b = r_cyl (v_phi - v_bg); spherical radial w = r v_theta
+      if constexpr (TH) {
+        Wp.u[h] = f3[h];
+        Wp.v[h] = f1[h];
+        Wp.w[h] = rc[h] * f2[h];
+      } else {
+        Wp.u[h] = f1[h];
+        Wp.v[h] = rc[h] * f2[h];
+        if constexpr (G == kGeoRsph) Wp.w[h] = Wp.rad[h] * f3[h];
+      }
+    }
     Para<float, K> P0;
-    P0.c0 = tx.get(kTabP0);
-    P0.c1 = tx.get(kTabP1);
-    P0.c2 = tx.get(kTabP2);
-    P0.c3 = tx.get(kTabP3);
-    P0.c4 = tx.get(kTabP4);
-    ppmlr_pert<K, 0>(Wp, tx, P0, lim, dt, 0.0f, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
-    if (VISC == 1 || (VISC == 2 && prm.visc)) visc_r_pert<K>(Wp, tx, lim, dt);
-    const C<float, K> ixc = tx.get(kTabIxc);
+    if constexpr (TH) {
+      P0 = f.uniform_theta ? paraset_uniform<K>() : paraset(0, lim, Wp.dx0);
+    } else {
+      P0.c0 = tx.get(kTabP0);
+      P0.c1 = tx.get(kTabP1);
+      P0.c2 = tx.get(kTabP2);
+      P0.c3 = tx.get(kTabP3);
+      P0.c4 = tx.get(kTabP4);
+    }
+    ppmlr_pert<K, G, (G == kGeoRcyl ? 2 : 3)>(Wp, tx, P0, lim, dt, 0.0f, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
+    if constexpr (G == kGeoRcyl)
+      if (VISC == 1 || (VISC == 2 && prm.visc)) visc_r_pert<K>(Wp, tx, lim, dt);
+    const C<float, K> irc = tx.get(kTabIrc);
+    C<float, K> ixc;
+    if constexpr (G == kGeoRsph) ixc = tx.get(kTabIxc);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
       const int c = K * lane + h;
       if (c >= kNpad && c < lim - kNpad) {   // only this wave touches column q
         tile[0][q][c] = Wp.r[h];
-        tile[1][q][c] = Wp.u[h];
-        tile[2][q][c] = Wp.v[h] * ixc[h];
+        if constexpr (TH) {
+          tile[3][q][c] = Wp.u[h];
+          tile[1][q][c] = Wp.v[h];
+          tile[2][q][c] = Wp.w[h] * irc[h];
+        } else {
+          tile[1][q][c] = Wp.u[h];
+          tile[2][q][c] = Wp.v[h] * irc[h];
+          if constexpr (G == kGeoRsph) tile[3][q][c] = Wp.w[h] * ixc[h];
+        }
       }
     }
   }
   __syncthreads();
-  {
-    float* __restrict__ dr = f.f[src ^ 1][0];
-    float* __restrict__ du = f.f[src ^ 1][1];
-    float* __restrict__ dv = f.f[src ^ 1][2];
-    for (int e = threadIdx.x; e < WIN * NC; e += 64 * W) {
-      const int q = e % NC, c = e / NC;
-      const int jl = jb + q;
-      if (jl < jmax && c >= kNpad && c < lim - kNpad) {
-        const long a = (long)(s0 + c) * jmax + wrapj(jl - srot[c], jmax);
-        dr[a] = tile[0][q][c];
-        du[a] = tile[1][q][c];
-        dv[a] = tile[2][q][c];
+  float m = 0.0f;
+  const double FrRot = ss->FrRot;
+  for (int e = threadIdx.x; e < WIN * NC; e += 64 * W) {
+    const int q = e % NC, c = e / NC;
+    const int jl = jb + q;
+    if (jl < jmax && c >= kNpad && c < lim - kNpad) {
+      const int sl = s0 + step * c;
+      const int j = wrapj(jl - srot[c], jmax);
+      const long a = (long)sl * jmax + j;
+#pragma unroll
+      for (int fi = 0; fi < NF; ++fi) f.f[src ^ 1][fi][a] = tile[fi][q][c];
+      if constexpr (CFL) {
+        // the next step's CFL term of this cell (timestep.cu:12-108), isothermal c_s
+        const float* T = f.tab + sl;
+        const long ts = f.tstride;
+        const float csnd = sqrt_gam * T[kTabCs * ts];
+        const float cfl_off = (float)(f.vbg[sl] - g.rot_v[sl] - FrRot * g.radc[sl]);
+        const float t0 = (fabsf(tile[1][q][c]) + csnd) / T[kTabDx0 * ts];
+        const float t1 = (fabsf(tile[2][q][c] + cfl_off) + csnd) / (f.dy[j] * T[kTabRc * ts]);
+        const float t2 = (fabsf(tile[NF - 1][q][c]) + csnd) / (T[kTabDz0 * ts] * T[kTabRad * ts]);
+        m = fmaxf(m, fmaxf(t0, fmaxf(t1, t2)));
       }
     }
   }
+  if constexpr (CFL) {
+    m = wave_maxf(m);
+    if (lane == 0) T.red[wv] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int w = 1; w < W; ++w) m = fmaxf(m, T.red[w]);
+  }
+  return m;
 }
 
 // one disk: one workgroup per item, XCD-contiguous
 template <int K, int W, int Q, int VISC>
 __global__ __launch_bounds__(64 * W) void k_sweep_x_f32(DevGrid g, DevF32 f, DevParams prm,
                                                         const StepScalars* __restrict__ ss, int src) {
-  __shared__ XTile<K, W, Q> T;
+  __shared__ XTile<K, W, Q, 3> T;
   const int ngrp = (g.jmax + W * Q - 1) / (W * Q);
   const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
   const int grp = bid % ngrp, l = bid / ngrp;
   if (l >= f.nwin_x) return;   // workgroup-uniform, before any barrier
-  x_item<K, W, Q, VISC>(g, f, prm, ss, src, grp, l, T);
+  pencil_item<K, W, Q, kGeoRcyl, VISC, false>(g, f, prm, ss, src, grp, l, 0, T);
+}
+
+// 3-D: the spherical radial sweep (item = theta row k, radial window, column group) and the
+// theta sweep (item = ring i, theta window, column group; with do_cfl it is the step's last
+// sweep and writes one CFL partial per workgroup)
+template <int K, int W, int Q, int G>
+__global__ __launch_bounds__(64 * W) void k_sweep_pencil3_f32(DevGrid g, DevF32 f, DevParams prm,
+                                                              const StepScalars* __restrict__ ss, double* cflp,
+                                                              int src, int do_cfl) {
+  __shared__ XTile<K, W, Q, 4> T;
+  const int ngrp = (g.jmax + W * Q - 1) / (W * Q);
+  const int nwin = (G == kGeoTheta) ? f.nwin_z : f.nwin_x;
+  const int nrow = (G == kGeoTheta) ? g.imax : g.kmax;
+  const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int grp = bid % ngrp, rest = bid / ngrp;
+  const int l = rest % nwin, row = rest / nwin;
+  if (row >= nrow) return;   // workgroup-uniform, before any barrier
+  if (do_cfl) {
+    const float m = pencil_item<K, W, Q, G, 0, true>(g, f, prm, ss, src, grp, l, row, T);
+    if (threadIdx.x == 0) cflp[blockIdx.x] = (double)m;
+  } else {
+    pencil_item<K, W, Q, G, 0, false>(g, f, prm, ss, src, grp, l, row, T);
+  }
 }
 
 // the disk owning work item it: base[d] <= it < base[d + 1] (ndisk small; base in device memory)
@@ -228,7 +315,7 @@ template <int K, int W, int Q>
 __global__ __launch_bounds__(64 * W) void k_sweep_x_f32_batch(const DiskDev* __restrict__ dd,
                                                               const DevF32* __restrict__ dfs,
                                                               const int* __restrict__ base, int ndisk, int src) {
-  __shared__ XTile<K, W, Q> T;
+  __shared__ XTile<K, W, Q, 3> T;
   const int nitems = base[ndisk];
   for (int t = blockIdx.x; t < nitems; t += gridDim.x) {
     const int it = xcd_swizzle(t, nitems);
@@ -236,23 +323,24 @@ __global__ __launch_bounds__(64 * W) void k_sweep_x_f32_batch(const DiskDev* __r
     const DevGrid g = dd[d].g;
     const int ngrp = (g.jmax + W * Q - 1) / (W * Q);
     const int local = it - base[d];
-    x_item<K, W, Q, 2>(g, dfs[d], dd[d].p, dd[d].ss, src, local % ngrp, local / ngrp, T);
+    pencil_item<K, W, Q, kGeoRcyl, 2, false>(g, dfs[d], dd[d].p, dd[d].ss, src, local % ngrp, local / ngrp, 0, T);
   }
 }
 
 // ---------------------------------------------------------------------------------------
-// Azimuthal sweep of window l of interior ring n by one wavefront; returns the wave's
-// maximum CFL term of the cells it stored.
-template <int K, int VISC>
+// Azimuthal sweep of window l of interior ring n (3-D: ring n % imax of theta row n / imax)
+// by one wavefront; returns the wave's maximum CFL term of the cells it stored (2-D).  The
+// sweep's radius is r_cyl; in 3-D it also carries v_theta and the planet's spherical forms.
+template <int K, int VISC, int NDIM>
 __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, const DevParams& prm,
                                           const StepScalars* __restrict__ ss, int src, int n, int l) {
   constexpr int REAL = 64 * K - 2 * kNpad;
   const int lane = threadIdx.x & 63;
-  const int s = slot_of(g, n, 0);
+  const int s = (NDIM == 3) ? slot_of(g, n % g.imax, n / g.imax) : slot_of(g, n, 0);
   const int jmax = g.jmax;
   const float dt = (float)ss->dt;
   const double FrRot = ss->FrRot;
-  const PlanetF pl = planet_f(ss);
+  const PlanetF pl = planet_f(ss, NDIM == 3);
   const int lim = win_len_k<K>(jmax, f.nwin_y, l);
   const int rot = g.rot_j[s];
   const double rcd = g.radc[s];
@@ -263,12 +351,19 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
   const float cfl_off = (float)(vbg - g.rot_v[s] - FrRot * rcd);
   const float* __restrict__ T = f.tab + s;
   const long ts = f.tstride;
-  const float rc = T[kTabRad * ts], dxr = T[kTabDx0 * ts], rb = T[kTabRb * ts], pb = T[kTabPb * ts],
+  const float rc = T[kTabRc * ts], dxr = T[kTabDx0 * ts], rb = T[kTabRb * ts], pb = T[kTabPb * ts],
               cs2 = T[kTabCs2 * ts], cs = T[kTabCs * ts], vrot = T[kTabVrot * ts];
+  float rsph = 0.0f, sth = 1.0f, oms = 0.0f;
+  if constexpr (NDIM == 3) {
+    rsph = T[kTabRad * ts];
+    sth = T[kTabSth * ts];
+    oms = T[kTabOms * ts];
+  }
   const long base = (long)s * jmax;
   const float* __restrict__ fr = f.f[src][0] + base;
   const float* __restrict__ fu = f.f[src][1] + base;
   const float* __restrict__ fv = f.f[src][2] + base;
+  const float* __restrict__ fw = (NDIM == 3) ? f.f[src][3] + base : nullptr;
 
   WinP<K> W;
   int jd[K];
@@ -296,19 +391,25 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
     W.cs[h] = cs;
     W.Lb[h] = 0.0f;
     W.dLb[h] = 0.0f;
+    W.sth[h] = sth;
+    W.oms[h] = oms;
     dyc[h] = f.dy[j];
     W.r[h] = fr[j];
     W.u[h] = fv[j];   // v_phi - v_bg
     W.v[h] = fu[j];   // v_r
+    if constexpr (NDIM == 3) W.w[h] = fw[j];
   }
   const float sqrt_gam = sqrtf((float)prm.gam);
   const Para<float, K> P0 = f.uniform_phi ? paraset_uniform<K>() : paraset(0, lim, W.dx0);
-  ppmlr_pert<K, 1>(W, TabLane<K>{nullptr, 0, 0}, P0, lim, dt, wres, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
-  if (VISC == 1 || (VISC == 2 && prm.visc)) visc_p_pert<K>(W, lim, dt, T[kTabNu * ts], rc);
+  ppmlr_pert<K, kGeoPhi, NDIM>(W, TabLane<K>{nullptr, 0, 0}, P0, lim, dt, wres, pl, prm.plnt > 0, prm.flat != 0,
+                               sqrt_gam, rsph);
+  if constexpr (NDIM == 2)
+    if (VISC == 1 || (VISC == 2 && prm.visc)) visc_p_pert<K>(W, lim, dt, T[kTabNu * ts], rc);
 
   float* __restrict__ dr = f.f[src ^ 1][0] + base;
   float* __restrict__ du = f.f[src ^ 1][1] + base;
   float* __restrict__ dv = f.f[src ^ 1][2] + base;
+  float* __restrict__ dw = (NDIM == 3) ? f.f[src ^ 1][3] + base : nullptr;
   const float csnd = sqrt_gam * cs;
   const float idx = 1.0f / dxr;
   float m = 0.0f;
@@ -320,6 +421,7 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
       dr[j] = W.r[h];
       du[j] = W.v[h];
       dv[j] = W.u[h];
+      if constexpr (NDIM == 3) dw[j] = W.w[h];
       // the next step's CFL term of this cell (timestep.cu:48-56)
       const float t0 = (fabsf(W.v[h]) + csnd) * idx;
       const float t1 = (fabsf(W.u[h] + cfl_off) + csnd) / (dyc[h] * rc);
@@ -330,8 +432,8 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
 }
 
 // one wavefront per (ring, window); with do_cfl each workgroup writes the maximum CFL term
-// of its waves to cflp[workgroup]
-template <int K, int WPB, int VISC>
+// of its waves to cflp[workgroup] (2-D)
+template <int K, int WPB, int VISC, int NDIM>
 __global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, DevParams prm,
                                                           const StepScalars* __restrict__ ss, double* cflp, int src,
                                                           int do_cfl) {
@@ -341,7 +443,7 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, D
   const int l = gw % f.nwin_y;
   const int n = gw / f.nwin_y;
   float m = 0.0f;
-  if (n < g.imax) m = y_pencil<K, VISC>(g, f, prm, ss, src, n, l);   // wave-uniform
+  if (n < g.imax * g.kmax) m = y_pencil<K, VISC, NDIM>(g, f, prm, ss, src, n, l);   // wave-uniform
   if (do_cfl) {   // kernel-uniform: every wave reaches the barrier
     if ((threadIdx.x & 63) == 0) shm[wv] = m;
     __syncthreads();
@@ -371,7 +473,7 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32_batch(const DiskDev* _
     const int l = gw % f.nwin_y;
     const int n = gw / f.nwin_y;
     float m = 0.0f;
-    if (n < g.imax) m = y_pencil<K, 2>(g, f, dd[d].p, dd[d].ss, src, n, l);   // wave-uniform
+    if (n < g.imax) m = y_pencil<K, 2, 2>(g, f, dd[d].p, dd[d].ss, src, n, l);   // wave-uniform
     __syncthreads();   // the previous item's partial has been read
     if ((threadIdx.x & 63) == 0) shm[wv] = m;
     __syncthreads();
@@ -393,22 +495,23 @@ __device__ __forceinline__ float block_max256(float m, float* sh) {
 __global__ __launch_bounds__(256) void k_cfl_f32(DevGrid g, DevF32 f, DevParams prm, const StepScalars* ss,
                                                  double* cflp, int src) {
   __shared__ float sh[4];
-  const long ncell = (long)g.imax * g.jmax;
+  const long ncell = (long)g.imax * g.kmax * g.jmax;
   const double FrRot = ss->FrRot;
   const float sqrt_gam = sqrtf((float)prm.gam);
   float m = 0.0f;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
     const int j = (int)(t % g.jmax);
     const int n = (int)(t / g.jmax);
-    const int s = slot_of(g, n, 0);
+    const int s = slot_of(g, n % g.imax, n / g.imax);
     const long a = (long)s * g.jmax + j;
     const float* T = f.tab + s;
     const long ts = f.tstride;
     const float csnd = sqrt_gam * T[kTabCs * ts];
     const float cfl_off = (float)(f.vbg[s] - g.rot_v[s] - FrRot * g.radc[s]);
     const float t0 = (fabsf(f.f[src][1][a]) + csnd) / T[kTabDx0 * ts];
-    const float t1 = (fabsf(f.f[src][2][a] + cfl_off) + csnd) / (f.dy[j] * T[kTabRad * ts]);
+    const float t1 = (fabsf(f.f[src][2][a] + cfl_off) + csnd) / (f.dy[j] * T[kTabRc * ts]);
     m = fmaxf(m, fmaxf(t0, t1));
+    if (g.ndim == 3) m = fmaxf(m, (fabsf(f.f[src][3][a]) + csnd) / (T[kTabDz0 * ts] * T[kTabRad * ts]));
   }
   m = block_max256(m, sh);
   if (threadIdx.x == 0) cflp[blockIdx.x] = (double)m;
@@ -425,20 +528,20 @@ __global__ __launch_bounds__(256) void k_reduce_f32(DevGrid g, DevF32 f, DevPara
                                                     const StepScalars* __restrict__ ss, int src, int mode,
                                                     double exclude, double* partials) {
   __shared__ double sh[4];
-  const long ncell = (long)g.imax * g.jmax;
+  const long ncell = (long)g.imax * g.kmax * g.jmax;
   const Planet pl{ss->planet.m, ss->planet.x, ss->planet.y, ss->planet.vx, ss->planet.vy, ss->planet.rs};
   double acc = 0.0;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
     const int j = (int)(t % g.jmax);
     const int n = (int)(t / g.jmax);
-    const int s = slot_of(g, n, 0);
+    const int s = slot_of(g, n % g.imax, n / g.imax);
     int j0 = j + g.rot_j[s];
     if (j0 >= g.jmax) j0 -= g.jmax;
     const double r = f.rbg[s] + (double)f.f[src][0][(long)s * g.jmax + j];
     double tmp = 0.0;
     if (mode == 0 || prm.plnt == 0) {
       tmp = r * g.xvol[s] * g.yvol[j0] * g.zvol[s];
-    } else if (mode == 8 || mode == 9) {
+    } else if (g.ndim == 2 && (mode == 8 || mode == 9)) {
       const double rad = g.xc[s], phi = g.yc[j0];
       const double dd = (rad - pl.x) * (rad - pl.x) + rad * rad * (phi - pl.y) * (phi - pl.y);
       if (dd > exclude * exclude) {
@@ -457,12 +560,12 @@ __global__ __launch_bounds__(256) void k_reduce_f32(DevGrid g, DevF32 f, DevPara
 
 __global__ __launch_bounds__(256) void k_totals_f32(DevGrid g, DevF32 f, int src, double* partials) {
   __shared__ double sh[8];
-  const long ncell = (long)g.imax * g.jmax;
+  const long ncell = (long)g.imax * g.kmax * g.jmax;
   double sm = 0.0, cm = 0.0, sl = 0.0, cl = 0.0;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
     const int j = (int)(t % g.jmax);
     const int n = (int)(t / g.jmax);
-    const int s = slot_of(g, n, 0);
+    const int s = slot_of(g, n % g.imax, n / g.imax);
     int j0 = j + g.rot_j[s];
     if (j0 >= g.jmax) j0 -= g.jmax;
     const long a = (long)s * g.jmax + j;
@@ -490,7 +593,7 @@ __global__ __launch_bounds__(256) void k_totals_f32(DevGrid g, DevF32 f, int src
 }
 
 int red_blocks(const DevGrid& g) {
-  long ncell = (long)g.imax * g.jmax;
+  long ncell = (long)g.imax * g.kmax * g.jmax;
   long b = (ncell + 255) / 256;
   if (b > 2048) b = 2048;
   return (int)(b < 1 ? 1 : b);
@@ -514,11 +617,30 @@ template <int K, int WPB>
 hipError_t sweep_y_k(const LaunchCtx& c, const DevF32& f0, int src, int do_cfl) {
   DevF32 f = f0;
   f.nwin_y = f32_windows(c.g.jmax, K);
-  const long waves = (long)f.nwin_y * c.g.imax;
+  const long waves = (long)f.nwin_y * c.g.imax * c.g.kmax;
   const long nblk = (waves + WPB - 1) / WPB;
-  if (do_cfl && nblk > c.ncflp) return hipErrorInvalidValue;
+  if (do_cfl && (nblk > c.ncflp || c.g.ndim == 3)) return hipErrorInvalidValue;
   dim3 grid((unsigned)nblk), block(64 * WPB);
-  k_sweep_y_f32<K, WPB, 2><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  if (c.g.ndim == 3) k_sweep_y_f32<K, WPB, 0, 3><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, 0);
+  else k_sweep_y_f32<K, WPB, 2, 2><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  return hipGetLastError();
+}
+
+// 3-D radial (G = kGeoRsph) and theta (kGeoTheta) sweeps; items: column group x window x row
+template <int K, int W, int Q, int G>
+hipError_t sweep_pencil3_k(const LaunchCtx& c, const DevF32& f0, int src, int do_cfl) {
+  DevF32 f = f0;
+  const bool th = (G == kGeoTheta);
+  if (th) f.nwin_z = f32_windows(c.g.kmax, K);
+  else f.nwin_x = f32_windows(c.g.imax, K);
+  if (c.g.ndim != 3 || f.f[0][3] == nullptr) return hipErrorInvalidValue;
+  // radial windows read table entries up to 64 K - 1 slots past their first slot
+  if (!th && f.tstride < c.g.sr * c.g.sk + 64 * K) return hipErrorInvalidValue;
+  const long ngrp = (c.g.jmax + W * Q - 1) / (W * Q);
+  const long nblk = ngrp * (th ? (long)f.nwin_z * c.g.imax : (long)f.nwin_x * c.g.kmax);
+  if (do_cfl && (!th || nblk > c.ncflp)) return hipErrorInvalidValue;
+  dim3 grid((unsigned)nblk), block(64 * W);
+  k_sweep_pencil3_f32<K, W, Q, G><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
   return hipGetLastError();
 }
 
@@ -532,8 +654,13 @@ int f32_windows(int len, int K) {
 const int kF32VariantsX[kNumF32Variants][3] = {{2, 4, 2}, {2, 8, 1}, {2, 16, 1}, {2, 4, 4}};
 const int kF32VariantsY[kNumF32Variants][2] = {{2, 2}, {2, 4}, {4, 2}, {4, 4}};
 
+int f32_cfl_partials_z(const DevGrid& g, int variant) {
+  const int nc = kF32VariantsX[variant][1] * kF32VariantsX[variant][2];
+  return ((g.jmax + nc - 1) / nc) * f32_windows(g.kmax, kF32VariantsX[variant][0]) * g.imax;
+}
+
 int f32_cfl_partials_y(const DevGrid& g, int variant) {
-  const int waves = f32_windows(g.jmax, kF32VariantsY[variant][0]) * g.imax;
+  const int waves = f32_windows(g.jmax, kF32VariantsY[variant][0]) * g.imax * g.kmax;
   const int wpb = kF32VariantsY[variant][1];
   return (waves + wpb - 1) / wpb;
 }
@@ -551,6 +678,26 @@ hipError_t f32_sweep_x(const LaunchCtx& c, const DevF32& f, int variant, int src
     case 1: return sweep_x_k<2, 8, 1>(c, f, src);
     case 2: return sweep_x_k<2, 16, 1>(c, f, src);
     case 3: return sweep_x_k<2, 4, 4>(c, f, src);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t f32_sweep_x3(const LaunchCtx& c, const DevF32& f, int variant, int src) {
+  switch (variant) {
+    case 0: return sweep_pencil3_k<2, 4, 2, kGeoRsph>(c, f, src, 0);
+    case 1: return sweep_pencil3_k<2, 8, 1, kGeoRsph>(c, f, src, 0);
+    case 2: return sweep_pencil3_k<2, 16, 1, kGeoRsph>(c, f, src, 0);
+    case 3: return sweep_pencil3_k<2, 4, 4, kGeoRsph>(c, f, src, 0);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t f32_sweep_z(const LaunchCtx& c, const DevF32& f, int variant, int src, int do_cfl) {
+  switch (variant) {
+    case 0: return sweep_pencil3_k<2, 4, 2, kGeoTheta>(c, f, src, do_cfl);
+    case 1: return sweep_pencil3_k<2, 8, 1, kGeoTheta>(c, f, src, do_cfl);
+    case 2: return sweep_pencil3_k<2, 16, 1, kGeoTheta>(c, f, src, do_cfl);
+    case 3: return sweep_pencil3_k<2, 4, 4, kGeoTheta>(c, f, src, do_cfl);
   }
   return hipErrorInvalidValue;
 }

@@ -20,6 +20,10 @@
 using namespace pn;
 using namespace pn::rt;
 
+namespace {
+constexpr double kPiD = 3.1415926535897932384626433832795;
+}
+
 
 namespace pn {
 namespace rt {
@@ -76,6 +80,16 @@ bool is_f32(const pn_ctx* c) { return c->precision == PN_FP32; }
 // change inside a step, i.e. unless the planet moves under the disk's torque
 bool fused_cfl(const pn_ctx* c) { return is_f32(c) && c->P.plnt_flag < 2; }
 
+// the FP32 radial sweep of the context's geometry
+hipError_t f32_x(const pn_ctx* c, const LaunchCtx& L, int variant, int src) {
+  return (c->P.ndim == 3) ? f32_sweep_x3(L, c->df, variant, src) : f32_sweep_x(L, c->df, variant, src);
+}
+
+// CFL partials the step's last FP32 sweep writes
+int f32_fused_partials(const pn_ctx* c) {
+  return (c->P.ndim == 3) ? f32_cfl_partials_z(c->dg, c->var_x) : f32_cfl_partials_y(c->dg, c->var_y);
+}
+
 int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision) {
   if (g->imax < 1 || g->jmax < 2 * kNpad || g->kmax < 1) return fail(c, PN_ERR_ARG, "bad grid %d x %d x %d", g->imax, g->jmax, g->kmax);
   if (p->ndim != 2 && p->ndim != 3) return fail(c, PN_ERR_ARG, "ndim must be 2 or 3");
@@ -90,8 +104,8 @@ int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision)
   if (p->fargo_flag < 0 || p->fargo_flag > 2) return fail(c, PN_ERR_ARG, "fargo_flag %d", p->fargo_flag);
   if (p->plnt_flag < 0 || p->plnt_flag > 2) return fail(c, PN_ERR_ARG, "plnt_flag %d", p->plnt_flag);
   if (p->plnt_flag == 2 && p->ndim == 3) return fail(c, PN_ERR_UNSUPPORTED, "moving planet needs the 2-D force reductions");
-  if (precision == PN_FP32 && (p->ndim != 2 || p->eos != 0))
-    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for the 2-D isothermal disk");
+  if (precision == PN_FP32 && p->eos != 0)
+    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for the isothermal disk (EOS 0)");
   return PN_OK;
 }
 
@@ -130,8 +144,14 @@ int launch_sweeps(pn_ctx* c, int& cur, int do_cfl, bool prep, bool roll) {
   LaunchCtx L = lctx(c);
   if (is_f32(c)) {
     if (prep) PN_LAUNCH(c, "prep", f32_prep(L, c->df));
-    PN_LAUNCH(c, "sweep_x", f32_sweep_x(L, c->df, c->var_x, cur));
-    PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, cur ^ 1, do_cfl));
+    PN_LAUNCH(c, "sweep_x", f32_x(c, L, c->var_x, cur));
+    if (c->P.ndim == 3) {
+      PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, cur ^ 1, 0));
+      PN_LAUNCH(c, "sweep_z", f32_sweep_z(L, c->df, c->var_x, cur, do_cfl));
+      cur ^= 1;
+    } else {
+      PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, cur ^ 1, do_cfl));
+    }
   } else {
     PN_LAUNCH(c, "sweep_x", fp64_sweep_x(L, cur));
     PN_LAUNCH(c, "sweep_y", fp64_sweep_y(L, cur ^ 1));
@@ -161,7 +181,7 @@ int launch_step(pn_ctx* c, int& cur) {
   PN_LAUNCH(c, "pre step", launch_pre_step(L, ncfl, trig, vrot, 1));
   const bool post = c->P.plnt_flag < 2;
   if ((rc = launch_sweeps(c, cur, fused ? 1 : 0, false, !post)) != PN_OK) return rc;
-  if (post) PN_LAUNCH(c, "post step", launch_post_step(L, fused ? f32_cfl_partials_y(c->dg, c->var_y) : 0));
+  if (post) PN_LAUNCH(c, "post step", launch_post_step(L, fused ? f32_fused_partials(c) : 0));
   else PN_LAUNCH(c, "planet update", launch_planet_after_sweep(L, cur));
   return PN_OK;
 }
@@ -210,7 +230,7 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
       for (int f = 0; f < kNField && r2 == PN_OK; ++f) r2 = dalloc(c, &d.f[b][f], ncell);
   } else {
     for (int b = 0; b < 2 && r2 == PN_OK; ++b)
-      for (int f = 0; f < 3 && r2 == PN_OK; ++f) r2 = dalloc(c, &c->df.f[b][f], ncell);
+      for (int f = 0; f < (p->ndim == 3 ? 4 : 3) && r2 == PN_OK; ++f) r2 = dalloc(c, &c->df.f[b][f], ncell);
   }
   double *x, *dx, *xc, *xvol, *z, *dz, *zc, *zvol, *radc, *cs2, *nu, *y, *dy, *yc, *yvol;
   pn_symdisk* val;
@@ -227,9 +247,13 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
   if (r2 == PN_OK) r2 = dalloc(c, &c->partials, kMaxPartials);
   if (r2 == PN_OK) r2 = dalloc(c, &c->out2, 2);
   // CFL partials: one per block of the grid-stride CFL kernels (at most 2048) or one per
-  // azimuthal-sweep wave of the fused FP32 CFL (largest with 2 cells per lane)
+  // workgroup of the FP32 sweep that takes the CFL in its epilogue, in any variant
   c->ncflp = 2048;
-  if (precision == PN_FP32) c->ncflp = std::max(c->ncflp, f32_windows(g->jmax, 2) * g->imax);
+  if (precision == PN_FP32)
+    for (int v = 0; v < kNumF32Variants; ++v) {
+      c->ncflp = std::max(c->ncflp, f32_cfl_partials_y(d, v));
+      if (p->ndim == 3) c->ncflp = std::max(c->ncflp, f32_cfl_partials_z(d, v));
+    }
   if (r2 == PN_OK) r2 = dalloc(c, &c->cflp, c->ncflp);
   if (precision == PN_FP32) {
     float *fdy, *fyvol;
@@ -367,7 +391,8 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
         has_w = 1;
         break;
       }
-  if (is_f32(c) && has_w) return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path carries no w (2-D disk with w = 0)");
+  if (is_f32(c) && has_w && c->P.ndim == 2)
+    return fail(c, PN_ERR_UNSUPPORTED, "the 2-D FP32 path carries no w (2-D disk with w = 0)");
   c->dp.has_w = has_w;
   // hoisted per-ring closures, evaluated on the host in FP64 exactly as the reference's
   // device code evaluates them per cell (ppm.cu:81-82, 139; disk_profile.cu:39-42, 149-152)
@@ -399,28 +424,33 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
       for (int f = 0; f < kNField; ++f)
         PN_HIP(c, hipMemcpyAsync(d.f[b][f], F[f].data(), cells * sizeof(double), hipMemcpyHostToDevice, st));
   } else {
+    const bool d3 = (c->P.ndim == 3);
+    const int nf = d3 ? 4 : 3;
     // backgrounds: the SymDisk record of an interior ring, a ghost ring's own (uniform) values
     std::vector<double> rbg(nslot);
     for (long s = 0; s < nslot; ++s) rbg[s] = F[FR][static_cast<size_t>(s) * J];
     for (int k = 0; k < K; ++k)
       for (int i = 0; i < I; ++i) rbg[slot_of(d, i, k)] = val[i + static_cast<long>(I) * k].r;
     c->h_rbg = rbg;
-    // Sigma - Sigma_b, v_r and v_phi - v_bg, rounded once
-    std::vector<float> G[3];
-    for (int f = 0; f < 3; ++f) G[f].resize(cells);
+    // Sigma - Sigma_b, v_r, v_phi - v_bg and (3-D) v_theta, rounded once
+    std::vector<float> G[4];
+    for (int f = 0; f < nf; ++f) G[f].resize(cells);
     for (long s = 0; s < nslot; ++s)
       for (int j = 0; j < J; ++j) {
         const size_t a = static_cast<size_t>(s) * J + j;
         G[0][a] = static_cast<float>(F[FR][a] - rbg[s]);
         G[1][a] = static_cast<float>(F[FU][a]);
         G[2][a] = static_cast<float>(F[FV][a] - svbg[s]);
+        if (d3) G[3][a] = static_cast<float>(F[FW][a]);
       }
     for (int b = 0; b < 2; ++b)
-      for (int f = 0; f < 3; ++f)
+      for (int f = 0; f < nf; ++f)
         PN_HIP(c, hipMemcpyAsync(c->df.f[b][f], G[f].data(), cells * sizeof(float), hipMemcpyHostToDevice, st));
-    // the per-slot table, entry-major, every entry evaluated in FP64 and rounded once (2-D:
-    // slots are consecutive rings, r_cyl = rad); the overhang past the last slot repeats it
+    // the per-slot table, entry-major, every entry evaluated in FP64 and rounded once; the
+    // overhang past the last slot repeats it.  Radial neighbours are slots s +- 1 of the same
+    // theta row, theta neighbours s +- sr.
     const long ts = c->df.tstride;
+    const long sr = d.sr;
     std::vector<float> tab(static_cast<size_t>(kTabEntries) * ts, 0.0f);
     std::vector<double> pb(nslot), Lb(nslot);
     for (long s = 0; s < nslot; ++s) {
@@ -428,18 +458,27 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
       Lb[s] = sradc[s] * svbg[s];
     }
     for (long s = 0; s < nslot; ++s) {
-      const bool hn = s + 1 < nslot, hp = s > 0;
-      const double rad = sxc[s], xa0 = sx[s];
-      const double V0 = hn ? 0.5 * (sx[s] + sx[s + 1]) : sx[s] + 0.5 * sdx[s];
+      const long ir = s % sr, kz = s / sr;
+      const bool hn = ir + 1 < sr, hp = ir > 0;
+      const bool hnz = d3 && kz + 1 < d.sk, hpz = d3 && kz > 0;
+      const double rad = sxc[s], xa0 = sx[s], w0 = sdx[s];
+      const double th = d3 ? szc[s] : 0.5 * kPiD;
+      const double st_ = d3 ? std::sin(th) : 1.0, ct_ = d3 ? std::cos(th) : 0.0;
+      const double sh = std::sin(0.5 * (0.5 * kPiD - th));
+      const double oms = 2.0 * sh * sh;   // 1 - sin(theta) without cancellation
+      const double x1 = hn ? sx[s + 1] : xa0 + w0;
+      // radial balance of the background: -1/r^2 + L_b^2 / (r^3 sin^2) - pressure
+      const double V0 = d3 ? 0.5 * (xa0 * xa0 + x1 * x1) : 0.5 * (xa0 + x1);
       const double dpbm = hp ? pb[s] - pb[s - 1] : 0.0;
       const double abg = dpbm * V0 / (rbg[s] * sxvol[s]);
-      const double fb = -1.0 / rad / rad + Lb[s] * Lb[s] / rad / rad / rad;
+      const double fb = -1.0 / rad / rad + Lb[s] * Lb[s] / rad / rad / rad / (st_ * st_);
       const double Lf = hp ? 0.5 * (Lb[s] + Lb[s - 1]) : Lb[s];
-      const double ffb = -1.0 / xa0 / xa0 + Lf * Lf / xa0 / xa0 / xa0;
+      const double ffb = -1.0 / xa0 / xa0 + Lf * Lf / xa0 / xa0 / xa0 / (st_ * st_);
       double q[kTabEntries];
+      for (double& v : q) v = 0.0;
       q[kTabRad] = rad;
       q[kTabXa0] = xa0;
-      q[kTabDx0] = sdx[s];
+      q[kTabDx0] = w0;
       q[kTabIxc] = 1.0 / rad;
       q[kTabRb] = rbg[s];
       q[kTabDrb] = hn ? rbg[s + 1] - rbg[s] : 0.0;
@@ -454,13 +493,19 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
       q[kTabDpbm] = dpbm;
       q[kTabFface] = ffb;
       q[kTabLf] = Lf;
-      q[kTabIxa3] = 1.0 / (xa0 * xa0 * xa0);
+      q[kTabIxa3] = 1.0 / (xa0 * xa0 * xa0) / (st_ * st_);
       q[kTabVrot] = sres_v[s] / sradc[s];
       q[kTabDvol0] = sxvol[s];
+      q[kTabRc] = sradc[s];
+      q[kTabIrc] = 1.0 / sradc[s];
+      q[kTabIs2] = 1.0 / (st_ * st_);
+      q[kTabIxa3w] = 1.0 / (xa0 * xa0 * xa0);
+      q[kTabSth] = st_;
+      q[kTabCth] = ct_;
+      q[kTabOms] = oms;
       // the viscosity's quadratic fit (viscosity.cu:2-19) through the left edges of s-1, s,
       // s+1, applied to the backgrounds nu rho_b and Omega_b = L_b / r^2
       q[kTabNu] = snu[s];
-      q[kTabDnb] = q[kTabD1Ob] = q[kTabD2Ob] = 0.0;
       if (hp && hn) {
         auto fit = [&](double um, double u0, double up, double& first, double& second) {
           const double x2 = sx[s] - sx[s - 1], x3 = sx[s + 1] - sx[s - 1];
@@ -477,20 +522,53 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
         q[kTabD1Ob] = d1;
         q[kTabD2Ob] = d2;
       }
-      // paraset (parabola.cu:236-274) on the slot widths; neighbours past the grid repeat
-      // the last width (those coefficients never reach a kept parabola)
+      // paraset (parabola.cu:236-274) on the row's widths; neighbours past the row repeat
+      // its end widths (those coefficients never reach a kept parabola)
       {
-        auto w = [&](long t) { return sdx[std::min(std::max(t, 0L), nslot - 1)]; };
+        auto w = [&](long t) { return sdx[kz * sr + std::min(std::max(t - kz * sr, 0L), sr - 1)]; };
         auto an = [&](long t) { return w(t) + w(t + 1); };
         auto bn = [&](long t) { return an(t) + w(t); };
         auto cc = [&](long t) { return an(t) + w(t + 1); };
-        const double d = 1.0 / (an(s - 1) + an(s + 1));
-        q[kTabP0] = w(s) / an(s) * (1.0 + 2.0 * d * w(s + 1) * (an(s - 1) / bn(s) - an(s + 1) / cc(s)));
-        q[kTabP1] = -d * w(s) * an(s - 1) / bn(s);
-        q[kTabP2] = d * w(s + 1) * an(s + 1) / cc(s);
+        const double dd = 1.0 / (an(s - 1) + an(s + 1));
+        q[kTabP0] = w(s) / an(s) * (1.0 + 2.0 * dd * w(s + 1) * (an(s - 1) / bn(s) - an(s + 1) / cc(s)));
+        q[kTabP1] = -dd * w(s) * an(s - 1) / bn(s);
+        q[kTabP2] = dd * w(s + 1) * an(s + 1) / cc(s);
         const double e = w(s) / (an(s - 1) + w(s + 1));
         q[kTabP3] = e * bn(s - 1) / an(s);
         q[kTabP4] = e * cc(s) / an(s - 1);
+      }
+      if (d3) {
+        // theta: edges z(s), z(s) + dz(s); the background's balance of the centrifugal
+        // theta component L_b^2 cos / (r sin)^3 and the theta pressure gradient
+        const double za = sz[s], zb = sz[s] + sdz[s];
+        const double sa = std::sin(za), ca = std::cos(za), sb = std::sin(zb), cb = std::cos(zb);
+        const double dvz = szvol[s] * rad;
+        const double dpbmz = hpz ? pb[s] - pb[s - sr] : 0.0;
+        const double abgz = dpbmz * 0.5 * (sa + sb) / (rbg[s] * dvz);
+        const double r3 = rad * rad * rad;
+        const double Gc = ct_ / (r3 * st_ * st_ * st_);
+        const double Lfz = hpz ? 0.5 * (Lb[s] + Lb[s - sr]) : Lb[s];
+        const double Gf = ca / (r3 * sa * sa * sa);
+        q[kTabZa0] = za;
+        q[kTabDz0] = sdz[s];
+        q[kTabDvolZ] = dvz;
+        q[kTabSzl] = sa;
+        q[kTabCzl] = ca;
+        q[kTabSzr] = sb;
+        q[kTabCzr] = cb;
+        q[kTabDrbZ] = hnz ? rbg[s + sr] - rbg[s] : 0.0;
+        q[kTabDpbZ] = hnz ? pb[s + sr] - pb[s] : 0.0;
+        q[kTabDLbZ] = hnz ? Lb[s + sr] - Lb[s] : 0.0;
+        q[kTabDpbmZ] = dpbmz;
+        q[kTabResBgZ] = Lb[s] * Lb[s] * Gc - abgz;
+        q[kTabAbgZ] = abgz;
+        q[kTabFfaceZ] = Lfz * Lfz * Gf;
+        q[kTabLfZ] = Lfz;
+        q[kTabGfZ] = Gf;
+        q[kTabGcZ] = Gc;
+        const double is = 1.0 / st_;
+        q[kTabGc1Z] = (-is * is - 3.0 * ct_ * ct_ * is * is * is * is) / r3;
+        q[kTabGc2Z] = (8.0 * ct_ * is * is * is + 12.0 * ct_ * ct_ * ct_ * is * is * is * is * is) / r3;
       }
       for (int e = 0; e < kTabEntries; ++e) {
         tab[e * ts + s] = static_cast<float>(q[e]);
@@ -510,9 +588,13 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
     PN_HIP(c, hipMemcpyAsync(const_cast<double*>(c->df.vbg), svbg.data(), nslot * sizeof(double), hipMemcpyHostToDevice, st));
     c->df.nwin_x = f32_windows(I, kF32VariantsX[c->var_x][0]);
     c->df.nwin_y = f32_windows(J, kF32VariantsY[c->var_y][0]);
+    c->df.nwin_z = f32_windows(K, kF32VariantsX[c->var_x][0]);
     c->df.uniform_phi = 1;
     for (int j = 1; j < J; ++j)
       if (c->h_dy[j] != c->h_dy[0]) c->df.uniform_phi = 0;
+    c->df.uniform_theta = d3 ? 1 : 0;
+    for (long s = 1; d3 && s < nslot; ++s)
+      if (sdz[s] != sdz[0]) c->df.uniform_theta = 0;
     // the host vectors above must outlive the copies
     PN_HIP(c, hipStreamSynchronize(st));
   }
@@ -571,12 +653,13 @@ int pn_store_rings(pn_ctx* c, void* rings, size_t rings_bytes) {
   const size_t cells = static_cast<size_t>(nslot) * J;
   std::vector<double> F[kNField];
   for (int f = 0; f < kNField; ++f) F[f].assign(cells, 0.0);
-  std::vector<float> G[3];
+  std::vector<float> G[4];
+  const int nf = (c->P.ndim == 3) ? 4 : 3;
   if (!is_f32(c)) {
     for (int f = 0; f < kNField; ++f)
       PN_HIP(c, hipMemcpyAsync(F[f].data(), d.f[c->cur][f], cells * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   } else {
-    for (int f = 0; f < 3; ++f) {
+    for (int f = 0; f < nf; ++f) {
       G[f].resize(cells);
       PN_HIP(c, hipMemcpyAsync(G[f].data(), c->df.f[c->cur][f], cells * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     }
@@ -598,6 +681,7 @@ int pn_store_rings(pn_ctx* c, void* rings, size_t rings_bytes) {
         F[FP][a] = r * c->h_cs2[s];   // isothermal closure (ppm.cu:139)
         F[FU][a] = static_cast<double>(G[1][a]);
         F[FV][a] = c->h_vbg[s] + static_cast<double>(G[2][a]);
+        if (nf == 4) F[FW][a] = static_cast<double>(G[3][a]);
       }
   std::memset(rings, 0, rb * I * K);
   for (int k = 0; k < K; ++k)
@@ -690,8 +774,9 @@ int pn_sweep_axis(pn_ctx* c, int axis, double dt, const pn_body* planet, double 
   LaunchCtx L = lctx(c);
   if (is_f32(c)) {
     PN_LAUNCH(c, "prep", f32_prep(L, c->df));
-    if (axis == 0) PN_LAUNCH(c, "sweep_x", f32_sweep_x(L, c->df, c->var_x, c->cur));
-    else PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, c->cur, 0));
+    if (axis == 0) PN_LAUNCH(c, "sweep_x", f32_x(c, L, c->var_x, c->cur));
+    else if (axis == 1) PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, c->cur, 0));
+    else PN_LAUNCH(c, "sweep_z", f32_sweep_z(L, c->df, c->var_x, c->cur, 0));
   } else {
     hipError_t e = (axis == 0) ? fp64_sweep_x(L, c->cur) : (axis == 1) ? fp64_sweep_y(L, c->cur) : fp64_sweep_z(L, c->cur);
     if (e != hipSuccess) return fail(c, PN_ERR_HIP, "sweep %d: %s", axis, hipGetErrorString(e));
@@ -724,14 +809,17 @@ int pn_time_kernels(pn_ctx* c, int nrep, double* ms6) {
           break;
         case 1: e = launch_index_shift(L); break;
         case 2:
-          e = is_f32(c) ? f32_sweep_x(L, c->df, c->var_x, c->cur) : fp64_sweep_x(L, c->cur);
+          e = is_f32(c) ? f32_x(c, L, c->var_x, c->cur) : fp64_sweep_x(L, c->cur);
           c->cur ^= 1;
           break;
         case 3:
-          e = is_f32(c) ? f32_sweep_y(L, c->df, c->var_y, c->cur, 1) : fp64_sweep_y(L, c->cur);
+          e = is_f32(c) ? f32_sweep_y(L, c->df, c->var_y, c->cur, c->P.ndim == 2) : fp64_sweep_y(L, c->cur);
           c->cur ^= 1;
           break;
-        case 4: e = fp64_sweep_z(L, c->cur); c->cur ^= 1; break;
+        case 4:
+          e = is_f32(c) ? f32_sweep_z(L, c->df, c->var_x, c->cur, 1) : fp64_sweep_z(L, c->cur);
+          c->cur ^= 1;
+          break;
         case 5: e = launch_rotate(L); break;
       }
       return e;
@@ -756,7 +844,13 @@ int pn_tune(pn_ctx* c, int nrep, int* var_x, int* var_y) {
   int bx = c->var_x, by = c->var_y;
   for (int v = 0; v < kNumF32Variants; ++v) {
     double tx = 0.0, ty = 0.0;
-    int rc = event_time(c, c->stream, nrep, &tx, [&]() { return f32_sweep_x(L, c->df, v, c->cur); });
+    // in 3-D the radial variant also shapes the theta sweep: both are timed
+    int rc = event_time(c, c->stream, nrep, &tx, [&]() { return f32_x(c, L, v, c->cur); });
+    if (rc == PN_OK && c->P.ndim == 3) {
+      double tz = 0.0;
+      rc = event_time(c, c->stream, nrep, &tz, [&]() { return f32_sweep_z(L, c->df, v, c->cur, 0); });
+      tx += tz;
+    }
     if (rc == PN_OK) rc = event_time(c, c->stream, nrep, &ty, [&]() { return f32_sweep_y(L, c->df, v, c->cur, 0); });
     if (rc != PN_OK) return rc;
     if (tx < best_x) { best_x = tx; bx = v; }
@@ -766,6 +860,7 @@ int pn_tune(pn_ctx* c, int nrep, int* var_x, int* var_y) {
   c->var_y = by;
   c->df.nwin_x = f32_windows(c->g.imax, kF32VariantsX[bx][0]);
   c->df.nwin_y = f32_windows(c->g.jmax, kF32VariantsY[by][0]);
+  c->df.nwin_z = f32_windows(c->g.kmax, kF32VariantsX[bx][0]);
   c->tuned = true;
   if (c->gexec) {
     hipGraphExecDestroy(c->gexec);
@@ -785,6 +880,7 @@ int pn_set_variants(pn_ctx* c, int var_x, int var_y) {
   c->var_y = var_y;
   c->df.nwin_x = f32_windows(c->g.imax, kF32VariantsX[var_x][0]);
   c->df.nwin_y = f32_windows(c->g.jmax, kF32VariantsY[var_y][0]);
+  c->df.nwin_z = f32_windows(c->g.kmax, kF32VariantsX[var_x][0]);
   c->tuned = true;
   if (c->gexec) {
     hipGraphExecDestroy(c->gexec);

@@ -62,6 +62,37 @@ __device__ __forceinline__ float planet_fphi(const PlanetF& p, float rad, float 
   return -p.m * p.x * sind * (ri * ri * ri) + p.m * sind * p.inv_x2;
 }
 
+// spherical forms (forces.cu:23-110) for a planet in the midplane and a cell at (r, theta),
+// st = sin theta, oms = 1 - sin theta: D^2 = r^2 + r_p^2 - 2 r r_p cos(dphi) sin(theta)
+// = (r - r_p)^2 + 2 r r_p k with k = oms + 2 st sin^2(dphi/2) = 1 - cos(dphi) st.  Here p.m is
+// the reduced mass m/(1+m) of these forms; the star's -1/(1+m)/r^2 = -1/r^2 + p.m/r^2 is
+// left to the caller.
+__device__ __forceinline__ float planet_fr_sph(const PlanetF& p, float r, float st, float oms, float s, float c) {
+  const float s2 = s * s;
+  const float cosd = 1.0f - 2.0f * s2;
+  const float k = oms + 2.0f * st * s2;
+  const float dr = r - p.x;
+  const float ri = __frsqrt_rn(dr * dr + 2.0f * r * p.x * k + p.rs2);
+  return -p.m * (dr + p.x * k) * (ri * ri * ri) - p.m * st * cosd * p.inv_x2;
+}
+__device__ __forceinline__ float planet_fphi_sph(const PlanetF& p, float r, float st, float oms, float s, float c) {
+  const float s2 = s * s;
+  const float sind = 2.0f * s * c;
+  const float k = oms + 2.0f * st * s2;
+  const float dr = r - p.x;
+  const float ri = __frsqrt_rn(dr * dr + 2.0f * r * p.x * k + p.rs2);
+  return -p.m * p.x * sind * (ri * ri * ri) + p.m * st * sind * p.inv_x2;
+}
+__device__ __forceinline__ float planet_fth_sph(const PlanetF& p, float r, float st, float ct, float oms, float s,
+                                                float c) {
+  const float s2 = s * s;
+  const float cosd = 1.0f - 2.0f * s2;
+  const float k = oms + 2.0f * st * s2;
+  const float dr = r - p.x;
+  const float ri = __frsqrt_rn(dr * dr + 2.0f * r * p.x * k + p.rs2);
+  return p.m * p.x * cosd * ct * (ri * ri * ri) - p.m * cosd * ct / st * p.inv_x2;
+}
+
 // log(1 + x): 2 atanh(x / (2 + x)) as an odd series for |x| < 1/4, else the library log
 // (both evaluated, one selected: no divergent branch)
 __device__ __forceinline__ float log1p_acc(float x) {
@@ -140,16 +171,22 @@ __device__ __forceinline__ Parab<float, K> parabola_off(int nmin, int nmaxp, con
   return o;
 }
 
+// sweep geometries of the FP32 engine: radial (cylindrical, 2-D), azimuthal, radial
+// (spherical, 3-D), polar angle theta (3-D)
+enum SweepGeo { kGeoRcyl = 0, kGeoPhi = 1, kGeoRsph = 2, kGeoTheta = 3 };
+
 template <int K>
 struct WinP {
   // carried state: deviations from the cell's backgrounds
   C<float, K> r;   // rho - rho_b
-  C<float, K> u;   // sweep-normal velocity (radial: v_r; azimuthal: v_phi - v_bg)
-  C<float, K> v;   // radial: l - L_b (l = r v_phi); azimuthal: v_r
-  // backgrounds (azimuthal sweep: uniform in the window, differences zero)
+  C<float, K> u;   // sweep-normal velocity: v_r (radial), v_phi - v_bg (azimuthal), v_theta (theta)
+  C<float, K> v;   // radial: l - L_b (l = r_cyl v_phi); azimuthal and theta: v_r
+  C<float, K> w;   // 3-D only -- spherical radial: r v_theta; azimuthal: v_theta; theta: l - L_b
+  // backgrounds (azimuthal sweep: uniform in the window, differences zero); dLb is the
+  // difference of L_b along the sweep
   C<float, K> rb, drb, pb, dpb, Lb, dLb, cs, cs2;
-  // geometry
-  C<float, K> rad, xa0, dx0, dvol0, vrot, ts, tc;
+  // geometry; sth / oms: sin theta and 1 - sin theta of the cell centre (3-D planet terms)
+  C<float, K> rad, xa0, dx0, dvol0, vrot, ts, tc, sth, oms;
 };
 
 // the per-slot table entries of this lane's K cells, entry-major (entry e of slot s at
@@ -179,15 +216,54 @@ struct TabLane {
   }
 };
 
-// AX 0 radial (cylindrical), AX 1 azimuthal.  tx: the per-slot table of this lane's cells
-// (radial sweep only; the radial balance terms are read where they are used).  P0: paraset
-// of the Eulerian grid (its values outside the window's paraset ranges never reach a
-// parabola the sweep keeps).  wres: the azimuthal grid's uniform angular rate.  On return
-// cells [6, nmax-6) hold the update.
-template <int K, int AX>
-__device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, const Para<float, K>& P0, int nmax,
+// the same for cells `step` slots apart (the theta sweep: one theta row of slots); c0 = this
+// lane's first cell of the window, reads past cell cmax (the window's last) repeat it
+template <int K>
+struct TabLaneS {
+  const float* __restrict__ p;
+  int stride, base, step, c0, cmax;
+  __device__ __forceinline__ C<float, K> get(int e) const {
+    const float* q = p + (long)e * stride + base;
+    C<float, K> o;
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      const int c = c0 + h;
+      o[h] = q[(long)step * (c < cmax ? c : cmax)];
+    }
+    return o;
+  }
+};
+
+// cos(a + d) - cos(a) from sin a, cos a and a small d (|d| < ~0.3), without cancellation
+__device__ __forceinline__ float dcos(float sa, float ca, float d) {
+  const float d2 = d * d;
+  const float sd = d * (1.0f - d2 * (1.0f / 6.0f) * (1.0f - d2 * (1.0f / 20.0f)));
+  const float omc = 0.5f * d2 * (1.0f - d2 * (1.0f / 12.0f) * (1.0f - d2 * (1.0f / 30.0f)));
+  return -sa * sd - ca * omc;
+}
+
+// sin / cos / 1 - sin of (theta + e) from those of theta and a small e
+__device__ __forceinline__ void rot_theta(float st, float ct, float oms, float e, float& so, float& co,
+                                          float& omso) {
+  const float e2 = e * e;
+  const float se = e * (1.0f - e2 * (1.0f / 6.0f) * (1.0f - e2 * (1.0f / 20.0f)));
+  const float omc = 0.5f * e2 * (1.0f - e2 * (1.0f / 12.0f) * (1.0f - e2 * (1.0f / 30.0f)));
+  so = st * (1.0f - omc) + ct * se;
+  co = ct * (1.0f - omc) - st * se;
+  omso = oms + st * omc - ct * se;
+}
+
+// The window update.  G: sweep geometry; NDIM: 2 or 3 (the azimuthal sweep carries w and uses
+// the spherical planet terms in 3-D).  tx: the per-slot table of this lane's cells (TabLane
+// or TabLaneS; radial and theta sweeps read their balance terms where they are used).  P0: paraset of the
+// Eulerian grid (its values outside the window's paraset ranges never reach a parabola the
+// sweep keeps).  wres: the azimuthal grid's uniform angular rate.  rsph: the spherical
+// radius of an azimuthal 3-D ring.  On return cells [6, nmax-6) hold the update.
+template <int K, int G, int NDIM, class Tab>
+__device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para<float, K>& P0, int nmax,
                                            float dt, float wres, const PlanetF& pl, bool planet, bool flat_on,
-                                           float sqrt_gam) {
+                                           float sqrt_gam, float rsph = 0.0f) {
+  constexpr bool RAD = (G == kGeoRcyl || G == kGeoRsph);
   const float hdt = 0.5f * dt;
   const float fourthd = 1.3333333333333333f;
   const float smallr = 1.0e-15f;
@@ -226,29 +302,64 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, con
 
   // ---- states (states.cu:3-73)
   C<float, K> fhalf;   // hdt * force at the left edge
-  if constexpr (AX == 0) {
+  if constexpr (RAD) {
     C<float, K> vm = prev1(W.v);
     const C<float, K> ffb = tx.get(kTabFface), Lf = tx.get(kTabLf), ixa3 = tx.get(kTabIxa3);
+    C<float, K> wm, ixa3w;
+    if constexpr (G == kGeoRsph) {
+      wm = prev1(W.w);
+      ixa3w = tx.get(kTabIxa3w);
+    }
 #pragma unroll
     for (int h = 0; h < K; ++h) {
-      // l at the edge = L_f + dlf; f = f_face + dlf (2 L_f + dlf) / xa0^3 (+ planet)
+      // l at the edge = L_f + dlf; f = f_face + dlf (2 L_f + dlf) / (xa0^3 sin^2) (+ w^2 / xa0^3)
       const float dlf = 0.5f * (W.v[h] + vm[h]);
       float f = ffb[h] + dlf * (2.0f * Lf[h] + dlf) * ixa3[h];
+      if constexpr (G == kGeoRsph) {
+        const float wf = 0.5f * (W.w[h] + wm[h]);
+        f += wf * wf * ixa3w[h];
+      }
       if (planet) {
         float s, c;
         rot_half(W.ts[h], W.tc[h], hdt * (W.vrot[h] - pl.vy), s, c);
-        f += planet_fr(pl, W.xa0[h], s, c);
+        if constexpr (G == kGeoRcyl) {
+          f += planet_fr(pl, W.xa0[h], s, c);
+        } else {
+          const float ix = 1.0f / W.xa0[h];
+          f += pl.m * ix * ix + planet_fr_sph(pl, W.xa0[h], W.sth[h], W.oms[h], s, c);
+        }
       }
       fhalf[h] = hdt * f;
     }
-  } else {
+  } else if constexpr (G == kGeoPhi) {
 #pragma unroll
     for (int h = 0; h < K; ++h) {
       float f = 0.0f;
       if (planet) {
         float s, c;
         rot_half(W.ts[h], W.tc[h], -0.5f * W.dx0[h] + hdt * (W.vrot[h] - pl.vy), s, c);
-        f = planet_fphi(pl, W.rad[h], s, c);
+        if constexpr (NDIM == 2) f = planet_fphi(pl, W.rad[h], s, c);
+        else f = planet_fphi_sph(pl, rsph, W.sth[h], W.oms[h], s, c);
+      }
+      fhalf[h] = hdt * f;
+    }
+  } else {
+    // theta edge: f = f_face + dlf (2 L_f + dlf) cos / (r sin)^3 of the edge (+ planet)
+    C<float, K> wm = prev1(W.w);
+    const C<float, K> ffb = tx.get(kTabFfaceZ), Lf = tx.get(kTabLfZ), Gf = tx.get(kTabGfZ);
+    C<float, K> szl, czl;
+    if (planet) {
+      szl = tx.get(kTabSzl);
+      czl = tx.get(kTabCzl);
+    }
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      const float dlf = 0.5f * (W.w[h] + wm[h]);
+      float f = ffb[h] + dlf * (2.0f * Lf[h] + dlf) * Gf[h];
+      if (planet) {
+        float s, c;
+        rot_half(W.ts[h], W.tc[h], hdt * (W.vrot[h] - pl.vy), s, c);
+        f += planet_fth_sph(pl, W.rad[h], szl[h], czl[h], 1.0f - szl[h], s, c);
       }
       fhalf[h] = hdt * f;
     }
@@ -258,7 +369,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, con
   for (int h = 0; h < K; ++h) {
     // sqrt(gam p / rho) = sqrt(gam) c_s for the isothermal closure
     float s = hdt * sqrt_gam * W.cs[h] / W.dx0[h];
-    if constexpr (AX == 1) s = s / W.rad[h];
+    if constexpr (G == kGeoPhi || G == kGeoTheta) s = s / W.rad[h];
     csn[h] = s;
     csfac[h] = 1.0f - fourthd * s;
   }
@@ -273,7 +384,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, con
   // interface c: left state = right edge of cell c-1, right state = left edge of cell c, as
   // relative jumps against the backgrounds of cell c-1
   C<float, K> epl, epr, erl, err, ul, ur;
-  if constexpr (AX == 0) {
+  if constexpr (G != kGeoPhi) {
     C<float, K> pp, re, le;
 #pragma unroll
     for (int h = 0; h < K; ++h) pp[h] = W.r[h] * W.cs2[h];
@@ -316,7 +427,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, con
   //   t1 = (sqrt(pl/rl) + sqrt(pr/rr))/2, t2 = -(ur-ul)/(2 t1), p* = sqrt(pl pr) e^t2,
   //   u* = ul - t1 log(p*/pl); here with pl = p_b(1+epl), ..., p* = p_b(1+em)
   C<float, K> umid, em;
-  if constexpr (AX == 0) {
+  if constexpr (G != kGeoPhi) {
     C<float, K> csm = prev1(W.cs);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
@@ -344,13 +455,13 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, con
   // ---- evolve (evolve.cu:3-166): edges as displacements from the Eulerian edges.  The
   // reference moves edges [3, nmax-2) and cells [3, nmax-3).
   C<float, K> smv, dlt;
-  const float Wd = (AX == 1) ? dt * wres : 0.0f;
+  const float Wd = (G == kGeoPhi) ? dt * wres : 0.0f;
 #pragma unroll
   for (int h = 0; h < K; ++h) {
     const int c = cell_of(K, h);
     const bool mv = (c >= 3 && c < nmax - 2);
     float s;
-    if constexpr (AX == 0) s = dt * umid[h];
+    if constexpr (RAD) s = dt * umid[h];
     else s = dt * umid[h] / W.rad[h];
     smv[h] = mv ? s : 0.0f;
     dlt[h] = mv ? s + Wd : 0.0f;
@@ -368,12 +479,19 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, con
     }
   }
   C<float, K> rL;   // Lagrangian density deviation
+  C<float, K> szl, czl;   // theta sweep: sin / cos of the left edge
+  if constexpr (G == kGeoTheta) {
+    szl = tx.get(kTabSzl);
+    czl = tx.get(kTabCzl);
+  }
   {
     C<float, K> emp = next1(em);
-    if constexpr (AX == 0) {
-      C<float, K> pbm = prev1(W.pb), dltxp = next1(dltx);
+    if constexpr (RAD) {
+      C<float, K> pbm = prev1(W.pb), dltxp = next1(dltx), xa0p = next1(W.xa0);
       const C<float, K> tixc = tx.get(kTabIxc), res_bg = tx.get(kTabResBg), abg = tx.get(kTabAbg),
                         dpbm = tx.get(kTabDpbm);
+      C<float, K> is2;
+      if constexpr (G == kGeoRsph) is2 = tx.get(kTabIs2);
 #pragma unroll
       for (int h = 0; h < K; ++h) {
         const int c = cell_of(K, h);
@@ -383,45 +501,75 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, con
         const float rho = W.rb[h] + W.r[h];
         const float dl = W.v[h];
         // force at the Eulerian centre (f1) and at the moved centre (f2) less the
-        // background's: gravity -1/r^2 and centrifugal l^2/r^3 with l = L_b + dl
-        const float q1 = dl * (2.0f * W.Lb[h] + dl) * ixc3;
+        // background's: gravity -1/r^2 and centrifugal l^2/r^3 (spherical: l^2/(r^3 sin^2)
+        // + w^2/r^3) with l = L_b + dl
+        const float il = (G == kGeoRsph) ? ixc3 * is2[h] : ixc3;
+        float q1 = dl * (2.0f * W.Lb[h] + dl) * il;
+        if constexpr (G == kGeoRsph) q1 += W.w[h] * W.w[h] * ixc3;
         const float dxc = dltx[h] + 0.5f * Dl[h];             // moved centre - centre
         const float e = dxc * ixc;
         const float g2 = -e * (2.0f - e * (3.0f - 4.0f * e));    // (1+e)^-2 - 1
         const float g3 = -e * (3.0f - e * (6.0f - 10.0f * e));   // (1+e)^-3 - 1
-        const float B = W.Lb[h] * W.Lb[h] * ixc3;
+        const float B = W.Lb[h] * W.Lb[h] * il;
         float fdiff = -(ixc * ixc) * g2 + (B + q1) * g3;
         if (planet) {
-          fdiff += planet_fr(pl, W.rad[h], W.ts[h], W.tc[h]);
           float s, cc;
           rot_half(W.ts[h], W.tc[h], dt * (W.vrot[h] - pl.vy), s, cc);
-          fdiff += planet_fr(pl, W.rad[h] + dxc, s, cc);
+          if constexpr (G == kGeoRcyl) {
+            fdiff += planet_fr(pl, W.rad[h], W.ts[h], W.tc[h]) + planet_fr(pl, W.rad[h] + dxc, s, cc);
+          } else {
+            // the star pulls with 1/(1+m): -1/r^2 + plm/r^2 at both centres
+            fdiff += pl.m * ixc * ixc * (2.0f + g2) + planet_fr_sph(pl, W.rad[h], W.sth[h], W.oms[h], W.ts[h], W.tc[h]) +
+                     planet_fr_sph(pl, W.rad[h] + dxc, W.sth[h], W.oms[h], s, cc);
+          }
         }
         // pressure: (p*(c+1) - p*(c)) area / (rho dvol0), area = V0 + dV, and
         //   p*(c+1) - p*(c) = dp_bm + dp,  dp = p_b(c) em(c+1) - p_b(c-1) em(c)
         const float dp = W.pb[h] * emp[h] - pbm[h] * em[h];
-        const float dV = 0.25f * (dltx[h] + dltxp[h]);
+        float V0, dV;
+        const float d = dltx[h], x0 = W.xa0[h], w0 = W.dx0[h], D = Dl[h];
+        if constexpr (G == kGeoRcyl) {
+          V0 = W.rad[h];
+          dV = 0.25f * (d + dltxp[h]);
+        } else {
+          // area factor at an edge x0 moved by d: (x0 + d)^2 - d (x0 + d) + d^2/3 = x0^2 + d (x0 + d/3)
+          const float dp_ = dltxp[h];
+          V0 = 0.5f * (x0 * x0 + xa0p[h] * xa0p[h]);
+          dV = 0.5f * (d * (x0 + d * (1.0f / 3.0f)) + dp_ * (xa0p[h] + dp_ * (1.0f / 3.0f)));
+        }
         const float acc = res_bg[h] + q1 + 0.5f * fdiff + abg[h] * (W.r[h] / rho) -
-                          (dpbm[h] * dV + dp * (W.rad[h] + dV)) / (rho * W.dvol0[h]);
-        // Lagrangian volume change, offset form of dvol = dxl (xa + dxl/2)
-        const float d = dltx[h], x0 = W.xa0[h], w0 = W.dx0[h];
-        const float ddv = w0 * (d + 0.5f * Dl[h]) + Dl[h] * (x0 + d + 0.5f * w0 + 0.5f * Dl[h]);
+                          (dpbm[h] * dV + dp * (V0 + dV)) / (rho * W.dvol0[h]);
+        // Lagrangian volume change in offset form
+        float ddv;
+        if constexpr (G == kGeoRcyl) {
+          // dvol = dxl (xa + dxl/2)
+          ddv = w0 * (d + 0.5f * D) + D * (x0 + d + 0.5f * w0 + 0.5f * D);
+        } else {
+          // dvol = dxl (xa (xa + dxl) + dxl^2/3)
+          const float a1 = x0 + d;
+          ddv = w0 * d * (2.0f * x0 + d) + D * a1 * a1 + w0 * w0 * d + (2.0f * w0 * D + D * D) * a1 + w0 * w0 * D +
+                w0 * D * D + D * D * D * (1.0f / 3.0f);
+        }
         const float dvol = W.dvol0[h] + ddv;
         rL[h] = in ? (W.r[h] * W.dvol0[h] - W.rb[h] * ddv) / dvol : W.r[h];
         if (in) W.u[h] = W.u[h] + dt * acc;
       }
-    } else {
+    } else if constexpr (G == kGeoPhi) {
 #pragma unroll
       for (int h = 0; h < K; ++h) {
         const int c = cell_of(K, h);
         const bool in = (c >= 3 && c < nmax - 3);
         float f = 0.0f;
         if (planet) {
-          f = planet_fphi(pl, W.rad[h], W.ts[h], W.tc[h]);
           const float off = dltx[h] + 0.5f * Dl[h] + dt * (W.vrot[h] - pl.vy);
           float s, cc;
           rot_half(W.ts[h], W.tc[h], off, s, cc);
-          f += planet_fphi(pl, W.rad[h], s, cc);
+          if constexpr (NDIM == 2) {
+            f = planet_fphi(pl, W.rad[h], W.ts[h], W.tc[h]) + planet_fphi(pl, W.rad[h], s, cc);
+          } else {
+            f = planet_fphi_sph(pl, rsph, W.sth[h], W.oms[h], W.ts[h], W.tc[h]) +
+                planet_fphi_sph(pl, rsph, W.sth[h], W.oms[h], s, cc);
+          }
         }
         const float rho = W.rb[h] + W.r[h];
         const float dpm = W.pb[h] * (emp[h] - em[h]);   // p_b is uniform along the ring
@@ -430,6 +578,51 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, con
         const float dvol = W.dvol0[h] + ddv;
         rL[h] = in ? (W.r[h] * W.dvol0[h] - W.rb[h] * ddv) / dvol : W.r[h];
         if (in) W.u[h] = un;
+      }
+    } else {
+      // theta: centrifugal theta component L^2 G(theta), G = cos / (r sin)^3, minus the
+      // background's; G at the moved centre by its Taylor series; the area factor at an edge
+      // moved by d is (cos x0 - cos(x0 + d)) / d (sin x0 for d = 0)
+      C<float, K> pbm = prev1(W.pb), dltxp = next1(dltx), szlp = next1(szl);
+      const C<float, K> res_bg = tx.get(kTabResBgZ), abg = tx.get(kTabAbgZ), dpbm = tx.get(kTabDpbmZ),
+                        Gc = tx.get(kTabGcZ), G1 = tx.get(kTabGc1Z), G2 = tx.get(kTabGc2Z),
+                        szr = tx.get(kTabSzr), czr = tx.get(kTabCzr), cth = tx.get(kTabCth);
+      C<float, K> vfd;   // area factor of the moved left edge minus sin x0
+#pragma unroll
+      for (int h = 0; h < K; ++h) {
+        const float d = dltx[h], d2 = d * d;
+        vfd[h] = szl[h] * d2 * (-1.0f / 6.0f + d2 * (1.0f / 120.0f)) + czl[h] * d * (0.5f - d2 * (1.0f / 24.0f));
+      }
+      C<float, K> vfdp = next1(vfd);
+#pragma unroll
+      for (int h = 0; h < K; ++h) {
+        const int c = cell_of(K, h);
+        const bool in = (c >= 3 && c < nmax - 3);
+        const float rho = W.rb[h] + W.r[h];
+        const float dl = W.w[h];
+        const float qq = dl * (2.0f * W.Lb[h] + dl);
+        const float L2 = W.Lb[h] * W.Lb[h] + qq;
+        const float e = dltx[h] + 0.5f * Dl[h];   // angle of the moved centre - centre
+        // 4 x (the mean of the forces at both centres - the background's): centrifugal
+        // L^2 (G(theta + e) - G(theta)) and the planet at both centres, twice
+        float fdiff = L2 * e * (2.0f * G1[h] + e * G2[h]);
+        if (planet) {
+          float s, cc, so, co, omso;
+          rot_half(W.ts[h], W.tc[h], dt * (W.vrot[h] - pl.vy), s, cc);
+          rot_theta(W.sth[h], cth[h], W.oms[h], e, so, co, omso);
+          fdiff += 2.0f * (planet_fth_sph(pl, W.rad[h], W.sth[h], cth[h], W.oms[h], W.ts[h], W.tc[h]) +
+                           planet_fth_sph(pl, W.rad[h], so, co, omso, s, cc));
+        }
+        const float dp = W.pb[h] * emp[h] - pbm[h] * em[h];
+        const float V0 = 0.5f * (szl[h] + szlp[h]);
+        const float dV = 0.5f * (vfd[h] + vfdp[h]);
+        const float acc = res_bg[h] + qq * Gc[h] + 0.25f * fdiff + abg[h] * (W.r[h] / rho) -
+                          (dpbm[h] * dV + dp * (V0 + dV)) / (rho * W.dvol0[h]);
+        // dvol = r (cos(x0 + d) - cos(x1 + d + D)) with x1 the right edge
+        const float ddv = W.rad[h] * (dcos(szl[h], czl[h], dltx[h]) - dcos(szr[h], czr[h], dltx[h] + Dl[h]));
+        const float dvol = W.dvol0[h] + ddv;
+        rL[h] = in ? (W.r[h] * W.dvol0[h] - W.rb[h] * ddv) / dvol : W.r[h];
+        if (in) W.u[h] = W.u[h] + dt * acc;
       }
     }
   }
@@ -444,8 +637,10 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, con
 #pragma unroll
   for (int h = 0; h < K; ++h) {
     const float d = dltx[h];
-    if constexpr (AX == 0) lvol[h] = d * (W.xa0[h] + 0.5f * d);
-    else lvol[h] = d * W.rad[h];
+    if constexpr (G == kGeoRcyl) lvol[h] = d * (W.xa0[h] + 0.5f * d);
+    else if constexpr (G == kGeoRsph) lvol[h] = d * (W.xa0[h] * (W.xa0[h] + d) + d * d * (1.0f / 3.0f));
+    else if constexpr (G == kGeoPhi) lvol[h] = d * W.rad[h];
+    else lvol[h] = -W.rad[h] * dcos(szl[h], czl[h], d);
     delC[h] = (d >= 0.0f) ? 1.0f : 0.0f;
     float e;
     if (d >= 0.0f) {
@@ -475,7 +670,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, con
     face(Q, W.drb, fv);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
-      if constexpr (AX == 0) {
+      if constexpr (G != kGeoPhi) {
         frd[h] = (W.rb[h] + fv[h]) * lvol[h];
         fr[h] = frd[h];
       } else {
@@ -509,11 +704,10 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, con
       if (c >= 6 && c < nmax - 6) X[h] = (X[h] * mass[h] + (fl[h] - flp[h])) * invm[h];
     }
   };
-  remap_plain(W.u);
-  if constexpr (AX == 0) {
-    // l - L_b: fluxes relative to the cell's own L_b; the right edge's flux comes from the
-    // next cell's frame, L_b(c+1) = L_b(c) + dL_b
-    Parab<float, K> Q = parabola_off(3, nmax - 3, flat, W.v, W.dLb, P1);
+  // l - L_b: fluxes relative to the cell's own L_b; the right edge's flux comes from the
+  // next cell's frame, L_b(c+1) = L_b(c) + dL_b
+  auto remap_bg = [&](C<float, K>& X) {
+    Parab<float, K> Q = parabola_off(3, nmax - 3, flat, X, W.dLb, P1);
     C<float, K> fl;
     face(Q, W.dLb, fl);
 #pragma unroll
@@ -522,11 +716,14 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const TabLane<K>& tx, con
 #pragma unroll
     for (int h = 0; h < K; ++h) {
       const int c = cell_of(K, h);
-      if (c >= 6 && c < nmax - 6) W.v[h] = (W.v[h] * mass[h] + (fl[h] - (flp[h] + W.dLb[h] * frp[h]))) * invm[h];
+      if (c >= 6 && c < nmax - 6) X[h] = (X[h] * mass[h] + (fl[h] - (flp[h] + W.dLb[h] * frp[h]))) * invm[h];
     }
-  } else {
-    remap_plain(W.v);
-  }
+  };
+  remap_plain(W.u);
+  if constexpr (RAD) remap_bg(W.v);
+  else remap_plain(W.v);
+  if constexpr (G == kGeoRsph || (G == kGeoPhi && NDIM == 3)) remap_plain(W.w);
+  if constexpr (G == kGeoTheta) remap_bg(W.w);
 }
 
 // Explicit viscosity after the radial remap (device_viscosity_r, viscosity.cu:370-421) on

@@ -170,3 +170,57 @@ def test_f32_totals_track_fp64():
     say(f"  fp32 {a}\n  fp64 {b}")
     assert abs(a[0] - b[0]) <= 1e-12 * b[0] and abs(a[1] - b[1]) <= 1e-12 * abs(b[1])
     assert abs(a[2] - b[2]) <= 1e-9 * b[2] and abs(a[3] - b[3]) <= 1e-9 * abs(b[3])
+
+
+HPI = 1.570796326794896619231321691639751442
+
+
+def disk3d(nr, nphi, nth):
+    p = configs.disk2d(nr, nphi, q=3.0e-5).replace(
+        ndim=3, ngeomx=2, ngeomy=4, ngeomz=5, p_alpha=1.5, p_beta=1.0, zmin=HPI - 0.15, zmax=HPI + 0.15)
+    return configs.Workload(f"disk3d_{nr}x{nphi}x{nth}", abi.PnGrid(nr, nphi, nth), p, seed=11, steps=10)
+
+
+def test_f32_3d_load_store_roundtrip():
+    w = disk3d(24, 72, 12)
+    say("case: fp32 3-D roundtrip", w.name)
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    with abi.Context(w.grid, w.params, abi.PN_FP32) as ctx:
+        ctx.load(d)
+        out = ctx.store()
+    val = d.val_array()
+    for f, bg in (("r", val[:, 0:1]), ("v", val[:, 3:4]), ("u", 0.0), ("w", 0.0)):
+        dev = np.abs(d.rings[f] - bg)
+        err = np.abs(out[f] - d.rings[f])
+        assert np.all(err <= 2.0 ** -24 * dev + 4e-16 * np.abs(d.rings[f]) + 1e-300), f
+
+
+# 3-D bars: Sigma and v_phi as in 2-D; v_r and v_theta start at zero and are judged
+# against their own largest values
+@pytest.mark.parametrize("w,n", [(disk3d(24, 72, 12), 20), (disk3d(64, 192, 32), 50)],
+                         ids=["disk3d_24x72x12", "disk3d_64x192x32"])
+def test_f32_3d_vs_fp64(w, n):
+    say("case: fp32 3-D vs fp64", w.name, n, "steps")
+    a, sa = run(w, abi.PN_FP32, n)
+    b, sb = run(w, abi.PN_FP64, n)
+    assert np.array_equal(a["rot_j"], b["rot_j"])
+    assert abs(sa.dt - sb.dt) <= 1e-5 * sb.dt
+    for f, t1, tinf in (("r", 1e-6, 1e-4), ("v", 1e-6, 1e-5), ("u", 1e-3, 1e-2), ("w", 1e-3, 1e-2)):
+        l1, li = rel(lab(a, f), lab(b, f))
+        say(f"  {f}: L1 {l1:.3e} Linf {li:.3e}")
+        assert l1 <= t1 and li <= tinf, (f, l1, li)
+
+
+def test_f32_3d_every_variant_and_graph():
+    """Every radial / theta variant is a valid 3-D step, and graph replay equals eager."""
+    w = disk3d(24, 72, 12)
+    say("case: fp32 3-D variants", w.name)
+    b, _ = run(w, abi.PN_FP64, 10)
+    for vx in range(4):
+        a, _ = run(w, abi.PN_FP32, 10, variants=(vx, 3))
+        l1, li = rel(lab(a, "r"), lab(b, "r"))
+        say(f"  variant {vx}: rho L1 {l1:.3e} Linf {li:.3e}")
+        assert l1 <= 1e-6 and li <= 1e-4
+    e, _ = run(w, abi.PN_FP32, 6, variants=(0, 3), graph=False)
+    g, _ = run(w, abi.PN_FP32, 6, variants=(0, 3), graph=True)
+    assert np.array_equal(e.view(np.uint8), g.view(np.uint8))

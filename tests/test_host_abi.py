@@ -155,6 +155,18 @@ def test_batch_argument_errors_before_any_device_call():
     with pytest.raises(abi.PnError) as e:
         abi.Batch([], [])
     assert e.value.code == abi.PN_ERR_ARG
+    # batches hold 2-D disks
+    p3 = configs.disk2d(24, 72).replace(ndim=3, ngeomx=2, ngeomy=4, ngeomz=5, zmin=1.4, zmax=1.7)
+    with pytest.raises(abi.PnError) as e:
+        abi.Batch([PnGrid(24, 72, 12)], [p3])
+    assert e.value.code == abi.PN_ERR_UNSUPPORTED
+
+
+def test_fp32_parameter_errors_before_any_device_call():
+    # the FP32 path is built for the isothermal closure only
+    with pytest.raises(abi.PnError) as e:
+        abi.Context(PnGrid(32, 48, 1), configs.disk2d(32, 48, eos=2, gam=1.4), abi.PN_FP32)
+    assert e.value.code == abi.PN_ERR_UNSUPPORTED
 
 
 @pytest.mark.skipif(_gpu_visible(), reason="checks the no-device error path")
