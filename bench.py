@@ -18,7 +18,8 @@ The roofline entry prices the dominant sweep kernel of the measured path -- its 
 bytes per launch (one read and one write of the minimal state fields of every cell: 3 for
 the 2-D isothermal disk, one more in 3-D, one more for a non-isothermal EOS) over its
 mean launch time, measured with HIP events on the context's stream -- against HBM
-bandwidth.  The CPU baseline times the FP64 oracle on the host cores on a bounded prefix of
+bandwidth (and against the bandwidth a plain 1 GiB device copy reaches on the same GPU,
+copy_measured_gbs).  The CPU baseline times the FP64 oracle on the host cores on a bounded prefix of
 the same workload.
 """
 from __future__ import annotations
@@ -133,6 +134,14 @@ def batched_run(steps: int, warmup: int) -> dict:
                                  "worst": dict(worst, fp64_from_fp32_rounded_start_l1_rho=floor)}}
 
 
+def copy_probe() -> int:
+    """The library's plain copy kernel on 256 MiB, 5 times after one warm-up: run under the
+    HBM byte counters to scale them (the sweeps' traffic is quoted in the copy's units)."""
+    gbs = abi.copy_bandwidth(256 << 20, 5)
+    print(json.dumps({"copy_probe": {"bytes": 256 << 20, "launches": 6, "gbs": gbs}}), flush=True)
+    return 0
+
+
 def lab(state: np.ndarray, f: str) -> np.ndarray:
     """Field f with each ring's data rolled to lab columns (data j sits at j + rot_j)."""
     a = state[f]
@@ -155,7 +164,10 @@ def main(argv=None) -> int:
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 leg of an FP32 run")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched parameter-sweep leg")
     ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--copy-probe", action="store_true", help="only run a plain device copy of known size")
     a = ap.parse_args(argv)
+    if a.copy_probe:
+        return copy_probe()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported (the disk is not "
               f"sharded across GPUs in this build)", file=sys.stderr)
@@ -181,9 +193,11 @@ def main(argv=None) -> int:
             traffic = json.loads(PROFILE_TRAFFIC.read_text()).get(f"{w.name}:{a.precision}:{dom}")
         except Exception:
             traffic = None
+    copy_gbs = abi.copy_bandwidth()
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "kernel": dom, "kernel_ms": km[dom], "bytes_per_launch": bytes_per_launch, "kernels_ms": km}
+                "kernel": dom, "kernel_ms": km[dom], "bytes_per_launch": bytes_per_launch, "kernels_ms": km,
+                "copy_measured_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs}
     out = {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": run["wall"] / a.steps * 1e3, "higher_is_better": True,

@@ -194,15 +194,24 @@ int pn_run(pn_ctx* ctx, int nsteps, int graph, pn_run_state* st);
  * seconds per step, main.cu:430-437).  The state advances while it is timed. */
 int pn_time_kernels(pn_ctx* ctx, int nrep, double* ms6);
 
-/* Autotune the FP32 sweeps: time every (cells per lane, waves per workgroup)
- * variant of the radial and the azimuthal sweep on the current state (nrep launches
- * each, state left unchanged) and keep the fastest.  *var_x / *var_y receive the
- * choice (may be NULL).  FP64 contexts have one variant (0).  pn_run tunes an FP32
- * context on its first call.  No counterpart in the reference. */
+/* Autotune the FP32 sweeps: time every workgroup-shape variant of the radial sweep (in
+ * 3-D: radial + theta sweep, which share it) and of the azimuthal sweep on the current
+ * state (nrep launches each, state left unchanged) and keep the fastest.  *var_x / *var_y
+ * receive the choice (may be NULL).  FP64 contexts have one variant (0).  pn_run tunes an
+ * FP32 context on its first call.  No counterpart in the reference. */
 int pn_tune(pn_ctx* ctx, int nrep, int* var_x, int* var_y);
-/* Pin the FP32 sweep variants (0..3: K = 2, 2, 4, 4 cells per lane with 2, 4, 2, 4
- * waves per workgroup) instead of tuning. */
+/* Pin the FP32 sweep variants instead of tuning.  var_x 0..3, radial / theta sweeps as
+ * (cells per lane, waves per workgroup, lab columns per wave): (2,4,2), (2,8,1), (2,16,1),
+ * (2,4,4).  var_y 0..3, azimuthal sweep as (cells per lane, waves per workgroup): (2,2),
+ * (2,4), (4,2), (4,4). */
 int pn_set_variants(pn_ctx* ctx, int var_x, int var_y);
+
+/* The achievable HBM bandwidth: a plain device copy of nbytes (a multiple of 16, at least
+ * 1 MiB) launched reps times back to back after one warm-up, timed with HIP events on a
+ * stream of its own; *gbs = 2 nbytes reps / time in GB/s (read + write).  Allocates and
+ * frees its two buffers.  No counterpart in the reference; the measurements quote kernel
+ * bandwidth against it. */
+int pn_copy_bandwidth(size_t nbytes, int reps, double* gbs);
 
 /* Mass and angular-momentum totals in FP64 with compensated summation. */
 int pn_totals(pn_ctx* ctx, double* mass, double* angmom);

@@ -114,6 +114,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_run_begin": (i, [vp, P(PnRunState)]),
         "pn_run": (i, [vp, i, i, P(PnRunState)]),
         "pn_totals": (i, [vp, P(d), P(d)]),
+        "pn_copy_bandwidth": (i, [sz, i, P(d)]),
         "pn_save_snapshot": (i, [vp, C.c_char_p]),
         "pn_load_snapshot": (i, [vp, C.c_char_p]),
         "pn_batch_create": (i, [i, P(PnGrid), P(PnParams), i, P(vp)]),
@@ -134,6 +135,15 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     if path is None:
         _lib = lib
     return lib
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """GB/s of a plain device copy of nbytes (read + write), timed on the device."""
+    out = C.c_double()
+    rc = load_library().pn_copy_bandwidth(nbytes, reps, C.byref(out))
+    if rc != PN_OK:
+        raise PnError(rc, "pn_copy_bandwidth failed")
+    return out.value
 
 
 def params_default() -> PnParams:

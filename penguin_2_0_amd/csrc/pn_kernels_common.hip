@@ -503,6 +503,20 @@ hipError_t launch_dt_finalize(const LaunchCtx& c, int n) {
   return hipGetLastError();
 }
 
+// a plain copy of n 16-byte words, grid-stride: the HBM reference of the measurements
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a, float4* __restrict__ b, long n) {
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) b[i] = a[i];
+}
+
+hipError_t launch_copy(hipStream_t s, const void* a, void* b, size_t nbytes) {
+  const long n = (long)(nbytes / 16);
+  long nb = (n + 255) / 256;
+  if (nb > 16384) nb = 16384;
+  if (nb < 1) return hipErrorInvalidValue;
+  k_copy<<<(unsigned)nb, 256, 0, s>>>(static_cast<const float4*>(a), static_cast<float4*>(b), n);
+  return hipGetLastError();
+}
+
 hipError_t launch_sum_partials(const LaunchCtx& c, int nb, int stride, double* out) {
   k_sum_partials<<<1, kSumThreads, 0, c.stream>>>(c.partials, nb, stride, out);
   return hipGetLastError();

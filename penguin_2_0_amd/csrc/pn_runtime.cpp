@@ -828,6 +828,34 @@ int pn_time_kernels(pn_ctx* c, int nrep, double* ms6) {
   return rc;
 }
 
+int pn_copy_bandwidth(size_t nbytes, int reps, double* gbs) {
+  if (!gbs || reps < 1 || nbytes < (1u << 20) || nbytes % 16) return PN_ERR_ARG;
+  *gbs = 0.0;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&a, nbytes);
+  if (e == hipSuccess) e = hipMalloc(&b, nbytes);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipMemsetAsync(a, 0, nbytes, st);
+  if (e == hipSuccess) e = launch_copy(st, a, b, nbytes);
+  if (e == hipSuccess) e = hipEventRecord(e0, st);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy(st, a, b, nbytes);
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess && ms > 0.0f) *gbs = 2.0 * static_cast<double>(nbytes) * reps / (ms * 1e-3) / 1e9;
+  if (e1) hipEventDestroy(e1);
+  if (e0) hipEventDestroy(e0);
+  if (st) hipStreamDestroy(st);
+  if (b) hipFree(b);
+  if (a) hipFree(a);
+  return e == hipSuccess ? PN_OK : PN_ERR_HIP;
+}
+
 int pn_tune(pn_ctx* c, int nrep, int* var_x, int* var_y) {
   if (!c || nrep < 1) return PN_ERR_ARG;
   if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
