@@ -108,6 +108,14 @@ void pn_params_default(pn_params* p);
 /* sizeof(hydr_ring) for a given jmax: 72*jmax + 96 (variable_types.h:67-97). */
 size_t pn_ring_bytes(int32_t jmax);
 
+/* The snapshot reader's un-rotation (update_grid, read/p_init.cpp:108-132): from the
+ * imax*kmax hydr_ring records in `rings` (any ring order; each record names its i and k)
+ * fill lab-frame arrays indexed [i][j][k] -- element (i*jmax + j)*kmax + k -- with lab
+ * column j taken from data index (j - rot_j) mod jmax.  Any output may be NULL.  Host only.
+ * PN_ERR_ARG on a size mismatch or a ring whose (i, k) is out of range or repeated. */
+int pn_lab_fields(const void* rings, size_t bytes, int32_t imax, int32_t jmax, int32_t kmax,
+                  double* r, double* p, double* u, double* v, double* w);
+
 /* ---- host-side synthetic disk (init.cpp:3-543, disk_profile.cu:9-171) --
  * Writes imax*kmax interior rings (ring order i + imax*k, init.cpp:241),
  * PN_NPAD*kmax fixed ghost rings on each radial side (lft, rgh), and, in 3-D,

@@ -115,6 +115,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_run": (i, [vp, i, i, P(PnRunState)]),
         "pn_totals": (i, [vp, P(d), P(d)]),
         "pn_copy_bandwidth": (i, [sz, i, P(d)]),
+        "pn_lab_fields": (i, [vp, sz, i, i, i, vp, vp, vp, vp, vp]),
         "pn_save_snapshot": (i, [vp, C.c_char_p]),
         "pn_load_snapshot": (i, [vp, C.c_char_p]),
         "pn_batch_create": (i, [i, P(PnGrid), P(PnParams), i, P(vp)]),
@@ -193,6 +194,17 @@ def init_disk(grid: PnGrid, params: PnParams, seed: int, noise: float) -> Disk:
     if rc != PN_OK:
         raise PnError(rc, "pn_init_disk")
     return d
+
+
+def lab_fields(rings: np.ndarray, grid: PnGrid) -> dict:
+    """The reader's lab-frame fields (pn_lab_fields): r, p, u, v, w as [i][j][k] arrays."""
+    out = {f: np.empty((grid.imax, grid.jmax, grid.kmax)) for f in ("r", "p", "u", "v", "w")}
+    rings = np.ascontiguousarray(rings)
+    rc = load_library().pn_lab_fields(_ptr(rings), rings.nbytes, grid.imax, grid.jmax, grid.kmax,
+                                      *(_ptr(out[f]) for f in ("r", "p", "u", "v", "w")))
+    if rc != PN_OK:
+        raise PnError(rc, "pn_lab_fields")
+    return out
 
 
 class Context:

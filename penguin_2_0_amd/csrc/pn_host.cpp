@@ -195,6 +195,37 @@ void pn_params_default(pn_params* p) {
 
 size_t pn_ring_bytes(int32_t jmax) { return static_cast<size_t>(72) * static_cast<size_t>(jmax) + 96; }
 
+// update_grid (read/p_init.cpp:108-132): lab column j of ring (i, k) holds data index
+// j - rot_j (mod jmax); outputs are [i][j][k]
+int pn_lab_fields(const void* rings, size_t bytes, int32_t imax, int32_t jmax, int32_t kmax, double* r, double* p,
+                  double* u, double* v, double* w) {
+  if (!rings || imax < 1 || jmax < 1 || kmax < 1) return PN_ERR_ARG;
+  const size_t rb = pn_ring_bytes(jmax);
+  const long nring = static_cast<long>(imax) * kmax;
+  if (bytes != rb * static_cast<size_t>(nring)) return PN_ERR_ARG;
+  std::vector<char> seen(static_cast<size_t>(nring), 0);
+  for (long n = 0; n < nring; ++n) {
+    RingView R = ring_view(const_cast<void*>(rings), jmax, n);
+    const int i = *R.i, k = *R.k;
+    if (i < 0 || i >= imax || k < 0 || k >= kmax || seen[static_cast<size_t>(i) + static_cast<size_t>(imax) * k])
+      return PN_ERR_ARG;
+    seen[static_cast<size_t>(i) + static_cast<size_t>(imax) * k] = 1;
+    const long rot = *R.rot_j % jmax;
+    const double* src[5] = {R.r, R.p, R.u, R.v, R.w};
+    double* dst[5] = {r, p, u, v, w};
+    for (int f = 0; f < 5; ++f) {
+      if (!dst[f]) continue;
+      for (int j = 0; j < jmax; ++j) {
+        long j0 = j - rot;
+        if (j0 < 0) j0 += jmax;
+        if (j0 >= jmax) j0 -= jmax;
+        dst[f][(static_cast<size_t>(i) * jmax + j) * kmax + k] = src[f][j0];
+      }
+    }
+  }
+  return PN_OK;
+}
+
 // init_grid / init_den / init_speed / init_cells / init_bound (init.cpp:3-543)
 int pn_init_disk(const pn_grid* g, const pn_params* p, uint64_t seed, double noise, void* rings, void* lft,
                  void* rgh, void* udr, void* top, pn_symdisk* val) {

@@ -177,3 +177,33 @@ def test_fp32_and_batch_without_device_fail_loudly():
     with pytest.raises(abi.PnError) as e:
         abi.Batch([PnGrid(32, 48, 1)], [configs.disk2d(32, 48)])
     assert e.value.code == abi.PN_ERR_HIP
+
+
+def test_lab_fields_unrotates_like_the_reader():
+    """pn_lab_fields against a brute-force roll: lab column j of ring (i, k) is data index
+    (j - rot_j) mod jmax, output [i][j][k]; ring order does not matter."""
+    for grid, p in ((PnGrid(6, 20, 1), configs.disk2d(6, 20)),
+                    (PnGrid(5, 16, 3), configs.disk2d(5, 16).replace(ndim=3, ngeomx=2, ngeomy=4, ngeomz=5,
+                                                                      zmin=1.4, zmax=1.7))):
+        d = abi.init_disk(grid, p, 3, 1e-3)
+        rng = np.random.default_rng(1)
+        d.rings["rot_j"] = rng.integers(0, grid.jmax, size=len(d.rings))
+        for f in ("u", "v", "w"):
+            d.rings[f] = rng.standard_normal(d.rings[f].shape)
+        out = abi.lab_fields(d.rings, grid)
+        for f in ("r", "p", "u", "v", "w"):
+            ref = np.empty((grid.imax, grid.jmax, grid.kmax))
+            for ring in d.rings:
+                for j in range(grid.jmax):
+                    ref[ring["i"], j, ring["k"]] = ring[f][(j - ring["rot_j"]) % grid.jmax]
+            assert np.array_equal(out[f], ref), f
+        perm = rng.permutation(len(d.rings))
+        out2 = abi.lab_fields(d.rings[perm], grid)
+        assert all(np.array_equal(out[f], out2[f]) for f in out)
+        # a repeated (i, k) and a short buffer are errors
+        bad = d.rings.copy()
+        bad[1]["i"], bad[1]["k"] = bad[0]["i"], bad[0]["k"]
+        with pytest.raises(abi.PnError):
+            abi.lab_fields(bad, grid)
+        with pytest.raises(abi.PnError):
+            abi.lab_fields(d.rings[:-1], grid)
