@@ -52,6 +52,9 @@ struct DevF32 {
   float* f[2][4];
   float* tab;                  // kTabEntries x tstride, entry-major
   int tstride;                 // nslot + kTabOverhang, a multiple of 4
+  // 3-D: the same table theta-major (slot (ir, kz) at kz + sk ir, so a theta pencil's slots
+  // are consecutive); stride tstride.  kTabVrot is not kept current here.
+  float* tabz;
   const double *rbg, *vbg;     // Sigma_b and v_bg per slot, FP64
   const float *dy, *yvol;      // per column
   float2* trig;                // per column, per step: sin / cos of half the angle to the planet

@@ -137,10 +137,10 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
   const PlanetF pl = planet_f(ss, G != kGeoRcyl);
   const float sqrt_gam = sqrtf((float)prm.gam);
   // radial: overhang cells read copies of the last slot; theta: reads clamped to the window
-  const auto tx = [&] {
-    if constexpr (TH) return TabLaneS<K>{f.tab, f.tstride, s0, g.sr, K * lane, lim - 1};
-    else return TabLane<K>{f.tab, f.tstride, s0 + K * lane};
-  }();
+  // theta: the theta-major table, whose slots run along the pencil (reads past the window
+  // land on the next ring's slots or the overhang: finite values no kept cell depends on)
+  const TabLane<K> tx = TH ? TabLane<K>{f.tabz, f.tstride, (REAL * l - kNpad + g.k0) + g.sk * (row + kNpad) + K * lane}
+                           : TabLane<K>{f.tab, f.tstride, s0 + K * lane};
   for (int qq = 0; qq < Q; ++qq) {
     const int q = wv * Q + qq;
     const int j0 = jb + q;
@@ -167,7 +167,12 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
     Wp.cs2 = tx.get(kTabCs2);
     Wp.cs = tx.get(kTabCs);
     Wp.Lb = tx.get(kTabLb);
-    Wp.vrot = tx.get(kTabVrot);
+    if constexpr (TH) {
+      // vrot changes every step and is kept current in the slot-major table only
+      Wp.vrot = TabLaneS<K>{f.tab, f.tstride, s0, g.sr, K * lane, lim - 1}.get(kTabVrot);
+    } else {
+      Wp.vrot = tx.get(kTabVrot);
+    }
     if constexpr (G != kGeoRcyl) {
       Wp.sth = tx.get(kTabSth);
       Wp.oms = tx.get(kTabOms);
@@ -633,7 +638,9 @@ hipError_t sweep_pencil3_k(const LaunchCtx& c, const DevF32& f0, int src, int do
   const bool th = (G == kGeoTheta);
   if (th) f.nwin_z = f32_windows(c.g.kmax, K);
   else f.nwin_x = f32_windows(c.g.imax, K);
-  if (c.g.ndim != 3 || f.f[0][3] == nullptr) return hipErrorInvalidValue;
+  if (c.g.ndim != 3 || f.f[0][3] == nullptr || f.tabz == nullptr) return hipErrorInvalidValue;
+  // theta windows read the theta-major table up to 64 K - 1 slots past their first slot
+  if (th && f.tstride < c.g.sr * c.g.sk + 64 * K) return hipErrorInvalidValue;
   // radial windows read table entries up to 64 K - 1 slots past their first slot
   if (!th && f.tstride < c.g.sr * c.g.sk + 64 * K) return hipErrorInvalidValue;
   const long ngrp = (c.g.jmax + W * Q - 1) / (W * Q);

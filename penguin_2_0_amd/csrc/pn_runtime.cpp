@@ -260,6 +260,7 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
     double *rbg, *vbg;
     c->df.tstride = static_cast<int>((nslot + kTabOverhang + 3) / 4 * 4);
     if (r2 == PN_OK) r2 = dalloc(c, &c->df.tab, static_cast<size_t>(kTabEntries) * c->df.tstride);
+    if (r2 == PN_OK && p->ndim == 3) r2 = dalloc(c, &c->df.tabz, static_cast<size_t>(kTabEntries) * c->df.tstride);
     if (r2 == PN_OK) r2 = dalloc(c, &rbg, nslot);
     if (r2 == PN_OK) r2 = dalloc(c, &vbg, nslot);
     if (r2 == PN_OK) r2 = dalloc(c, &fdy, g->jmax);
@@ -582,6 +583,16 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
       t_yvol[j] = static_cast<float>(c->h_yvol[j]);
     }
     PN_HIP(c, hipMemcpyAsync(c->df.tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    std::vector<float> tabz;
+    if (d3) {
+      // theta-major copy: slot s = ir + sr kz goes to kz + sk ir; the overhang repeats the last
+      tabz.assign(tab.size(), 0.0f);
+      for (int e = 0; e < kTabEntries; ++e) {
+        for (long s = 0; s < nslot; ++s) tabz[e * ts + (s / sr) + d.sk * (s % sr)] = tab[e * ts + s];
+        for (long o = nslot; o < ts; ++o) tabz[e * ts + o] = tab[e * ts + nslot - 1];
+      }
+      PN_HIP(c, hipMemcpyAsync(c->df.tabz, tabz.data(), tabz.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    }
     PN_HIP(c, hipMemcpyAsync(const_cast<float*>(c->df.dy), t_dy.data(), J * sizeof(float), hipMemcpyHostToDevice, st));
     PN_HIP(c, hipMemcpyAsync(const_cast<float*>(c->df.yvol), t_yvol.data(), J * sizeof(float), hipMemcpyHostToDevice, st));
     PN_HIP(c, hipMemcpyAsync(const_cast<double*>(c->df.rbg), c->h_rbg.data(), nslot * sizeof(double), hipMemcpyHostToDevice, st));
