@@ -20,8 +20,10 @@ bytes per launch (one read and one write of the minimal state fields of every ce
 the 2-D isothermal disk, one more in 3-D, one more for a non-isothermal EOS) over its
 mean launch time, measured with HIP events on the context's stream -- against HBM
 bandwidth (and against the bandwidth a plain 1 GiB device copy reaches on the same GPU,
-copy_measured_gbs).  The CPU baseline times the FP64 oracle on the host cores on a bounded prefix of
-the same workload.
+copy_measured_gbs); traffic and the VALU instruction count per launch come from the
+committed rocprofv3 counter runs (profiles/counters.json), the latter priced against the
+VALU issue rate, which is what bounds these sweeps.  The CPU baseline times the FP64 oracle
+on the host cores on a bounded prefix of the same workload.
 """
 from __future__ import annotations
 
@@ -41,7 +43,10 @@ from penguin_2_0_amd import abi, configs  # noqa: E402
 
 METRIC = "zone-updates/s per GPU: FP32-batched vs FP64 plain at L1(ρ)≤1e-5; % HBM peak"
 HBM_PEAK_GBS = 8000.0
-PROFILE_TRAFFIC = ROOT / "profiles" / "traffic.json"
+PROFILE_COUNTERS = ROOT / "profiles" / "counters.json"
+# one wave64 FP32 VALU instruction per CU per clock (4 SIMDs x 16 lanes; packed FP32 not used):
+# 256 CUs x 2.4 GHz
+VALU_PEAK_INSTS = 256 * 2.4e9
 
 
 def cpu_baseline(w: configs.Workload, budget_s: float = 12.0) -> dict:
@@ -63,25 +68,26 @@ def cpu_baseline(w: configs.Workload, budget_s: float = 12.0) -> dict:
                       f"{n} steps in {dt:.2f} s"}
 
 
-def timed_run(w: configs.Workload, prec: int, steps: int, warmup: int) -> dict:
+def timed_run(w: configs.Workload, prec: int, steps: int, warmup: int, graph: bool = True) -> dict:
     disk = abi.init_disk(w.grid, w.params, w.seed, w.noise)
     ctx = abi.Context(w.grid, w.params, prec)
     ctx.load(disk)
     st = ctx.run_begin()
     m0, l0 = ctx.totals()
-    ctx.run(warmup, st, graph=True)
+    ctx.run(warmup, st, graph=graph)
     t0 = time.perf_counter()
-    ctx.run(steps, st, graph=True)   # returns after the stream has drained
+    ctx.run(steps, st, graph=graph)   # returns after the stream has drained
     wall = time.perf_counter() - t0
     m1, l1 = ctx.totals()
     state = ctx.store()
+    variants = ctx.variants()
     km = ctx.time_kernels(5)         # after the snapshot: timing advances the state
     ctx.close()
     return {"wall": wall, "value": w.cells * steps / wall, "state": state, "st": st, "km": km,
-            "dmass": (m1 - m0) / m0, "dangmom": (l1 - l0) / l0}
+            "variants": variants, "dmass": (m1 - m0) / m0, "dangmom": (l1 - l0) / l0}
 
 
-def batched_run(steps: int, warmup: int) -> dict:
+def batched_run(steps: int, warmup: int, graph: bool = True) -> dict:
     """The parameter sweep of 48 uneven viscous disks: the FP32 batch (one launch per sweep
     for all disks) against every disk's plain FP64 context run on its own, same steps."""
     ws = configs.c4_members()
@@ -91,9 +97,9 @@ def batched_run(steps: int, warmup: int) -> dict:
         for ctx, d in zip(b.disks, disks):
             ctx.load(d)
         st = b.run_begin()
-        b.run(warmup, st)
+        b.run(warmup, st, graph=graph)
         t0 = time.perf_counter()
-        b.run(steps, st)
+        b.run(steps, st, graph=graph)
         wall = time.perf_counter() - t0
         out32 = [ctx.store() for ctx in b.disks]
         km = b.time_kernels(5)
@@ -168,6 +174,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched parameter-sweep leg")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--copy-probe", action="store_true", help="only run a plain device copy of known size")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every step's kernels directly instead of replaying a captured HIP graph "
+                         "(the same kernels; for tools that do not follow graph launches)")
     a = ap.parse_args(argv)
     if a.copy_probe:
         return copy_probe()
@@ -179,7 +188,7 @@ def main(argv=None) -> int:
 
     w = getattr(configs, a.config)()
     prec = abi.PN_FP64 if a.precision == "fp64" else abi.PN_FP32
-    run = timed_run(w, prec, a.steps, a.warmup)
+    run = timed_run(w, prec, a.steps, a.warmup, graph=not a.eager)
     value = run["value"]
     km = run["km"]
 
@@ -190,29 +199,36 @@ def main(argv=None) -> int:
     fields = 3 + (1 if w.params.ndim == 3 else 0) + (1 if w.params.eos != 0 else 0)
     bytes_per_launch = 2 * fields * wbytes * w.cells
     achieved = bytes_per_launch / (km[dom] * 1e-3) / 1e9
-    traffic = None
-    if PROFILE_TRAFFIC.exists():
-        try:
-            traffic = json.loads(PROFILE_TRAFFIC.read_text()).get(f"{w.name}:{a.precision}:{dom}")
-        except Exception:
-            traffic = None
+    # HBM bytes and VALU instructions per launch of the dominant kernel (its variant) from the
+    # committed rocprofv3 counter runs
+    var = run["variants"][1] if dom == "sweep_y" else run["variants"][0]
+    prof = {}
+    if PROFILE_COUNTERS.exists():
+        prof = json.loads(PROFILE_COUNTERS.read_text()).get(f"{w.name}:{a.precision}:{dom}:v{var}", {})
+    traffic = prof.get("traffic")
     copy_gbs = abi.copy_bandwidth()
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "kernel": dom, "kernel_ms": km[dom], "bytes_per_launch": bytes_per_launch, "kernels_ms": km,
+                "kernel": dom, "variant": var, "kernel_ms": km[dom], "bytes_per_launch": bytes_per_launch,
+                "traffic_over_algorithmic": traffic / bytes_per_launch if traffic else None, "kernels_ms": km,
                 "copy_measured_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs}
+    if "sq_insts_valu" in prof:
+        # what actually bounds the sweeps: VALU issue, against one instruction per CU per clock
+        ips = prof["sq_insts_valu"] / (km[dom] * 1e-3)
+        roofline["valu"] = {"insts_per_launch": prof["sq_insts_valu"], "achieved_insts_per_s": ips,
+                            "peak_insts_per_s": VALU_PEAK_INSTS, "frac": ips / VALU_PEAK_INSTS}
     out = {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": run["wall"] / a.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == abi.PN_FP64 else "f32",
         "data": "synthetic (seeded disk initial condition of the workload)",
         "config": {"workload": w.name, "cells": w.cells,
-                   "path": "plain FP64" if prec == abi.PN_FP64 else "FP32 bulk (single disk)", "graph": True},
+                   "path": "plain FP64" if prec == abi.PN_FP64 else "FP32 bulk (single disk)", "graph": not a.eager},
         "roofline": roofline,
         "conservation": {"mass_drift": run["dmass"], "angmom_drift": run["dangmom"]},
     }
     if prec == abi.PN_FP32 and not a.no_ref:
-        ref = timed_run(w, abi.PN_FP64, a.steps, a.warmup)
+        ref = timed_run(w, abi.PN_FP64, a.steps, a.warmup, graph=not a.eager)
         errs = {f: rel_err(lab(run["state"], f), lab(ref["state"], f)) for f in ("r", "u", "v")}
         out["fp64_plain"] = {"value": ref["value"], "ms_per_step": ref["wall"] / a.steps * 1e3,
                              "speedup": value / ref["value"], "kernels_ms": ref["km"],
@@ -226,7 +242,7 @@ def main(argv=None) -> int:
             "dt_rel_diff": abs(run["st"].dt - ref["st"].dt) / ref["st"].dt,
         }
     if prec == abi.PN_FP32 and not a.no_batch:
-        out["batched"] = batched_run(a.batch_steps, a.warmup)
+        out["batched"] = batched_run(a.batch_steps, a.warmup, graph=not a.eager)
     if rank == 0 and not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
     if rank == 0:

@@ -213,6 +213,8 @@ int pn_tune(pn_ctx* ctx, int nrep, int* var_x, int* var_y);
  * (2,4,4).  var_y 0..3, azimuthal sweep as (cells per lane, waves per workgroup): (2,2),
  * (2,4), (4,2), (4,4). */
 int pn_set_variants(pn_ctx* ctx, int var_x, int var_y);
+/* The variants in use (after tuning or pinning; 0 for FP64 contexts). */
+int pn_get_variants(const pn_ctx* ctx, int* var_x, int* var_y);
 
 /* The achievable HBM bandwidth: a plain device copy of nbytes (a multiple of 16, at least
  * 1 MiB) launched reps times back to back after one warm-up, timed with HIP events on a

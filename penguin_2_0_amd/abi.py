@@ -108,6 +108,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_time_kernels": (i, [vp, i, P(d)]),
         "pn_tune": (i, [vp, i, P(i), P(i)]),
         "pn_set_variants": (i, [vp, i, i]),
+        "pn_get_variants": (i, [vp, P(i), P(i)]),
         "pn_init_planet": (None, [P(PnParams), P(PnBody), P(d)]),
         "pn_planet_forces": (i, [vp, P(PnBody), d, d]),
         "pn_kick_drift_kick": (None, [P(PnParams), P(PnBody), P(d), d, i]),
@@ -280,6 +281,11 @@ class Context:
 
     def set_variants(self, var_x: int, var_y: int):
         self._chk(self.lib.pn_set_variants(self.h, var_x, var_y), "pn_set_variants")
+
+    def variants(self) -> tuple[int, int]:
+        vx, vy = C.c_int(), C.c_int()
+        self._chk(self.lib.pn_get_variants(self.h, C.byref(vx), C.byref(vy)), "pn_get_variants")
+        return vx.value, vy.value
 
     def reduce(self, planet: PnBody, exclude: float, mode: int) -> float:
         d = C.c_double()

@@ -910,6 +910,13 @@ int pn_tune(pn_ctx* c, int nrep, int* var_x, int* var_y) {
   return PN_OK;
 }
 
+int pn_get_variants(const pn_ctx* c, int* var_x, int* var_y) {
+  if (!c || !var_x || !var_y) return PN_ERR_ARG;
+  *var_x = is_f32(c) ? c->var_x : 0;
+  *var_y = is_f32(c) ? c->var_y : 0;
+  return PN_OK;
+}
+
 int pn_set_variants(pn_ctx* c, int var_x, int var_y) {
   if (!c) return PN_ERR_ARG;
   if (!is_f32(c)) return (var_x == 0 && var_y == 0) ? PN_OK : fail(c, PN_ERR_ARG, "the FP64 path has one variant");
