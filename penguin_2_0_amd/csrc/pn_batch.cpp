@@ -149,6 +149,12 @@ int btime(pn_batch* b, int nrep, double* ms, F&& launch) {
   return PN_OK;
 }
 
+bool any_viscous(const pn_batch* b) {
+  for (const pn_ctx* c : b->disks)
+    if (c->P.visc_flag) return true;
+  return false;
+}
+
 }  // namespace
 
 extern "C" {
@@ -170,6 +176,7 @@ int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int pr
     if (c) b->disks.push_back(c);
     if (rc != PN_OK) return bfail(b, rc, "disk %d: %s", d, c ? c->err.c_str() : "pn_create failed");
   }
+  if (any_viscous(b)) b->var_y = 1;   // (2, 4): the reference's azimuthal windows
   PB_HIP(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   PB_HIP(b, hipMalloc(reinterpret_cast<void**>(&b->dd), n * sizeof(DiskDev)));
   PB_HIP(b, hipMalloc(reinterpret_cast<void**>(&b->dfs), n * sizeof(DevF32)));
@@ -214,6 +221,8 @@ int pn_batch_set_variants(pn_batch* b, int var_x, int var_y) {
   if (!b) return PN_ERR_ARG;
   if (var_x < 0 || var_x >= kNumF32Variants || var_y < 0 || var_y >= kNumF32Variants)
     return bfail(b, PN_ERR_ARG, "variants (%d, %d) outside [0, %d)", var_x, var_y, kNumF32Variants);
+  if (!f32_y_variant_allowed(var_y, any_viscous(b)))
+    return bfail(b, PN_ERR_ARG, "azimuthal variant %d: viscous disks keep 128-cell azimuthal windows", var_y);
   if (var_x != b->var_x || var_y != b->var_y) drop_graph(b);
   b->var_x = var_x;
   b->var_y = var_y;
@@ -239,13 +248,14 @@ int pn_batch_tune(pn_batch* b, int nrep, int* var_x, int* var_y) {
     rc = btime(b, nrep, &tx, [&]() {
       return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x);
     });
-    if (rc == PN_OK)
+    const bool y_ok = f32_y_variant_allowed(v, any_viscous(b));
+    if (rc == PN_OK && y_ok)
       rc = btime(b, nrep, &ty, [&]() {
         return f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur, b->grid_y);
       });
     if (rc != PN_OK) return rc;
     if (tx < best_x) { best_x = tx; bx = v; }
-    if (ty < best_y) { best_y = ty; by = v; }
+    if (y_ok && ty < best_y) { best_y = ty; by = v; }
   }
   b->var_x = bx;
   b->var_y = by;

@@ -70,6 +70,15 @@ extern const int kF32VariantsX[kNumF32Variants][3];
 // azimuthal sweep: (cells per lane K, waves per workgroup W)
 extern const int kF32VariantsY[kNumF32Variants][2];
 
+// The reference's azimuthal viscosity (viscosity.cu:423-472) runs after the remap on the
+// window's cells and reads the neighbours c +- 1, so at the edges of the kept range it sees
+// halo cells the remap left Lagrangian: window boundaries are part of its discretization.
+// Viscous disks therefore keep the reference's 128-cell windows (K = 2) in the azimuthal
+// sweep; inviscid sweeps give the same result for any window length.
+inline bool f32_y_variant_allowed(int variant, bool viscous) {
+  return !viscous || kF32VariantsY[variant][0] == 2;
+}
+
 int f32_windows(int len, int K);
 // per-step tables: planet trig per column, vrot per slot (after Index_Shift)
 hipError_t f32_prep(const LaunchCtx& c, const DevF32& f);

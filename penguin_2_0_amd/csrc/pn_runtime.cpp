@@ -206,6 +206,7 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
   c->g = *g;
   c->P = *p;
   c->precision = precision;
+  if (p->visc_flag) c->var_y = 1;   // (2, 4): the reference's azimuthal windows
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     *out = c;
@@ -890,10 +891,12 @@ int pn_tune(pn_ctx* c, int nrep, int* var_x, int* var_y) {
       rc = event_time(c, c->stream, nrep, &tz, [&]() { return f32_sweep_z(L, c->df, v, c->cur, 0); });
       tx += tz;
     }
-    if (rc == PN_OK) rc = event_time(c, c->stream, nrep, &ty, [&]() { return f32_sweep_y(L, c->df, v, c->cur, 0); });
+    const bool y_ok = f32_y_variant_allowed(v, c->P.visc_flag != 0);
+    if (rc == PN_OK && y_ok)
+      rc = event_time(c, c->stream, nrep, &ty, [&]() { return f32_sweep_y(L, c->df, v, c->cur, 0); });
     if (rc != PN_OK) return rc;
     if (tx < best_x) { best_x = tx; bx = v; }
-    if (ty < best_y) { best_y = ty; by = v; }
+    if (y_ok && ty < best_y) { best_y = ty; by = v; }
   }
   c->var_x = bx;
   c->var_y = by;
@@ -922,6 +925,8 @@ int pn_set_variants(pn_ctx* c, int var_x, int var_y) {
   if (!is_f32(c)) return (var_x == 0 && var_y == 0) ? PN_OK : fail(c, PN_ERR_ARG, "the FP64 path has one variant");
   if (var_x < 0 || var_x >= kNumF32Variants || var_y < 0 || var_y >= kNumF32Variants)
     return fail(c, PN_ERR_ARG, "variants (%d, %d) outside [0, %d)", var_x, var_y, kNumF32Variants);
+  if (!f32_y_variant_allowed(var_y, c->P.visc_flag != 0))
+    return fail(c, PN_ERR_ARG, "azimuthal variant %d: a viscous disk keeps 128-cell azimuthal windows", var_y);
   c->var_x = var_x;
   c->var_y = var_y;
   c->df.nwin_x = f32_windows(c->g.imax, kF32VariantsX[var_x][0]);

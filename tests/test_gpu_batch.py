@@ -41,7 +41,7 @@ def members():
     return out
 
 
-def run_batch(ws, n, variants=(0, 3), graph=True):
+def run_batch(ws, n, variants=(0, 1), graph=True):
     disks = [abi.init_disk(w.grid, w.params, w.seed, w.noise) for w in ws]
     with abi.Batch([w.grid for w in ws], [w.params for w in ws]) as b:
         for ctx, d in zip(b.disks, disks):
@@ -52,7 +52,7 @@ def run_batch(ws, n, variants=(0, 3), graph=True):
         return [ctx.store() for ctx in b.disks], [st[d] for d in range(len(ws))]
 
 
-def run_single(w, prec, n, variants=(0, 3), disk=None):
+def run_single(w, prec, n, variants=(0, 1), disk=None):
     d = disk if disk is not None else abi.init_disk(w.grid, w.params, w.seed, w.noise)
     with abi.Context(w.grid, w.params, prec) as ctx:
         ctx.load(d)
@@ -79,17 +79,16 @@ def test_batch_equals_single_disks():
 
 
 def test_batch_vs_fp64():
-    """Each batched disk tracks its own FP64 run: at 30 steps to the FP32 bar; at 100 steps,
-    when the strongly forced disks (thermal planet masses up to 8) have become sensitive to
-    rounding, within a factor of the flow's own sensitivity -- the distance an FP64 run from
-    the FP32-rounded initial state has drifted by then."""
+    """Each batched disk tracks its own FP64 run at 30 and 100 steps (the strongly forced disks
+    have thermal planet masses up to 8); the flow's own sensitivity -- the drift of an FP64 run
+    from the FP32-rounded initial state -- is printed beside it."""
     ws = members()
     disks = [abi.init_disk(w.grid, w.params, w.seed, w.noise) for w in ws]
     say("case: batch vs fp64, 30 and 100 steps")
     with abi.Batch([w.grid for w in ws], [w.params for w in ws]) as b:
         for ctx, d in zip(b.disks, disks):
             ctx.load(d)
-        b.set_variants(0, 3)
+        b.set_variants(0, 1)
         st = b.run_begin()
         b.run(30, st)
         o30 = [(ctx.store(), st[k].dt) for k, ctx in enumerate(b.disks)]
@@ -111,8 +110,7 @@ def test_batch_vs_fp64():
         l1, _ = rel(lab(o100[k], "r"), lab(ref, "r"))
         floor, _ = rel(lab(flo, "r"), lab(ref, "r"))
         say(f"  {w.name} 100 steps: rho L1 {l1:.3e}, sensitivity floor {floor:.3e}")
-        # FP32 rounds every operation of every step, the floor perturbs the start once
-        assert l1 <= max(1e-6, 30.0 * floor)
+        assert l1 <= 1e-6
 
 
 def test_batch_graph_equals_eager_bitwise():
@@ -125,7 +123,7 @@ def test_batch_graph_equals_eager_bitwise():
         assert s.dt == t.dt and s.simtime == t.simtime
 
 
-@pytest.mark.parametrize("variants", [(1, 0), (2, 1), (3, 2)])
+@pytest.mark.parametrize("variants", [(1, 0), (2, 1), (3, 0)])
 def test_batch_variants(variants):
     ws = members()[:3]
     say("case: batch variants", variants)

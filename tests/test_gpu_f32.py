@@ -101,6 +101,31 @@ def test_f32_viscous_vs_fp64(w):
         assert l1 <= t1 and li <= tinf, (f, l1, li)
 
 
+def test_f32_viscous_keeps_reference_azimuthal_windows():
+    """The azimuthal viscosity reads halo cells at the window edges, so a viscous disk must run
+    the reference's 128-cell azimuthal windows: the tuner picks among those only and a longer
+    window is refused.  A strongly forced viscous disk then tracks FP64 closely (with 256-cell
+    windows it drifted to L1 4e-6 by 100 steps)."""
+    w = configs.small(64, 192, M_p=3e-4, visc_flag=1, ss_alpha=1e-2, vis_nu=1e-2 * 0.05 * 0.05)
+    w.seed = 101
+    say("case: fp32 viscous windows", w.name)
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    with abi.Context(w.grid, w.params, abi.PN_FP32) as ctx:
+        ctx.load(d)
+        for vy in (2, 3):
+            with pytest.raises(abi.PnError) as e:
+                ctx.set_variants(0, vy)
+            assert e.value.code == abi.PN_ERR_ARG
+        st = ctx.run_begin()
+        ctx.run(100, st)
+        assert ctx.variants()[1] in (0, 1)
+        a = ctx.store()
+    b, _ = run(w, abi.PN_FP64, 100, disk=d)
+    l1, li = rel(lab(a, "r"), lab(b, "r"))
+    say(f"  rho L1 {l1:.3e} Linf {li:.3e}")
+    assert l1 <= 1e-6 and li <= 1e-4
+
+
 def test_f32_every_variant():
     """Every (cells per lane, waves, columns) variant of both sweeps is a valid step."""
     w = configs.small(130, 250)
