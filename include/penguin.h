@@ -87,6 +87,17 @@ typedef struct pn_params {
   double endtime, tmovie;
   /* grid bounds, global.h:76-81 */
   double xmin, xmax, ymin, ymax, zmin, zmax;
+  /* radial boundaries, global.h:59-60, with the meanings set_boundx gives the codes
+   * (boundary.cu:86-206; the comment at global.h:59 lists them differently):
+   * 0 reflect (ghost t cells outside the edge cell take the cell t inside it, v_r negated),
+   * 1 outflow (ghosts copy the edge cell), 2 periodic in radius, 3 fixed ghost rings (the
+   * reference's only live choice, the default), 4 outflow of r, p, u (left) or r, p (right)
+   * with the fixed ghosts' other fields.  Codes other than 3 are FP64-path only. */
+  int32_t nlft, nrgh;
+  /* 1: damping zones of width sc_h/2 at xmin and xmax relaxing every field towards the
+   * background at rate (distance into the zone / width) r^-1.5 per unit time
+   * (kill_wave.cu:3-43; kill_flag, global.h:22).  FP64 path only. */
+  int32_t kill_flag;
 } pn_params;
 
 /* State a step loop carries between steps (main.cu:270-275, 325-327). */
@@ -222,6 +233,11 @@ int pn_get_variants(const pn_ctx* ctx, int* var_x, int* var_y);
  * frees its two buffers.  No counterpart in the reference; the measurements quote kernel
  * bandwidth against it. */
 int pn_copy_bandwidth(size_t nbytes, int reps, double* gbs);
+
+/* One application of the damping zones (kill_wave, kill_wave.cu:3-43) with time step dt to
+ * the current state, whatever kill_flag says; pn_run applies it after every step's sweeps
+ * when kill_flag is 1.  FP64 contexts only. */
+int pn_damp_edges(pn_ctx* ctx, double dt);
 
 /* Mass and angular-momentum totals in FP64 with compensated summation. */
 int pn_totals(pn_ctx* ctx, double* mass, double* angmom);

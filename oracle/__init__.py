@@ -35,6 +35,7 @@ def lib() -> C.CDLL:
             "orc_sweepy": (None, [G, Pp, vp, d, B, d]),
             "orc_sweepz": (None, [G, Pp, vp, vp, vp, d, B, d]),
             "orc_rotate": (None, [G, vp]),
+            "orc_damp": (None, [G, Pp, vp, P(PnSymdisk), d]),
             "orc_reduce": (d, [G, Pp, vp, B, d, i]),
             "orc_totals": (None, [G, Pp, vp, dp, dp]),
             "orc_init_planet": (None, [Pp, B, dp]),
@@ -118,6 +119,9 @@ class OracleDisk:
 
     def rotate(self):
         lib().orc_rotate(C.byref(self.grid), vptr(self.rings))
+
+    def damp(self, dt: float):
+        lib().orc_damp(C.byref(self.grid), C.byref(self.params), vptr(self.rings), self.val, dt)
 
     def reduce(self, planet: PnBody, exclude: float, mode: int) -> float:
         return lib().orc_reduce(C.byref(self.grid), C.byref(self.params), vptr(self.rings), C.byref(planet),

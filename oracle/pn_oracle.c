@@ -1225,6 +1225,34 @@ static double eos_store_p(const pn_params* P, double r, double p, double rad_cyl
 }
 
 /* sweepx, ppm.cu:26-160: one radial pencil per lab column j0 and theta ring k */
+/* set_boundx (boundary.cu:86-206): the values a radial ghost cell of theta row k at lab
+   column j0 takes under the boundary codes; v[0..4] = r, p, u, v, w arrive holding the fixed
+   ghost ring's values (code 3).  Ghost ring i_ring < 0 lies t = -i_ring cells outside the
+   left edge cell, i_ring >= imax lies t = i_ring - imax + 1 cells outside the right one. */
+static void ghost_values(const pn_grid* g, const pn_params* P, const void* rings, int k, int j0, int i_ring,
+                         double* v) {
+  const int imax = g->imax, jmax = g->jmax;
+  const int left = i_ring < 0;
+  const int code = left ? P->nlft : P->nrgh;
+  const int t = left ? -i_ring : i_ring - imax + 1;
+  int src, first = 0, sign_u = 1;
+  switch (code) {
+    case 0: src = left ? t : imax - 1 - t; sign_u = -1; break;  /* reflect about the edge cell */
+    case 1: src = left ? 0 : imax - 1; break;                    /* outflow */
+    case 2: src = left ? imax - t : t - 1; break;                /* periodic */
+    case 4: src = left ? 0 : imax - 1; break;                    /* outflow of r, p (, u) */
+    default: return;                                             /* 3: fixed ghost rings */
+  }
+  const int last = (code == 4) ? (left ? 3 : 2) : 5;   /* fields [first, last) come from src */
+  ring_t S = ring_at(rings, jmax, src + (long)imax * k);
+  int j = j0 - *S.rot_j;
+  if (j < 0) j += jmax;
+  if (j >= jmax) j -= jmax;
+  const double sv[5] = {S.r[j], S.p[j], S.u[j], S.v[j], S.w[j]};
+  for (int f = first; f < last; f++) v[f] = sv[f];
+  if (code == 0) v[2] = sign_u * sv[2];
+}
+
 void orc_sweepx(const pn_grid* g, const pn_params* P, void* rings, const void* lft, const void* rgh,
                 double dt, const pn_body* planet, double FrRot) {
   const int imax = g->imax, jmax = g->jmax, kmax = g->kmax;
@@ -1257,6 +1285,7 @@ void orc_sweepx(const pn_grid* g, const pn_params* P, void* rings, const void* l
           pr[5 * c + 2] = R.u[j];
           pr[5 * c + 3] = R.v[j];
           pr[5 * c + 4] = R.w[j];
+          if (i_ring < 0 || i_ring >= imax) ghost_values(g, P, rings, k, j0, i_ring, pr + 5 * c);
         }
         for (int l = 0; l < L; l++) {
           int lim = win_lim(imax, l);
@@ -1672,6 +1701,33 @@ void orc_run_begin(const pn_grid* g, const pn_params* P, void* rings, const pn_s
   orc_index_shift(g, P, rings, val, st->dt, st->FrRot);
 }
 
+/* kill_wave (kill_wave.cu:3-43): cells within sc_h/2 of xmin or xmax relax towards the
+   ring's background by f = (depth into the zone / width) dt r^-1.5; w relaxes to 0 */
+void orc_damp(const pn_grid* g, const pn_params* P, void* rings, const pn_symdisk* val, double dt) {
+  const int imax = g->imax, jmax = g->jmax, kmax = g->kmax;
+  const double d_kill = 0.5 * P->sc_h;
+  const double inner = P->xmin + d_kill, outer = P->xmax - d_kill;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < kmax; k++)
+    for (int i = 0; i < imax; i++) {
+      ring_t R = ring_at(rings, jmax, i + (long)imax * k);
+      const double r = *R.xc;
+      double f;
+      if (r < inner) f = (inner - r) / d_kill;
+      else if (r > outer) f = (r - outer) / d_kill;
+      else continue;
+      f *= dt * pow(r, -1.5);
+      const pn_symdisk V = val[i + (long)imax * k];
+      for (int j = 0; j < jmax; j++) {
+        R.r[j] += f * (V.r - R.r[j]);
+        R.p[j] += f * (V.p - R.p[j]);
+        R.u[j] += f * (V.u - R.u[j]);
+        R.v[j] += f * (V.v - R.v[j]);
+        R.w[j] += f * (-R.w[j]);
+      }
+    }
+}
+
 void orc_run(const pn_grid* g, const pn_params* P, void* rings, const void* lft, const void* rgh,
              const void* udr, const void* top, const pn_symdisk* val, int nsteps, pn_run_state* st) {
   for (int s = 0; s < nsteps; s++) {
@@ -1683,6 +1739,7 @@ void orc_run(const pn_grid* g, const pn_params* P, void* rings, const void* lft,
     orc_sweepy(g, P, rings, dt, &st->planet, st->FrRot);
     if (P->ndim == 3) orc_sweepz(g, P, rings, udr, top, dt, &st->planet, st->FrRot);
     if (P->fargo_flag > 0) orc_rotate(g, rings);
+    if (P->kill_flag) orc_damp(g, P, rings, val, dt);
     st->simtime += dt;
     st->ncycle++;
     if (P->plnt_flag > 0) {

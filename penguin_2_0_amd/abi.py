@@ -39,7 +39,8 @@ class PnParams(C.Structure):
         "ngeomx", "ngeomy", "ngeomz", "grspx", "grspy", "grspz", "init_p_iso")] + \
         [(n, C.c_double) for n in (
             "p_alpha", "p_beta", "ss_alpha", "sc_h", "vis_nu", "M_p", "R_p", "t_growth", "rs_fac",
-            "gam", "courant", "endtime", "tmovie", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax")]
+            "gam", "courant", "endtime", "tmovie", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax")] + \
+        [(n, C.c_int32) for n in ("nlft", "nrgh", "kill_flag")]
 
     def replace(self, **kw) -> "PnParams":
         out = PnParams()
@@ -117,6 +118,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_totals": (i, [vp, P(d), P(d)]),
         "pn_copy_bandwidth": (i, [sz, i, P(d)]),
         "pn_lab_fields": (i, [vp, sz, i, i, i, vp, vp, vp, vp, vp]),
+        "pn_damp_edges": (i, [vp, d]),
         "pn_save_snapshot": (i, [vp, C.c_char_p]),
         "pn_load_snapshot": (i, [vp, C.c_char_p]),
         "pn_batch_create": (i, [i, P(PnGrid), P(PnParams), i, P(vp)]),
@@ -281,6 +283,9 @@ class Context:
 
     def set_variants(self, var_x: int, var_y: int):
         self._chk(self.lib.pn_set_variants(self.h, var_x, var_y), "pn_set_variants")
+
+    def damp_edges(self, dt: float):
+        self._chk(self.lib.pn_damp_edges(self.h, dt), "pn_damp_edges")
 
     def variants(self) -> tuple[int, int]:
         vx, vy = C.c_int(), C.c_int()

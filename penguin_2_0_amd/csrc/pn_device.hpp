@@ -48,6 +48,8 @@ struct DevParams {
   double gam, courant, sc_h, p_alpha, p_beta, vis_nu;
   double M_p, t_growth, rs_fac;
   int eos, flat, plnt, visc, fargo, frrot, ngeomx, ngeomy, ngeomz, ndim, has_w;
+  int nlft, nrgh, kill;
+  double xmin, xmax;
 };
 
 // Scalars one step reads: device-resident so a whole step can be replayed as a graph.

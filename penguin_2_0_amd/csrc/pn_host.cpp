@@ -191,6 +191,9 @@ void pn_params_default(pn_params* p) {
   p->ymax = 6.283185307179586476925286766559;
   p->zmin = 1.570796326794896619231321691639751442 - 0.065;
   p->zmax = 1.570796326794896619231321691639751442;
+  p->nlft = 3;
+  p->nrgh = 3;
+  p->kill_flag = 0;
 }
 
 size_t pn_ring_bytes(int32_t jmax) { return static_cast<size_t>(72) * static_cast<size_t>(jmax) + 96; }

@@ -43,6 +43,65 @@ __global__ void k_rotate(DevGrid g) {
   g.rot_j[s] = j;
 }
 
+// set_boundx (boundary.cu:86-206) on the FP64 state of buffer src: each radial ghost cell
+// takes the values its boundary code names from the interior (a ghost ring's data index is
+// its lab column less its own rot_j; the interior ring's, less that ring's).  Code 3 (fixed
+// ghost rings) writes nothing.  Thread: one lab column of one of the 12 ghost rings of a
+// theta row.
+__global__ void k_fill_ghosts(DevGrid g, DevParams prm, int src) {
+  const int jl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (jl >= g.jmax) return;
+  const int gi = blockIdx.y % (2 * kNpad), k = blockIdx.y / (2 * kNpad);
+  const bool left = gi < kNpad;
+  const int code = left ? prm.nlft : prm.nrgh;
+  const int t = left ? kNpad - gi : gi - kNpad + 1;   // cells outside the edge cell
+  int srci;
+  switch (code) {
+    case 0: srci = left ? t : g.imax - 1 - t; break;
+    case 1:
+    case 4: srci = left ? 0 : g.imax - 1; break;
+    case 2: srci = left ? g.imax - t : t - 1; break;
+    default: return;
+  }
+  const int sg = left ? slot_of(g, -t, k) : slot_of(g, g.imax - 1 + t, k);
+  const int ss = slot_of(g, srci, k);
+  int jg = jl - g.rot_j[sg], js = jl - g.rot_j[ss];
+  jg += (jg < 0) ? g.jmax : 0;
+  jg -= (jg >= g.jmax) ? g.jmax : 0;
+  js += (js < 0) ? g.jmax : 0;
+  js -= (js >= g.jmax) ? g.jmax : 0;
+  const long ag = (long)sg * g.jmax + jg, as = (long)ss * g.jmax + js;
+  const int last = (code == 4) ? (left ? FV : FU) : kNField;   // fields [FR, last) are copied
+  for (int f = FR; f < last; ++f) g.f[src][f][ag] = g.f[src][f][as];
+  if (code == 0) g.f[src][FU][ag] = -g.f[src][FU][as];
+}
+
+// kill_wave (kill_wave.cu:3-43) on the FP64 state of buffer src with the step's dt: cells
+// within sc_h/2 of xmin or xmax relax towards their ring's background; w relaxes to 0.
+// Thread: one cell of an interior ring.
+__global__ void k_damp(DevGrid g, DevParams prm, const StepScalars* __restrict__ ss, int src) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= g.jmax) return;
+  const int n = blockIdx.y;   // ring i + imax k
+  const int s = slot_of(g, n % g.imax, n / g.imax);
+  const double d_kill = 0.5 * prm.sc_h;
+  const double inner = prm.xmin + d_kill, outer = prm.xmax - d_kill;
+  const double r = g.xc[s];
+  double f;
+  if (r < inner) f = (inner - r) / d_kill;
+  else if (r > outer) f = (r - outer) / d_kill;
+  else return;
+  f *= ss->dt * pow(r, -1.5);
+  const pn_symdisk V = g.val[n];
+  const long a = (long)s * g.jmax + j;
+  double* F[kNField] = {g.f[src][FR], g.f[src][FP], g.f[src][FU], g.f[src][FV], g.f[src][FW]};
+  F[FR][a] += f * (V.r - F[FR][a]);
+  F[FP][a] += f * (V.p - F[FP][a]);
+  F[FU][a] += f * (V.u - F[FU][a]);
+  F[FV][a] += f * (V.v - F[FV][a]);
+  F[FW][a] += f * (-F[FW][a]);
+}
+
 __device__ __forceinline__ double wave_max(double v) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
   return v;
@@ -433,6 +492,19 @@ int red_blocks(const DevGrid& g) {
 hipError_t launch_index_shift(const LaunchCtx& c) {
   int n = c.g.imax * c.g.kmax;
   k_index_shift<<<(n + 255) / 256, 256, 0, c.stream>>>(c.g, c.p, c.ss);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_ghosts(const LaunchCtx& c, int src) {
+  if (c.p.nlft == 3 && c.p.nrgh == 3) return hipSuccess;
+  dim3 grid((c.g.jmax + 255) / 256, 2 * kNpad * c.g.kmax);
+  k_fill_ghosts<<<grid, 256, 0, c.stream>>>(c.g, c.p, src);
+  return hipGetLastError();
+}
+
+hipError_t launch_damp(const LaunchCtx& c, int src) {
+  dim3 grid((c.g.jmax + 255) / 256, c.g.imax * c.g.kmax);
+  k_damp<<<grid, 256, 0, c.stream>>>(c.g, c.p, c.ss, src);
   return hipGetLastError();
 }
 

@@ -44,6 +44,11 @@ hipError_t launch_totals(const LaunchCtx& c, int src, double* out2);  // out2: d
 hipError_t launch_planet_after_sweep(const LaunchCtx& c, int src);  // simtime, drift, forces, half kick
 hipError_t launch_dt_finalize(const LaunchCtx& c, int n);           // dt = min(1.1 dt, courant / max of n cflp)
 hipError_t launch_sum_partials(const LaunchCtx& c, int nb, int stride, double* out);
+// radial ghost rings of buffer src from the interior under the boundary codes (FP64 state;
+// nothing for fixed ghost rings)
+hipError_t launch_fill_ghosts(const LaunchCtx& c, int src);
+// damping zones at the radial edges with the step's dt (FP64 state)
+hipError_t launch_damp(const LaunchCtx& c, int src);
 // plain device copy of nbytes (a multiple of 16)
 hipError_t launch_copy(hipStream_t s, const void* a, void* b, size_t nbytes);
 // CFL maxima per block into cflp (no dt); *n receives the partial count

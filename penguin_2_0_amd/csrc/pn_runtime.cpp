@@ -104,8 +104,16 @@ int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision)
   if (p->fargo_flag < 0 || p->fargo_flag > 2) return fail(c, PN_ERR_ARG, "fargo_flag %d", p->fargo_flag);
   if (p->plnt_flag < 0 || p->plnt_flag > 2) return fail(c, PN_ERR_ARG, "plnt_flag %d", p->plnt_flag);
   if (p->plnt_flag == 2 && p->ndim == 3) return fail(c, PN_ERR_UNSUPPORTED, "moving planet needs the 2-D force reductions");
+  if (p->nlft < 0 || p->nlft > 4 || p->nrgh < 0 || p->nrgh > 4)
+    return fail(c, PN_ERR_ARG, "radial boundary codes (%d, %d) outside 0..4", p->nlft, p->nrgh);
+  if (p->kill_flag != 0 && p->kill_flag != 1) return fail(c, PN_ERR_ARG, "kill_flag %d", p->kill_flag);
+  for (int code : {p->nlft, p->nrgh})
+    if ((code == 0 || code == 2) && g->imax < kNpad + 1)
+      return fail(c, PN_ERR_ARG, "radial boundary code %d needs imax > %d", code, kNpad);
   if (precision == PN_FP32 && p->eos != 0)
     return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for the isothermal disk (EOS 0)");
+  if (precision == PN_FP32 && (p->nlft != 3 || p->nrgh != 3 || p->kill_flag))
+    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path keeps fixed radial ghost rings and no damping zones");
   return PN_OK;
 }
 
@@ -153,6 +161,7 @@ int launch_sweeps(pn_ctx* c, int& cur, int do_cfl, bool prep, bool roll) {
       PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, cur ^ 1, do_cfl));
     }
   } else {
+    PN_LAUNCH(c, "ghosts", launch_fill_ghosts(L, cur));
     PN_LAUNCH(c, "sweep_x", fp64_sweep_x(L, cur));
     PN_LAUNCH(c, "sweep_y", fp64_sweep_y(L, cur ^ 1));
     if (c->P.ndim == 3) {
@@ -181,6 +190,8 @@ int launch_step(pn_ctx* c, int& cur) {
   PN_LAUNCH(c, "pre step", launch_pre_step(L, ncfl, trig, vrot, 1));
   const bool post = c->P.plnt_flag < 2;
   if ((rc = launch_sweeps(c, cur, fused ? 1 : 0, false, !post)) != PN_OK) return rc;
+  // damping zones: ring-uniform relaxation, which commutes with the integer roll
+  if (c->P.kill_flag) PN_LAUNCH(c, "damp", launch_damp(L, cur));
   if (post) PN_LAUNCH(c, "post step", launch_post_step(L, fused ? f32_fused_partials(c) : 0));
   else PN_LAUNCH(c, "planet update", launch_planet_after_sweep(L, cur));
   return PN_OK;
@@ -301,6 +312,11 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
   q.ngeomz = p->ngeomz;
   q.ndim = p->ndim;
   q.has_w = (p->ndim == 3) ? 1 : 0;
+  q.nlft = p->nlft;
+  q.nrgh = p->nrgh;
+  q.kill = p->kill_flag;
+  q.xmin = p->xmin;
+  q.xmax = p->xmax;
   StepScalars s0{};
   rc = upload_scalars(c, s0);
   *out = c;
@@ -790,6 +806,7 @@ int pn_sweep_axis(pn_ctx* c, int axis, double dt, const pn_body* planet, double 
     else if (axis == 1) PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, c->cur, 0));
     else PN_LAUNCH(c, "sweep_z", f32_sweep_z(L, c->df, c->var_x, c->cur, 0));
   } else {
+    if (axis == 0) PN_LAUNCH(c, "ghosts", launch_fill_ghosts(L, c->cur));
     hipError_t e = (axis == 0) ? fp64_sweep_x(L, c->cur) : (axis == 1) ? fp64_sweep_y(L, c->cur) : fp64_sweep_z(L, c->cur);
     if (e != hipSuccess) return fail(c, PN_ERR_HIP, "sweep %d: %s", axis, hipGetErrorString(e));
   }
@@ -838,6 +855,21 @@ int pn_time_kernels(pn_ctx* c, int nrep, double* ms6) {
     });
   }
   return rc;
+}
+
+int pn_damp_edges(pn_ctx* c, double dt) {
+  if (!c) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  if (!(dt > 0.0)) return fail(c, PN_ERR_ARG, "dt %g", dt);
+  if (is_f32(c)) return fail(c, PN_ERR_UNSUPPORTED, "damping zones on the FP32 path");
+  StepScalars h{};
+  int rc = download_scalars(c, &h);
+  if (rc != PN_OK) return rc;
+  h.dt = dt;
+  if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
+  LaunchCtx L = lctx(c);
+  PN_LAUNCH(c, "damp", launch_damp(L, c->cur));
+  return sync(c);
 }
 
 int pn_copy_bandwidth(size_t nbytes, int reps, double* gbs) {

@@ -46,6 +46,11 @@ def visc(nr=48, nphi=144):
     return w
 
 
+def reparam(w, **kw):
+    w.params = w.params.replace(**kw)
+    return w
+
+
 def named(w, suffix):
     w.name += suffix
     return w
@@ -69,6 +74,20 @@ CASES = [
     (named(configs.small(48, 144, flat_flag=0), "_noflat"), ("r", "u", "v"), (1e-13, 1e-11), (1e-11, 1e-8)),
     (named(configs.small(48, 144, t_growth=5.0), "_growing_planet"), ("r", "u", "v"), (1e-13, 1e-11),
      (1e-11, 1e-8)),
+    # radial boundary codes of set_boundx and the damping zones of kill_wave (dead code in the
+    # reference, live here): ghost rings filled from the interior before every radial sweep
+    (named(configs.small(48, 144, M_p=1e-3, nlft=0, nrgh=0), "_reflect"), ("r", "u", "v"), (1e-13, 1e-11),
+     (1e-11, 1e-8)),
+    (named(configs.small(48, 144, M_p=1e-3, nlft=1, nrgh=1), "_outflow"), ("r", "u", "v"), (1e-13, 1e-11),
+     (1e-11, 1e-8)),
+    (named(configs.small(48, 144, M_p=1e-3, nlft=2, nrgh=2), "_periodic_r"), ("r", "u", "v"), (1e-13, 1e-11),
+     (1e-11, 1e-8)),
+    (named(configs.small(48, 144, M_p=1e-3, nlft=4, nrgh=4, visc_flag=1, ss_alpha=1e-2, vis_nu=2.5e-5),
+           "_mixed_bound_visc"), ("r", "u", "v"), (1e-13, 1e-11), (1e-11, 1e-8)),
+    (named(configs.small(48, 144, M_p=1e-3, kill_flag=1, nlft=1, nrgh=1), "_damping"), ("r", "u", "v"),
+     (1e-13, 1e-11), (1e-11, 1e-8)),
+    (named(reparam(disk3d(24, 72, 12), nlft=1, nrgh=0), "_bound"), ("r", "u", "v", "w"), (1e-12, 1e-9),
+     (1e-10, 1e-7)),
     # theta volumes are cos(x) - cos(x + d) for the small Lagrangian displacement d
     # (evolve.cu:110-124, remap.cu:50-54): the last-ulp difference of the device and host cos
     # is amplified by that cancellation, most visibly in v_theta, which starts at zero

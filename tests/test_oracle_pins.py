@@ -495,3 +495,43 @@ def test_radial_sweep_equilibrium_residual_converges():
         # the median over rings: cells next to the fixed ghost rings converge more slowly
         res.append(np.median(np.abs(u)) / dt)
     assert res[0] / res[1] > 3.5 and res[1] / res[2] > 3.5
+
+
+def test_damping_zones_relax_towards_background_only_near_edges():
+    """kill_wave: a state equal to its background is a fixed point; otherwise every field's
+    distance to the background shrinks by the same factor 1 - f per cell, f growing linearly
+    from 0 at the zone's inner edge, and cells outside both zones do not change."""
+    w = configs.small(48, 144, kill_flag=1)
+    o = oracle.OracleDisk(w.grid, w.params, w.seed, 1e-3)
+    val = np.array([[v.r, v.p, v.u, v.v, v.w] for v in o.val])
+    rng = np.random.default_rng(3)
+    o.rings["u"] = 1e-3 * rng.standard_normal(o.rings["u"].shape)
+    o.rings["w"] = 1e-3 * rng.standard_normal(o.rings["w"].shape)
+    before = o.rings.copy()
+    dt = 0.01
+    o.damp(dt)
+    xc = before["xc"]
+    d_kill = 0.5 * w.params.sc_h
+    inner, outer = w.params.xmin + d_kill, w.params.xmax - d_kill
+    zone = (xc < inner) | (xc > outer)
+    assert zone.any() and (~zone).any()
+    for k, f in enumerate(("r", "p", "u", "v")):
+        d0 = before[f] - val[:, k:k + 1]
+        d1 = o.rings[f] - val[:, k:k + 1]
+        assert np.array_equal(o.rings[f][~zone], before[f][~zone]), f
+        ratio = d1[zone] / np.where(d0[zone] == 0, 1, d0[zone])
+        fac = ratio[np.abs(d0[zone]) > 1e-12 * np.abs(val[zone.nonzero()[0], k:k + 1]).max()]
+        assert np.all((fac > 0) & (fac < 1)), f
+    # the relaxation factor is the same for every field of a cell and grows towards the edge
+    d0 = before["w"][zone]
+    fac_w = o.rings["w"][zone] / d0
+    depth = np.where(xc[zone] < inner, inner - xc[zone], xc[zone] - outer)
+    expect = 1 - depth / d_kill * dt * xc[zone] ** -1.5
+    assert np.allclose(fac_w, expect[:, None], rtol=1e-12, atol=0)
+    # a state on its background is left alone
+    for k, f in enumerate(("r", "p", "u", "v")):
+        o.rings[f] = np.repeat(val[:, k:k + 1], w.grid.jmax, axis=1)
+    o.rings["w"] = 0.0
+    fixed = o.rings.copy()
+    o.damp(dt)
+    assert np.array_equal(o.rings.view(np.uint8), fixed.view(np.uint8))
