@@ -98,6 +98,10 @@ typedef struct pn_params {
    * background at rate (distance into the zone / width) r^-1.5 per unit time
    * (kill_wave.cu:3-43; kill_flag, global.h:22).  FP64 path only. */
   int32_t kill_flag;
+  /* 1: barycentric frame (bary_flag, global.h:17): star and planet orbit their barycentre,
+   * the star's pull is computed from its offset position and the indirect term goes
+   * (forces.cu:28-182).  FP64 path only. */
+  int32_t bary_flag;
 } pn_params;
 
 /* State a step loop carries between steps (main.cu:270-275, 325-327). */

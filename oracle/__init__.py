@@ -51,6 +51,7 @@ def lib() -> C.CDLL:
             "orc_force_rad_cyl": (d, [B, d, d, d, d]),
             "orc_force_azi_cyl": (d, [B, d, d, d, d]),
             "orc_potential": (d, [Pp, B, d, d, d]),
+            "orc_gravity": (d, [i, i, i, B, d, d, d, d]),
             "orc_grid": (None, [i, d, d, i, dp, dp, dp]),
             "orc_ppmlr_window": (None, [Pp, i, i, d, d, B, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]),
             "orc_splitmix64": (C.c_uint64, [C.c_uint64]),

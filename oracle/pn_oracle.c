@@ -510,11 +510,22 @@ double orc_potential(const pn_params* P, const pn_body* p, double rad, double az
   return -1.0 / sqrt(rad * rad + z * z);
 }
 
-static double grav_rad(const pn_body* p, double rad, double azi, double pol, double dt) {
+/* star_planet_grav_rad / _azi / _pol (forces.cu:23-110), spherical.  Without bary_flag
+   the star sits at the origin (its pull -stm/r^2 and the indirect term of the planet
+   included); with it star and planet orbit their barycentre (star at -x m/(1+m), planet at
+   x/(1+m)) and there is no indirect term. */
+static double grav_rad(const pn_body* p, int bary, double rad, double azi, double pol, double dt) {
   double cosfac = cos(azi - (p->y + dt * p->vy));
   double sinpol = sin(pol);
   double stm = 1.0 / (1.0 + p->m);
   double plm = p->m * stm;
+  if (bary) {
+    double plx = p->x * stm, stx = p->x * plm;
+    double Rp = rad * rad + plx * plx - 2.0 * plx * rad * cosfac * sinpol;
+    Rp = sqrt(Rp + p->rs * p->rs);
+    double Rs = sqrt(rad * rad + stx * stx + 2.0 * stx * rad * cosfac * sinpol);
+    return -plm * (rad - plx * cosfac * sinpol) / (Rp * Rp * Rp) - stm * (rad + stx * cosfac * sinpol) / (Rs * Rs * Rs);
+  }
   double plx = p->x;
   double Rp = rad * rad + plx * plx - 2.0 * plx * rad * cosfac * sinpol;
   Rp = sqrt(Rp + p->rs * p->rs);
@@ -522,54 +533,109 @@ static double grav_rad(const pn_body* p, double rad, double azi, double pol, dou
          plm * sinpol * cosfac / (plx * plx);
 }
 
-static double grav_azi(const pn_body* p, double rad, double azi, double pol, double dt) {
+static double grav_azi(const pn_body* p, int bary, double rad, double azi, double pol, double dt) {
   double cosfac = cos(azi - (p->y + dt * p->vy));
   double sinfac = sin(azi - (p->y + dt * p->vy));
   double sinpol = sin(pol);
   rad /= sinpol;
   double stm = 1.0 / (1.0 + p->m);
   double plm = p->m * stm;
+  if (bary) {
+    double plx = p->x * stm, stx = p->x * plm;
+    double Rp = rad * rad + plx * plx - 2.0 * plx * rad * cosfac * sinpol;
+    Rp = sqrt(Rp + p->rs * p->rs);
+    double Rs = sqrt(rad * rad + stx * stx + 2.0 * stx * rad * cosfac * sinpol);
+    return -plm * plx * sinfac / (Rp * Rp * Rp) + stm * stx * sinfac / (Rs * Rs * Rs);
+  }
   double plx = p->x;
   double Rp = rad * rad + plx * plx - 2.0 * plx * rad * cosfac * sinpol;
   Rp = sqrt(Rp + p->rs * p->rs);
   return -plm * plx * sinfac / (Rp * Rp * Rp) + plm * sinpol * sinfac / plx / plx;
 }
 
-static double grav_pol(const pn_body* p, double rad, double azi, double pol, double dt) {
+static double grav_pol(const pn_body* p, int bary, double rad, double azi, double pol, double dt) {
   double cosfac = cos(azi - (p->y + dt * p->vy));
   double cospol = cos(pol);
   double sinpol = sin(pol);
   double stm = 1.0 / (1.0 + p->m);
   double plm = p->m * stm;
+  if (bary) {
+    double plx = p->x * stm, stx = p->x * plm;
+    double Rp = rad * rad + plx * plx - 2.0 * plx * rad * cosfac * sinpol;
+    Rp = sqrt(Rp + p->rs * p->rs);
+    double Rs = sqrt(rad * rad + stx * stx + 2.0 * stx * rad * cosfac * sinpol);
+    return plm * plx * cosfac * cospol / (Rp * Rp * Rp) - stm * stx * cosfac * cospol / (Rs * Rs * Rs);
+  }
   double plx = p->x;
   double Rp = rad * rad + plx * plx - 2.0 * plx * rad * cosfac * sinpol;
   Rp = sqrt(Rp + p->rs * p->rs);
   return plm * plx * cosfac * cospol / (Rp * Rp * Rp) - plm * cosfac * cospol / (sinpol * plx * plx);
 }
 
-double orc_force_rad_cyl(const pn_body* p, double rad, double azi, double z, double dt) {
+/* star_planet_grav_*_cyl (forces.cu:112-182), cylindrical; without bary_flag the planet with
+   its indirect term (the star's -1/r^2 is added by the caller), with it star and planet
+   about their barycentre. */
+static double grav_rad_cyl(const pn_body* p, int bary, double rad, double azi, double z, double dt) {
   double cosfac = cos(azi - (p->y + dt * p->vy));
+  if (bary) {
+    double stm = 1.0 / (1.0 + p->m), plm = p->m * stm, plx = p->x * stm, stx = p->x * plm;
+    double Rp = sqrt(rad * rad + plx * plx - 2.0 * plx * rad * cosfac + z * z + p->rs * p->rs);
+    double Rs = sqrt(rad * rad + stx * stx + 2.0 * stx * rad * cosfac + z * z);
+    return -plm * (rad - plx * cosfac) / (Rp * Rp * Rp) - stm * (rad + stx * cosfac) / (Rs * Rs * Rs);
+  }
   double plm = p->m;
   double plx = p->x;
   double Rp = sqrt(rad * rad + plx * plx - 2.0 * plx * rad * cosfac + z * z + p->rs * p->rs);
   return -plm * (rad - plx * cosfac) / (Rp * Rp * Rp) - plm * cosfac / (plx * plx);
 }
 
-double orc_force_azi_cyl(const pn_body* p, double rad, double azi, double z, double dt) {
+static double grav_azi_cyl(const pn_body* p, int bary, double rad, double azi, double z, double dt) {
   double cosfac = cos(azi - (p->y + dt * p->vy));
   double sinfac = sin(azi - (p->y + dt * p->vy));
+  if (bary) {
+    double stm = 1.0 / (1.0 + p->m), plm = p->m * stm, plx = p->x * stm, stx = p->x * plm;
+    double Rp = sqrt(rad * rad + plx * plx - 2.0 * plx * rad * cosfac + z * z + p->rs * p->rs);
+    double Rs = sqrt(rad * rad + stx * stx + 2.0 * stx * rad * cosfac + z * z);
+    return -plm * plx * sinfac / (Rp * Rp * Rp) + stm * stx * sinfac / (Rs * Rs * Rs);
+  }
   double plm = p->m;
   double plx = p->x;
   double Rp = sqrt(rad * rad + plx * plx - 2.0 * plx * rad * cosfac + z * z + p->rs * p->rs);
   return -plm * plx * sinfac / (Rp * Rp * Rp) + plm * sinfac / plx / plx;
 }
 
-static double grav_pol_cyl(const pn_body* p, double rad, double azi, double z, double dt) {
+static double grav_pol_cyl(const pn_body* p, int bary, double rad, double azi, double z, double dt) {
   double cosfac = cos(azi - (p->y + dt * p->vy));
+  if (bary) {
+    double stm = 1.0 / (1.0 + p->m), plm = p->m * stm, plx = p->x * stm, stx = p->x * plm;
+    double Rp = sqrt(rad * rad + plx * plx - 2.0 * plx * rad * cosfac + z * z + p->rs * p->rs);
+    double Rs = sqrt(rad * rad + stx * stx + 2.0 * stx * rad * cosfac + z * z);
+    return -plm * z / (Rp * Rp * Rp) - stm * z / (Rs * Rs * Rs);
+  }
   double plm = p->m;
   double plx = p->x;
   double Rp = sqrt(rad * rad + plx * plx - 2.0 * plx * rad * cosfac + z * z + p->rs * p->rs);
   return -plm * z / (Rp * Rp * Rp);
+}
+
+double orc_force_rad_cyl(const pn_body* p, double rad, double azi, double z, double dt) {
+  return grav_rad_cyl(p, 0, rad, azi, z, dt);
+}
+double orc_force_azi_cyl(const pn_body* p, double rad, double azi, double z, double dt) {
+  return grav_azi_cyl(p, 0, rad, azi, z, dt);
+}
+/* the gravity the sweeps apply, component comp (0 r, 1 phi, 2 z / theta) in geometry geo
+   (0 cylindrical, 1 spherical), with or without the barycentric frame; cylindrical r
+   includes the star's -1/r^2 outside the barycentric frame, as get_fx adds it */
+double orc_gravity(int geo, int comp, int bary, const pn_body* p, double rad, double azi, double pol, double dt) {
+  if (geo == 0) {
+    if (comp == 0) return grav_rad_cyl(p, bary, rad, azi, pol, dt) + (bary ? 0.0 : -1.0 / rad / rad);
+    if (comp == 1) return grav_azi_cyl(p, bary, rad, azi, pol, dt);
+    return grav_pol_cyl(p, bary, rad, azi, pol, dt);
+  }
+  if (comp == 0) return grav_rad(p, bary, rad, azi, pol, dt);
+  if (comp == 1) return grav_azi(p, bary, rad, azi, pol, dt);
+  return grav_pol(p, bary, rad, azi, pol, dt);
 }
 
 /* get_fx_bound / get_fx (forces.cu:185-263): interface (bound) forms use the
@@ -581,8 +647,8 @@ static void get_fx(const pn_params* P, int bound, int n, double dt, double rad, 
     *ftot = *grav;
   } else if (P->ngeomx == 1) {
     if (P->plnt_flag > 0) {
-      *grav = orc_force_rad_cyl(pl, rad, azi, pol, dt);
-      *grav += -1.0 / rad / rad;
+      *grav = grav_rad_cyl(pl, P->bary_flag, rad, azi, pol, dt);
+      if (!P->bary_flag) *grav += -1.0 / rad / rad;
     } else {
       double dis = sqrt(rad * rad + pol * pol);
       *grav = -rad / dis / dis / dis;
@@ -591,7 +657,7 @@ static void get_fx(const pn_params* P, int bound, int n, double dt, double rad, 
     if (bound) *ftot += pow(0.5 * (v[n] + v[n - 1]), 2) / rad / rad / rad;
     else *ftot += v[n] * v[n] / rad / rad / rad;
   } else if (P->ngeomx == 2) {
-    if (P->plnt_flag > 0) *grav = grav_rad(pl, rad, azi, pol, dt);
+    if (P->plnt_flag > 0) *grav = grav_rad(pl, P->bary_flag, rad, azi, pol, dt);
     else *grav = -1.0 / rad / rad;
     *ftot = *grav;
     if (bound) {
@@ -607,9 +673,9 @@ static void get_fx(const pn_params* P, int bound, int n, double dt, double rad, 
 static void get_fy(const pn_params* P, double dt, double rad, double azi, double pol, double* grav,
                    double* ftot, const pn_body* pl) {
   if (P->ngeomy == 3) {
-    *grav = (P->plnt_flag > 0) ? orc_force_azi_cyl(pl, rad, azi, pol, dt) : 0.0;
+    *grav = (P->plnt_flag > 0) ? grav_azi_cyl(pl, P->bary_flag, rad, azi, pol, dt) : 0.0;
   } else if (P->ngeomy == 4) {
-    *grav = (P->plnt_flag > 0) ? grav_azi(pl, rad, azi, pol, dt) : 0.0;
+    *grav = (P->plnt_flag > 0) ? grav_azi(pl, P->bary_flag, rad, azi, pol, dt) : 0.0;
   } else {
     *grav = 0.0;
   }
@@ -620,14 +686,14 @@ static void get_fz(const pn_params* P, int bound, int n, double dt, double rad, 
                    double* grav, double* ftot, const pn_body* pl, const double* w) {
   if (P->ngeomz == 0) {
     if (P->plnt_flag > 0) {
-      *grav = grav_pol_cyl(pl, rad, azi, pol, dt);
+      *grav = grav_pol_cyl(pl, P->bary_flag, rad, azi, pol, dt);
     } else {
       double dis = sqrt(rad * rad + pol * pol);
       *grav = -pol / dis / dis / dis;
     }
     *ftot = *grav;
   } else if (P->ngeomz == 5) {
-    *grav = (P->plnt_flag > 0) ? grav_pol(pl, rad, azi, pol, dt) : 0.0;
+    *grav = (P->plnt_flag > 0) ? grav_pol(pl, P->bary_flag, rad, azi, pol, dt) : 0.0;
     double rad_cyl = rad * sin(pol);
     *ftot = *grav;
     if (bound) *ftot += (pow(0.5 * (w[n] + w[n - 1]), 2) * cos(pol)) / rad_cyl / rad_cyl / rad_cyl;
@@ -1592,10 +1658,10 @@ double orc_reduce(const pn_grid* g, const pn_params* P, const void* rings, const
         double dd = (rad - planet->x) * (rad - planet->x) + rad * rad * (phi - planet->y) * (phi - planet->y);
         if (dd > exclude * exclude) {
           if (mode == 8) {
-            tmp = -orc_force_rad_cyl(planet, rad, phi, 0.0, 0.0);
+            tmp = -grav_rad_cyl(planet, P->bary_flag, rad, phi, 0.0, 0.0);
             tmp *= (R.r[j] - pow(*R.xc, -P->p_alpha)) * *R.xvol * R.yvol[j0];
           } else {
-            tmp = -orc_force_azi_cyl(planet, rad, phi, 0.0, 0.0);
+            tmp = -grav_azi_cyl(planet, P->bary_flag, rad, phi, 0.0, 0.0);
             tmp *= rad * (R.r[j] - pow(*R.xc, -P->p_alpha)) * *R.xvol * R.yvol[j0];
           }
         }

@@ -37,6 +37,7 @@ void orc_sweepy(const pn_grid* g, const pn_params* p, void* rings, double dt, co
 void orc_sweepz(const pn_grid* g, const pn_params* p, void* rings, const void* udr, const void* top,
                 double dt, const pn_body* planet, double FrRot);
 void orc_rotate(const pn_grid* g, void* rings);
+double orc_gravity(int geo, int comp, int bary, const pn_body* p, double rad, double azi, double pol, double dt);
 void orc_damp(const pn_grid* g, const pn_params* p, void* rings, const pn_symdisk* val, double dt);
 double orc_reduce(const pn_grid* g, const pn_params* p, const void* rings, const pn_body* planet,
                   double exclude, int mode);

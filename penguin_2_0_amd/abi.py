@@ -40,7 +40,7 @@ class PnParams(C.Structure):
         [(n, C.c_double) for n in (
             "p_alpha", "p_beta", "ss_alpha", "sc_h", "vis_nu", "M_p", "R_p", "t_growth", "rs_fac",
             "gam", "courant", "endtime", "tmovie", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax")] + \
-        [(n, C.c_int32) for n in ("nlft", "nrgh", "kill_flag")]
+        [(n, C.c_int32) for n in ("nlft", "nrgh", "kill_flag", "bary_flag")]
 
     def replace(self, **kw) -> "PnParams":
         out = PnParams()

@@ -48,7 +48,7 @@ struct DevParams {
   double gam, courant, sc_h, p_alpha, p_beta, vis_nu;
   double M_p, t_growth, rs_fac;
   int eos, flat, plnt, visc, fargo, frrot, ngeomx, ngeomy, ngeomz, ndim, has_w;
-  int nlft, nrgh, kill;
+  int nlft, nrgh, kill, bary;
   double xmin, xmax;
 };
 

@@ -194,6 +194,7 @@ void pn_params_default(pn_params* p) {
   p->nlft = 3;
   p->nrgh = 3;
   p->kill_flag = 0;
+  p->bary_flag = 0;
 }
 
 size_t pn_ring_bytes(int32_t jmax) { return static_cast<size_t>(72) * static_cast<size_t>(jmax) + 96; }

@@ -534,7 +534,7 @@ __global__ __launch_bounds__(256) void k_reduce_f32(DevGrid g, DevF32 f, DevPara
                                                     double exclude, double* partials) {
   __shared__ double sh[4];
   const long ncell = (long)g.imax * g.kmax * g.jmax;
-  const Planet pl{ss->planet.m, ss->planet.x, ss->planet.y, ss->planet.vx, ss->planet.vy, ss->planet.rs};
+  const Planet pl{ss->planet.m, ss->planet.x, ss->planet.y, ss->planet.vx, ss->planet.vy, ss->planet.rs, prm.bary};
   double acc = 0.0;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ncell; t += (long)gridDim.x * blockDim.x) {
     const int j = (int)(t % g.jmax);

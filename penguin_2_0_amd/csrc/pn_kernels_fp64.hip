@@ -38,8 +38,9 @@ __device__ __forceinline__ int win_len(int len, int nwin, int l) {
   return (l < nwin - 1) ? kWin : (len - kReal * (nwin - 1) + 2 * kNpad);
 }
 
-__device__ __forceinline__ Planet load_planet(const StepScalars* ss) {
+__device__ __forceinline__ Planet load_planet(const StepScalars* ss, int bary) {
   Planet p;
+  p.bary = bary;
   p.m = ss->planet.m;
   p.x = ss->planet.x;
   p.y = ss->planet.y;
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(DevGrid g, DevParams prm, const
   if (k >= g.kmax) return;   // wave-uniform; the kernel has no barriers
 
   const double dt = ss->dt, FrRot = ss->FrRot;
-  const Planet pl = load_planet(ss);
+  const Planet pl = load_planet(ss, prm.bary);
   RunFlags rf{prm.flat, prm.plnt > 0, 1, 1, prm.ndim};
   const int lim = win_len(g.imax, g.nwin_x, l);
   rf.remap_v = 1;
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(256) void k_sweep_y(DevGrid g, DevParams prm, const
   const int jmax = g.jmax;
 
   const double dt = ss->dt, FrRot = ss->FrRot;
-  const Planet pl = load_planet(ss);
+  const Planet pl = load_planet(ss, prm.bary);
   // sweep slots: u = v_phi, v = physical w, w = v_r (always carried)
   RunFlags rf{prm.flat, prm.plnt > 0, prm.has_w, 1, prm.ndim};
   const int lim = win_len(jmax, g.nwin_y, l);
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(256) void k_sweep_z(DevGrid g, DevParams prm, const
   if (i >= g.imax) return;
 
   const double dt = ss->dt, FrRot = ss->FrRot;
-  const Planet pl = load_planet(ss);
+  const Planet pl = load_planet(ss, prm.bary);
   RunFlags rf{prm.flat, prm.plnt > 0, 1, 1, prm.ndim};
   const int lim = win_len(g.kmax, g.nwin_z, l);
 

@@ -112,8 +112,11 @@ int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision)
       return fail(c, PN_ERR_ARG, "radial boundary code %d needs imax > %d", code, kNpad);
   if (precision == PN_FP32 && p->eos != 0)
     return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for the isothermal disk (EOS 0)");
+  if (p->bary_flag != 0 && p->bary_flag != 1) return fail(c, PN_ERR_ARG, "bary_flag %d", p->bary_flag);
   if (precision == PN_FP32 && (p->nlft != 3 || p->nrgh != 3 || p->kill_flag))
     return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path keeps fixed radial ghost rings and no damping zones");
+  if (precision == PN_FP32 && p->bary_flag)
+    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path balances its backgrounds about a star at the origin");
   return PN_OK;
 }
 
@@ -315,6 +318,7 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
   q.nlft = p->nlft;
   q.nrgh = p->nrgh;
   q.kill = p->kill_flag;
+  q.bary = p->bary_flag;
   q.xmin = p->xmin;
   q.xmax = p->xmax;
   StepScalars s0{};

@@ -113,16 +113,38 @@ __device__ __forceinline__ T max3(T A, T B, T Cc) {   // simple_func.cu:16-28
 }
 
 // ---------------------------------------------------------------------------------------
-// planet + star gravity (forces.cu:4-182, bary_flag undefined)
+// planet + star gravity (forces.cu:4-182).  Without bary: the star at the origin (its -1/r^2
+// added by the caller in cylindrical r, included in the spherical forms) and the planet's
+// indirect term; with bary (bary_flag): star at -x m/(1+m) and planet at x/(1+m) about their
+// barycentre, no indirect term.
 
 struct Planet {
   double m, x, y, vx, vy, rs;
+  int bary;
+};
+
+template <typename T>
+struct BaryPos {   // stm, plm and the two bodies' distances from the barycentre
+  T stm, plm, plx, stx;
+  __device__ __forceinline__ BaryPos(const Planet& p) {
+    stm = T(1.0) / (T(1.0) + (T)p.m);
+    plm = (T)p.m * stm;
+    plx = (T)p.x * stm;
+    stx = (T)p.x * plm;
+  }
 };
 
 template <typename T>
 __device__ __forceinline__ T grav_rad_cyl(const Planet& p, T rad, T azi, T z, T dt) {   // forces.cu:112-134
   T cosfac = cos(azi - ((T)p.y + dt * (T)p.vy));
-  T plm = (T)p.m, plx = (T)p.x, rs = (T)p.rs;
+  T rs = (T)p.rs;
+  if (p.bary) {
+    const BaryPos<T> b(p);
+    T Rp = sqrt(rad * rad + b.plx * b.plx - T(2.0) * b.plx * rad * cosfac + z * z + rs * rs);
+    T Rs = sqrt(rad * rad + b.stx * b.stx + T(2.0) * b.stx * rad * cosfac + z * z);
+    return -b.plm * (rad - b.plx * cosfac) / (Rp * Rp * Rp) - b.stm * (rad + b.stx * cosfac) / (Rs * Rs * Rs);
+  }
+  T plm = (T)p.m, plx = (T)p.x;
   T Rp = sqrt(rad * rad + plx * plx - T(2.0) * plx * rad * cosfac + z * z + rs * rs);
   return -plm * (rad - plx * cosfac) / (Rp * Rp * Rp) - plm * cosfac / (plx * plx);
 }
@@ -131,14 +153,28 @@ __device__ __forceinline__ T grav_azi_cyl(const Planet& p, T rad, T azi, T z, T 
   T ph = azi - ((T)p.y + dt * (T)p.vy);
   T cosfac = cos(ph);
   T sinfac = sin(ph);
-  T plm = (T)p.m, plx = (T)p.x, rs = (T)p.rs;
+  T rs = (T)p.rs;
+  if (p.bary) {
+    const BaryPos<T> b(p);
+    T Rp = sqrt(rad * rad + b.plx * b.plx - T(2.0) * b.plx * rad * cosfac + z * z + rs * rs);
+    T Rs = sqrt(rad * rad + b.stx * b.stx + T(2.0) * b.stx * rad * cosfac + z * z);
+    return -b.plm * b.plx * sinfac / (Rp * Rp * Rp) + b.stm * b.stx * sinfac / (Rs * Rs * Rs);
+  }
+  T plm = (T)p.m, plx = (T)p.x;
   T Rp = sqrt(rad * rad + plx * plx - T(2.0) * plx * rad * cosfac + z * z + rs * rs);
   return -plm * plx * sinfac / (Rp * Rp * Rp) + plm * sinfac / plx / plx;
 }
 template <typename T>
 __device__ __forceinline__ T grav_pol_cyl(const Planet& p, T rad, T azi, T z, T dt) {   // forces.cu:161-182
   T cosfac = cos(azi - ((T)p.y + dt * (T)p.vy));
-  T plm = (T)p.m, plx = (T)p.x, rs = (T)p.rs;
+  T rs = (T)p.rs;
+  if (p.bary) {
+    const BaryPos<T> b(p);
+    T Rp = sqrt(rad * rad + b.plx * b.plx - T(2.0) * b.plx * rad * cosfac + z * z + rs * rs);
+    T Rs = sqrt(rad * rad + b.stx * b.stx + T(2.0) * b.stx * rad * cosfac + z * z);
+    return -b.plm * z / (Rp * Rp * Rp) - b.stm * z / (Rs * Rs * Rs);
+  }
+  T plm = (T)p.m, plx = (T)p.x;
   T Rp = sqrt(rad * rad + plx * plx - T(2.0) * plx * rad * cosfac + z * z + rs * rs);
   return -plm * z / (Rp * Rp * Rp);
 }
@@ -146,9 +182,18 @@ template <typename T>
 __device__ __forceinline__ T grav_rad_sph(const Planet& p, T rad, T azi, T pol, T dt) {   // forces.cu:23-50
   T cosfac = cos(azi - ((T)p.y + dt * (T)p.vy));
   T sinpol = sin(pol);
+  T rs = (T)p.rs;
+  if (p.bary) {
+    const BaryPos<T> b(p);
+    T Rp = rad * rad + b.plx * b.plx - T(2.0) * b.plx * rad * cosfac * sinpol;
+    Rp = sqrt(Rp + rs * rs);
+    T Rs = sqrt(rad * rad + b.stx * b.stx + T(2.0) * b.stx * rad * cosfac * sinpol);
+    return -b.plm * (rad - b.plx * cosfac * sinpol) / (Rp * Rp * Rp) -
+           b.stm * (rad + b.stx * cosfac * sinpol) / (Rs * Rs * Rs);
+  }
   T stm = T(1.0) / (T(1.0) + (T)p.m);
   T plm = (T)p.m * stm;
-  T plx = (T)p.x, rs = (T)p.rs;
+  T plx = (T)p.x;
   T Rp = rad * rad + plx * plx - T(2.0) * plx * rad * cosfac * sinpol;
   Rp = sqrt(Rp + rs * rs);
   return -plm * (rad - plx * cosfac * sinpol) / (Rp * Rp * Rp) - stm / (rad * rad) -
@@ -161,9 +206,17 @@ __device__ __forceinline__ T grav_azi_sph(const Planet& p, T rad, T azi, T pol, 
   T sinfac = sin(ph);
   T sinpol = sin(pol);
   rad /= sinpol;
+  T rs = (T)p.rs;
+  if (p.bary) {
+    const BaryPos<T> b(p);
+    T Rp = rad * rad + b.plx * b.plx - T(2.0) * b.plx * rad * cosfac * sinpol;
+    Rp = sqrt(Rp + rs * rs);
+    T Rs = sqrt(rad * rad + b.stx * b.stx + T(2.0) * b.stx * rad * cosfac * sinpol);
+    return -b.plm * b.plx * sinfac / (Rp * Rp * Rp) + b.stm * b.stx * sinfac / (Rs * Rs * Rs);
+  }
   T stm = T(1.0) / (T(1.0) + (T)p.m);
   T plm = (T)p.m * stm;
-  T plx = (T)p.x, rs = (T)p.rs;
+  T plx = (T)p.x;
   T Rp = rad * rad + plx * plx - T(2.0) * plx * rad * cosfac * sinpol;
   Rp = sqrt(Rp + rs * rs);
   return -plm * plx * sinfac / (Rp * Rp * Rp) + plm * sinpol * sinfac / plx / plx;
@@ -173,9 +226,17 @@ __device__ __forceinline__ T grav_pol_sph(const Planet& p, T rad, T azi, T pol, 
   T cosfac = cos(azi - ((T)p.y + dt * (T)p.vy));
   T cospol = cos(pol);
   T sinpol = sin(pol);
+  T rs = (T)p.rs;
+  if (p.bary) {
+    const BaryPos<T> b(p);
+    T Rp = rad * rad + b.plx * b.plx - T(2.0) * b.plx * rad * cosfac * sinpol;
+    Rp = sqrt(Rp + rs * rs);
+    T Rs = sqrt(rad * rad + b.stx * b.stx + T(2.0) * b.stx * rad * cosfac * sinpol);
+    return b.plm * b.plx * cosfac * cospol / (Rp * Rp * Rp) - b.stm * b.stx * cosfac * cospol / (Rs * Rs * Rs);
+  }
   T stm = T(1.0) / (T(1.0) + (T)p.m);
   T plm = (T)p.m * stm;
-  T plx = (T)p.x, rs = (T)p.rs;
+  T plx = (T)p.x;
   T Rp = rad * rad + plx * plx - T(2.0) * plx * rad * cosfac * sinpol;
   Rp = sqrt(Rp + rs * rs);
   return plm * plx * cosfac * cospol / (Rp * Rp * Rp) - plm * cosfac * cospol / (sinpol * plx * plx);
@@ -337,7 +398,7 @@ __device__ __forceinline__ T force_x(const RunFlags& rf, const Planet& pl, bool 
   if constexpr (CF::GX == 1) {
     if (rf.planet) {
       grav = grav_rad_cyl<T>(pl, rad, azi, pol, dt);
-      grav += T(-1.0) / rad / rad;
+      if (!pl.bary) grav += T(-1.0) / rad / rad;
     } else {
       T dis = sqrt(rad * rad + pol * pol);
       grav = -rad / dis / dis / dis;

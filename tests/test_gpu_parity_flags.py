@@ -88,6 +88,14 @@ CASES = [
      (1e-13, 1e-11), (1e-11, 1e-8)),
     (named(reparam(disk3d(24, 72, 12), nlft=1, nrgh=0), "_bound"), ("r", "u", "v", "w"), (1e-12, 1e-9),
      (1e-10, 1e-7)),
+    # barycentric frame (bary_flag): star and planet about their barycentre, in the sweeps and
+    # in the disk-on-planet force reductions of the moving planet
+    (named(configs.small(48, 144, M_p=1e-3, bary_flag=1), "_bary"), ("r", "u", "v"), (1e-13, 1e-11),
+     (1e-11, 1e-8)),
+    (named(configs.small(48, 144, plnt_flag=2, M_p=1e-3, bary_flag=1), "_bary_moving_planet"), ("r", "u", "v"),
+     (1e-13, 1e-11), (1e-11, 1e-8)),
+    (named(reparam(disk3d(24, 72, 12), bary_flag=1), "_bary"), ("r", "u", "v", "w"), (1e-12, 1e-9),
+     (1e-10, 1e-7)),
     # theta volumes are cos(x) - cos(x + d) for the small Lagrangian displacement d
     # (evolve.cu:110-124, remap.cu:50-54): the last-ulp difference of the device and host cos
     # is amplified by that cancellation, most visibly in v_theta, which starts at zero
