@@ -121,26 +121,34 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
   const int step = TH ? g.sr : 1;   // slots between neighbouring cells of the pencil
   for (int c = threadIdx.x; c < WIN; c += 64 * W) srot[c] = g.rot_j[s0 + step * (c < lim ? c : lim - 1)];
   __syncthreads();
+  // the fields as buffer views with 32-bit byte offsets (a state buffer is below 2 GiB)
+  const unsigned fbytes = 4u * (unsigned)(g.sr * g.sk) * (unsigned)jmax;
+  Buf fin[NF];
+#pragma unroll
+  for (int fi = 0; fi < NF; ++fi) fin[fi] = Buf(f.f[src][fi], fbytes);
   for (int e = threadIdx.x; e < WIN * NC; e += 64 * W) {
     const int q = e % NC, c = e / NC;
     const int jl = jb + q;
     if (jl < jmax) {
-      const long a = (long)(s0 + step * (c < lim ? c : lim - 1)) * jmax + wrapj(jl - srot[c], jmax);
+      const unsigned o = 4u * (unsigned)((s0 + step * (c < lim ? c : lim - 1)) * jmax + wrapj(jl - srot[c], jmax));
 #pragma unroll
-      for (int fi = 0; fi < NF; ++fi) tile[fi][q][c] = f.f[src][fi][a];
+      for (int fi = 0; fi < NF; ++fi) tile[fi][q][c] = fin[fi].f(o);
     }
   }
   __syncthreads();
 
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform, in a scalar register
   const float dt = (float)ss->dt;
   const PlanetF pl = planet_f(ss, G != kGeoRcyl);
   const float sqrt_gam = sqrtf((float)prm.gam);
   // radial: overhang cells read copies of the last slot; theta: reads clamped to the window
   // theta: the theta-major table, whose slots run along the pencil (reads past the window
   // land on the next ring's slots or the overhang: finite values no kept cell depends on)
-  const TabLane<K> tx = TH ? TabLane<K>{f.tabz, f.tstride, (REAL * l - kNpad + g.k0) + g.sk * (row + kNpad) + K * lane}
-                           : TabLane<K>{f.tab, f.tstride, s0 + K * lane};
+  const unsigned tbytes = 4u * (unsigned)kTabEntries * (unsigned)f.tstride;
+  const TabLane<K> tx{Buf(TH ? f.tabz : f.tab, tbytes),
+                      4u * (unsigned)((TH ? (REAL * l - kNpad + g.k0) + g.sk * (row + kNpad) : s0) + K * lane),
+                      f.tstride};
   for (int qq = 0; qq < Q; ++qq) {
     const int q = wv * Q + qq;
     const int j0 = jb + q;
@@ -235,15 +243,18 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
   __syncthreads();
   float m = 0.0f;
   const double FrRot = ss->FrRot;
+  Buf fout[NF];
+#pragma unroll
+  for (int fi = 0; fi < NF; ++fi) fout[fi] = Buf(f.f[src ^ 1][fi], fbytes);
   for (int e = threadIdx.x; e < WIN * NC; e += 64 * W) {
     const int q = e % NC, c = e / NC;
     const int jl = jb + q;
     if (jl < jmax && c >= kNpad && c < lim - kNpad) {
       const int sl = s0 + step * c;
       const int j = wrapj(jl - srot[c], jmax);
-      const long a = (long)sl * jmax + j;
+      const unsigned o = 4u * (unsigned)(sl * jmax + j);
 #pragma unroll
-      for (int fi = 0; fi < NF; ++fi) f.f[src ^ 1][fi][a] = tile[fi][q][c];
+      for (int fi = 0; fi < NF; ++fi) fout[fi].put(tile[fi][q][c], o);
       if constexpr (CFL) {
         // the next step's CFL term of this cell (timestep.cu:12-108), isothermal c_s
         const float* T = f.tab + sl;
@@ -364,27 +375,31 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
     sth = T[kTabSth * ts];
     oms = T[kTabOms * ts];
   }
+  // the ring's fields and the column tables as buffer views (byte offsets 4 j)
   const long base = (long)s * jmax;
-  const float* __restrict__ fr = f.f[src][0] + base;
-  const float* __restrict__ fu = f.f[src][1] + base;
-  const float* __restrict__ fv = f.f[src][2] + base;
-  const float* __restrict__ fw = (NDIM == 3) ? f.f[src][3] + base : nullptr;
+  const unsigned rb4 = 4u * (unsigned)jmax;
+  const Buf bfr(f.f[src][0] + base, rb4), bfu(f.f[src][1] + base, rb4), bfv(f.f[src][2] + base, rb4);
+  const Buf btrig(f.trig, 2u * rb4), bdy(f.dy, rb4), byv(f.yvol, rb4);
 
   WinP<K> W;
-  int jd[K];
+  unsigned jd[K];   // byte offset of the cell's data column
   C<float, K> dyc;
 #pragma unroll
   for (int h = 0; h < K; ++h) {
     const int c = K * lane + h;
-    const int jref = REAL * l - kNpad + (c < lim ? c : lim - 1);
-    const int j0 = wrapj(jref, jmax);
-    const int j = wrapj(j0 - rot, jmax);
-    jd[h] = j;
-    const float2 tr = f.trig[j0];
+    // lab column, then data column: each needs at most one wrap (rot_j in [0, jmax))
+    int j0 = REAL * l - kNpad + (c < lim ? c : lim - 1);
+    j0 += (j0 < 0) ? jmax : 0;
+    j0 -= (j0 >= jmax) ? jmax : 0;
+    int j = j0 - rot;
+    j += (j < 0) ? jmax : 0;
+    const unsigned o0 = 4u * (unsigned)j0, o = 4u * (unsigned)j;
+    jd[h] = o;
+    const float2 tr = btrig.f2(2u * o0);
     W.ts[h] = tr.x;
     W.tc[h] = tr.y;
-    W.dx0[h] = f.dy[j0];
-    W.dvol0[h] = f.yvol[j0] * rc;
+    W.dx0[h] = bdy.f(o0);
+    W.dvol0[h] = byv.f(o0) * rc;
     W.xa0[h] = 0.0f;
     W.rad[h] = rc;
     W.vrot[h] = vrot;
@@ -398,23 +413,20 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
     W.dLb[h] = 0.0f;
     W.sth[h] = sth;
     W.oms[h] = oms;
-    dyc[h] = f.dy[j];
-    W.r[h] = fr[j];
-    W.u[h] = fv[j];   // v_phi - v_bg
-    W.v[h] = fu[j];   // v_r
-    if constexpr (NDIM == 3) W.w[h] = fw[j];
+    dyc[h] = bdy.f(o);
+    W.r[h] = bfr.f(o);
+    W.u[h] = bfv.f(o);   // v_phi - v_bg
+    W.v[h] = bfu.f(o);   // v_r
+    if constexpr (NDIM == 3) W.w[h] = Buf(f.f[src][3] + base, rb4).f(o);
   }
   const float sqrt_gam = sqrtf((float)prm.gam);
   const Para<float, K> P0 = f.uniform_phi ? paraset_uniform<K>() : paraset(0, lim, W.dx0);
-  ppmlr_pert<K, kGeoPhi, NDIM>(W, TabLane<K>{nullptr, 0, 0}, P0, lim, dt, wres, pl, prm.plnt > 0, prm.flat != 0,
+  ppmlr_pert<K, kGeoPhi, NDIM>(W, TabLane<K>{}, P0, lim, dt, wres, pl, prm.plnt > 0, prm.flat != 0,
                                sqrt_gam, rsph);
   if constexpr (NDIM == 2)
     if (VISC == 1 || (VISC == 2 && prm.visc)) visc_p_pert<K>(W, lim, dt, T[kTabNu * ts], rc);
 
-  float* __restrict__ dr = f.f[src ^ 1][0] + base;
-  float* __restrict__ du = f.f[src ^ 1][1] + base;
-  float* __restrict__ dv = f.f[src ^ 1][2] + base;
-  float* __restrict__ dw = (NDIM == 3) ? f.f[src ^ 1][3] + base : nullptr;
+  const Buf dr(f.f[src ^ 1][0] + base, rb4), du(f.f[src ^ 1][1] + base, rb4), dv(f.f[src ^ 1][2] + base, rb4);
   const float csnd = sqrt_gam * cs;
   const float idx = 1.0f / dxr;
   float m = 0.0f;
@@ -422,11 +434,11 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
   for (int h = 0; h < K; ++h) {
     const int c = K * lane + h;
     if (c >= kNpad && c < lim - kNpad) {
-      const int j = jd[h];
-      dr[j] = W.r[h];
-      du[j] = W.v[h];
-      dv[j] = W.u[h];
-      if constexpr (NDIM == 3) dw[j] = W.w[h];
+      const unsigned o = jd[h];
+      dr.put(W.r[h], o);
+      du.put(W.v[h], o);
+      dv.put(W.u[h], o);
+      if constexpr (NDIM == 3) Buf(f.f[src ^ 1][3] + base, rb4).put(W.w[h], o);
       // the next step's CFL term of this cell (timestep.cu:48-56)
       const float t0 = (fabsf(W.v[h]) + csnd) * idx;
       const float t1 = (fabsf(W.u[h] + cfl_off) + csnd) / (dyc[h] * rc);
@@ -443,7 +455,7 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, D
                                                           const StepScalars* __restrict__ ss, double* cflp, int src,
                                                           int do_cfl) {
   __shared__ float shm[WPB];
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform, in a scalar register
   const int gw = xcd_swizzle(blockIdx.x, gridDim.x) * WPB + wv;
   const int l = gw % f.nwin_y;
   const int n = gw / f.nwin_y;
@@ -466,7 +478,7 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32_batch(const DiskDev* _
                                                               const DevF32* __restrict__ dfs,
                                                               const int* __restrict__ base, int ndisk, int src) {
   __shared__ float shm[WPB];
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform, in a scalar register
   const int nitems = base[ndisk];
   for (int t = blockIdx.x; t < nitems; t += gridDim.x) {
     const int it = xcd_swizzle(t, nitems);

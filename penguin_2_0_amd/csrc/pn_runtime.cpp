@@ -239,6 +239,12 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
   d.nwin_z = nwin(g->kmax);
   const long nslot = static_cast<long>(d.sr) * d.sk;
   const size_t ncell = static_cast<size_t>(nslot) * g->jmax;
+  // the FP32 sweeps address a field buffer with 32-bit byte offsets
+  if (precision == PN_FP32 && ncell * sizeof(float) >= (size_t(1) << 31)) {
+    *out = c;
+    return fail(c, PN_ERR_UNSUPPORTED, "FP32 field buffers of %zu bytes: the FP32 path addresses below 2 GiB",
+                ncell * sizeof(float));
+  }
   int r2 = PN_OK;
   if (precision == PN_FP64) {
     for (int b = 0; b < 2 && r2 == PN_OK; ++b)

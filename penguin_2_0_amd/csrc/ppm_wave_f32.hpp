@@ -31,6 +31,32 @@
 
 namespace pn {
 
+// A raw buffer view of a device array: loads and stores take 32-bit byte offsets, so an
+// access costs no 64-bit address arithmetic; offsets at or past nbytes read 0 and drop
+// stores (the hardware range check).  Word 3 of the descriptor: raw 32-bit data (gfx9).
+// the wide buffer loads return integer vectors: take them whole and bit-cast the vector
+typedef int i32x2_t __attribute__((ext_vector_type(2)));
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+  Buf() = default;
+  __device__ __forceinline__ Buf(const void* p, unsigned nbytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)nbytes, 0x00020000)) {}
+  __device__ __forceinline__ float f(unsigned off) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+  }
+  __device__ __forceinline__ float2 f2(unsigned off) const {
+    const f32x2_t v = __builtin_bit_cast(f32x2_t, (i32x2_t)__builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    return make_float2(v.x, v.y);
+  }
+  __device__ __forceinline__ void put(float v, unsigned off) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
+  }
+};
+
 struct PlanetF {
   float m, x, rs2, inv_x2, vy;
 };
@@ -190,27 +216,32 @@ struct WinP {
 };
 
 // the per-slot table entries of this lane's K cells, entry-major (entry e of slot s at
-// p[e * stride + s]); base = the slot of the lane's first cell
+// float e * stride + s): one vector buffer load per entry, the lane's byte offset in a
+// register and the entry's offset e * stride * 4 in a scalar one
 template <int K>
 struct TabLane {
-  const float* __restrict__ p;
-  int stride, base;
+  Buf b;           // the table
+  unsigned voff;   // byte offset of the slot of this lane's first cell
+  int stride;
   __device__ __forceinline__ C<float, K> get(int e) const {
-    const float* q = p + (long)e * stride + base;
+    const int soff = e * stride * 4;
     C<float, K> o;
     if constexpr (K == 2) {
-      const float2 v = *reinterpret_cast<const float2*>(q);
+      const f32x2_t v =
+          __builtin_bit_cast(f32x2_t, (i32x2_t)__builtin_amdgcn_raw_buffer_load_b64(b.r, (int)voff, soff, 0));
       o[0] = v.x;
       o[1] = v.y;
     } else if constexpr (K == 4) {
-      const float4 v = *reinterpret_cast<const float4*>(q);
+      const f32x4_t v =
+          __builtin_bit_cast(f32x4_t, (i32x4_t)__builtin_amdgcn_raw_buffer_load_b128(b.r, (int)voff, soff, 0));
       o[0] = v.x;
       o[1] = v.y;
       o[2] = v.z;
       o[3] = v.w;
     } else {
 #pragma unroll
-      for (int h = 0; h < K; ++h) o[h] = q[h];
+      for (int h = 0; h < K; ++h)
+        o[h] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)(voff + 4 * h), soff, 0));
     }
     return o;
   }
