@@ -211,8 +211,9 @@ int pn_run_begin(pn_ctx* ctx, pn_run_state* st);
 int pn_run(pn_ctx* ctx, int nsteps, int graph, pn_run_state* st);
 
 /* Time the kernels of one step with HIP events on the context's stream: each of
- * {CFL, Index_Shift, r sweep, phi sweep, theta sweep, roll} is launched nrep times back to
- * back on the current state and its mean duration in milliseconds written to ms[0..5]
+ * {CFL, Index_Shift, r sweep, phi sweep, theta sweep, roll} is launched nrep times untimed
+ * (clocks back to speed after host-side idle), then nrep times back to back between two
+ * events, and its mean duration in milliseconds written to ms[0..5]
  * (0 where a kernel does not run).  No counterpart in the reference (which only prints
  * seconds per step, main.cu:430-437).  The state advances while it is timed. */
 int pn_time_kernels(pn_ctx* ctx, int nrep, double* ms6);
