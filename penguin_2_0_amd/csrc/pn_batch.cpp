@@ -79,7 +79,8 @@ int upload_desc(pn_batch* b) {
     D.cflp = c->cflp;
     D.ncfl_post = ny;
     D.trig = c->df.trig;
-    D.vrot = c->df.tab + static_cast<long>(kTabVrot) * c->df.tstride;
+    D.tab = c->df.tab;
+    D.tstride = c->df.tstride;
     df[d] = c->df;
     df[d].nwin_x = f32_windows(c->g.imax, kF32VariantsX[b->var_x][0]);
     df[d].nwin_y = f32_windows(c->g.jmax, kF32VariantsY[b->var_y][0]);

@@ -42,6 +42,12 @@ enum TabEntry {
   kTabZa0, kTabDz0, kTabDvolZ, kTabSzl, kTabCzl, kTabSzr, kTabCzr,
   kTabDrbZ, kTabDpbZ, kTabDLbZ, kTabDpbmZ,
   kTabResBgZ, kTabAbgZ, kTabFfaceZ, kTabLfZ, kTabGfZ, kTabGcZ, kTabGc1Z, kTabGc2Z,
+  // per step: sin / cos of dt (vrot - v_y) / 4, half the angle the planet's azimuth moves
+  // against the slot's frame in half a step (rewritten with vrot)
+  kTabRotS, kTabRotC,
+  // per step, azimuthal sweep on a uniform grid: the same from the column's left edge,
+  // sin / cos of (dt (vrot - v_y) / 2 - dphi / 2) / 2
+  kTabRotPhS, kTabRotPhC,
   kTabEntries
 };
 constexpr int kTabOverhang = 256;   // >= the longest radial window
@@ -53,7 +59,7 @@ struct DevF32 {
   float* tab;                  // kTabEntries x tstride, entry-major
   int tstride;                 // nslot + kTabOverhang, a multiple of 4
   // 3-D: the same table theta-major (slot (ir, kz) at kz + sk ir, so a theta pencil's slots
-  // are consecutive); stride tstride.  kTabVrot is not kept current here.
+  // are consecutive); stride tstride; its per-step rows are rewritten beside the others
   float* tabz;
   const double *rbg, *vbg;     // Sigma_b and v_bg per slot, FP64
   const float *dy, *yvol;      // per column

@@ -1,5 +1,6 @@
 // pn_kernels_common.hip -- the small per-step kernels: FARGO split and roll, CFL max,
 // reductions, device-side planet update.  FP64 throughout (-ffp-contract=off).
+#include "pn_f32.hpp"
 #include "pn_launch.hpp"
 #include "ppm_wave.hpp"
 
@@ -359,14 +360,40 @@ __device__ void index_shift_ring(DevGrid& g, const DevParams& prm, int n, double
   g.rot_v[s] = rot_v;
 }
 
+// the per-step rotation rows of the FP32 table, every slot (ghost slots keep res_v = 0):
+// sin / cos of dt (res_v / r_cyl - v_y) / 4 with the planet velocity the sweeps will read
+// (3-D: also into the theta-major copy tabz, where slot (ir, kz) sits at kz + sk ir)
+__device__ __forceinline__ void write_rot_rows(const DevGrid& g, const StepScalars* ss, float* tab, float* tabz,
+                                               int tstride, double dt) {
+  const double vy = ss->planet.vy;
+  const int nslot = g.sr * g.sk;
+  // one workgroup per disk (the batched pre-step runs one per disk of the grid)
+  for (int s = threadIdx.x; s < nslot; s += blockDim.x) {
+    const double h = 0.25 * dt * (g.res_v[s] / g.radc[s] - vy);
+    const float sh = (float)sin(h), ch = (float)cos(h);
+    tab[(long)kTabRotS * tstride + s] = sh;
+    tab[(long)kTabRotC * tstride + s] = ch;
+    const double hp = h - 0.25 * g.dy[0];
+    tab[(long)kTabRotPhS * tstride + s] = (float)sin(hp);
+    tab[(long)kTabRotPhC * tstride + s] = (float)cos(hp);
+    if (tabz) {
+      const long sz = s / g.sr + (long)g.sk * (s % g.sr);
+      tabz[(long)kTabRotS * tstride + sz] = sh;
+      tabz[(long)kTabRotC * tstride + sz] = ch;
+      tabz[(long)kTabVrot * tstride + sz] = tab[(long)kTabVrot * tstride + s];
+    }
+  }
+}
+
 // Everything of a step before the sweeps, in one workgroup (main.cu:345-370): dt from ncfl
 // CFL partials (ncfl 0: ss->dt is already this step's), Index_Shift of every ring, and the
 // planet's opening half kick.  For FP32 contexts also the per-step tables: sin / cos of half
-// the angle from the planet to each column centre (trig) and vrot = res_v / r_cyl per slot
-// (vrot_f[s * vrot_stride]).
+// the angle from the planet to each column centre (trig), and per slot of the table tab
+// vrot = res_v / r_cyl and the planet's azimuthal drift against the slot in half a step
+// (after the kick: the sweeps see that planet velocity).
 __device__ __forceinline__ void pre_step_body(DevGrid g, const DevParams& prm, StepScalars* ss,
-                                              const double* __restrict__ cflp, int ncfl, float2* trig, float* vrot_f,
-                                              int vrot_stride) {
+                                              const double* __restrict__ cflp, int ncfl, float2* trig, float* tab,
+                                              float* tabz, int tstride) {
   __shared__ double sh[kStepThreads / 64];
   __shared__ double s_dt, s_fr, s_py;
   __shared__ int s_trig;
@@ -390,9 +417,9 @@ __device__ __forceinline__ void pre_step_body(DevGrid g, const DevParams& prm, S
   const int nring = g.imax * g.kmax;
   for (int n = threadIdx.x; n < nring; n += blockDim.x) {
     index_shift_ring(g, prm, n, dt, FrRot);
-    if (vrot_f) {
+    if (tab) {
       const int s = slot_of(g, n % g.imax, n / g.imax);
-      vrot_f[(long)s * vrot_stride] = (float)(g.res_v[s] / g.radc[s]);
+      tab[(long)kTabVrot * tstride + s] = (float)(g.res_v[s] / g.radc[s]);
     }
   }
   if (s_trig) {
@@ -411,18 +438,22 @@ __device__ __forceinline__ void pre_step_body(DevGrid g, const DevParams& prm, S
     ss->FrRot = (prm.frrot == 1) ? b.vy : 0.0;
     b.vy -= ss->FrRot;
   }
+  if (tab) {
+    __syncthreads();
+    write_rot_rows(g, ss, tab, tabz, tstride, dt);
+  }
 }
 
 __global__ __launch_bounds__(kStepThreads) void k_pre_step(DevGrid g, DevParams prm, StepScalars* ss,
                                                            const double* __restrict__ cflp, int ncfl, float2* trig,
-                                                           float* vrot_f, int vrot_stride) {
-  pre_step_body(g, prm, ss, cflp, ncfl, trig, vrot_f, vrot_stride);
+                                                           float* tab, float* tabz, int tstride) {
+  pre_step_body(g, prm, ss, cflp, ncfl, trig, tab, tabz, tstride);
 }
 
 // one workgroup per disk of a batch (the fused CFL leaves dt already set: ncfl 0)
 __global__ __launch_bounds__(kStepThreads) void k_pre_step_batch(const DiskDev* __restrict__ dd) {
   const DiskDev& D = dd[blockIdx.x];
-  pre_step_body(D.g, D.p, D.ss, D.cflp, 0, D.trig, D.vrot, 1);
+  pre_step_body(D.g, D.p, D.ss, D.cflp, 0, D.trig, D.tab, nullptr, D.tstride);
 }
 
 // Everything of a step after the sweeps for a planet that does not feel the disk
@@ -537,9 +568,9 @@ hipError_t launch_totals(const LaunchCtx& c, int src, double* out2) {
   return hipGetLastError();
 }
 
-hipError_t launch_pre_step(const LaunchCtx& c, int ncfl, float2* trig, float* vrot_f, int vrot_stride) {
+hipError_t launch_pre_step(const LaunchCtx& c, int ncfl, float2* trig, float* tab, float* tabz, int tstride) {
   if (ncfl < 0 || ncfl > c.ncflp) return hipErrorInvalidValue;
-  k_pre_step<<<1, kStepThreads, 0, c.stream>>>(c.g, c.p, c.ss, c.cflp, ncfl, trig, vrot_f, vrot_stride);
+  k_pre_step<<<1, kStepThreads, 0, c.stream>>>(c.g, c.p, c.ss, c.cflp, ncfl, trig, tab, tabz, tstride);
   return hipGetLastError();
 }
 

@@ -53,14 +53,27 @@ __device__ __forceinline__ float wave_maxf(float m) {
 }
 
 // per step: sin / cos of half the angle from the planet to each lab column centre (FP64),
-// and each slot's vrot = res_v / r after Index_Shift
+// each slot's vrot = res_v / r after Index_Shift and its half-step rotation rows
 __global__ void k_prep(DevGrid g, DevF32 f, const StepScalars* __restrict__ ss, int nslot) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < g.jmax) {
     const double a = 0.5 * (g.yc[t] - ss->planet.y);
     f.trig[t] = make_float2((float)sin(a), (float)cos(a));
   }
-  if (t < nslot) f.tab[(long)kTabVrot * f.tstride + t] = (float)(g.res_v[t] / g.radc[t]);
+  if (t < nslot) {
+    f.tab[(long)kTabVrot * f.tstride + t] = (float)(g.res_v[t] / g.radc[t]);
+    const double h = 0.25 * ss->dt * (g.res_v[t] / g.radc[t] - ss->planet.vy);
+    f.tab[(long)kTabRotS * f.tstride + t] = (float)sin(h);
+    f.tab[(long)kTabRotC * f.tstride + t] = (float)cos(h);
+    f.tab[(long)kTabRotPhS * f.tstride + t] = (float)sin(h - 0.25 * g.dy[0]);
+    f.tab[(long)kTabRotPhC * f.tstride + t] = (float)cos(h - 0.25 * g.dy[0]);
+    if (f.tabz) {   // the theta-major copy: slot (ir, kz) at kz + sk ir
+      const long sz = t / g.sr + (long)g.sk * (t % g.sr);
+      f.tabz[(long)kTabVrot * f.tstride + sz] = f.tab[(long)kTabVrot * f.tstride + t];
+      f.tabz[(long)kTabRotS * f.tstride + sz] = (float)sin(h);
+      f.tabz[(long)kTabRotC * f.tstride + sz] = (float)cos(h);
+    }
+  }
 }
 
 // K consecutive floats from LDS (vector read; p is K-float aligned)
@@ -175,12 +188,9 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
     Wp.cs2 = tx.get(kTabCs2);
     Wp.cs = tx.get(kTabCs);
     Wp.Lb = tx.get(kTabLb);
-    if constexpr (TH) {
-      // vrot changes every step and is kept current in the slot-major table only
-      Wp.vrot = TabLaneS<K>{f.tab, f.tstride, s0, g.sr, K * lane, lim - 1}.get(kTabVrot);
-    } else {
-      Wp.vrot = tx.get(kTabVrot);
-    }
+    Wp.rs = tx.get(kTabRotS);
+    Wp.rcs = tx.get(kTabRotC);
+
     if constexpr (G != kGeoRcyl) {
       Wp.sth = tx.get(kTabSth);
       Wp.oms = tx.get(kTabOms);
@@ -368,7 +378,8 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
   const float* __restrict__ T = f.tab + s;
   const long ts = f.tstride;
   const float rc = T[kTabRc * ts], dxr = T[kTabDx0 * ts], rb = T[kTabRb * ts], pb = T[kTabPb * ts],
-              cs2 = T[kTabCs2 * ts], cs = T[kTabCs * ts], vrot = T[kTabVrot * ts];
+              cs2 = T[kTabCs2 * ts], cs = T[kTabCs * ts], vrot = T[kTabVrot * ts], rps = T[kTabRotPhS * ts],
+              rpc = T[kTabRotPhC * ts];
   float rsph = 0.0f, sth = 1.0f, oms = 0.0f;
   if constexpr (NDIM == 3) {
     rsph = T[kTabRad * ts];
@@ -403,6 +414,8 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
     W.xa0[h] = 0.0f;
     W.rad[h] = rc;
     W.vrot[h] = vrot;
+    W.rs[h] = rps;
+    W.rcs[h] = rpc;
     W.rb[h] = rb;
     W.drb[h] = 0.0f;
     W.pb[h] = pb;
@@ -422,7 +435,7 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
   const float sqrt_gam = sqrtf((float)prm.gam);
   const Para<float, K> P0 = f.uniform_phi ? paraset_uniform<K>() : paraset(0, lim, W.dx0);
   ppmlr_pert<K, kGeoPhi, NDIM>(W, TabLane<K>{}, P0, lim, dt, wres, pl, prm.plnt > 0, prm.flat != 0,
-                               sqrt_gam, rsph);
+                               sqrt_gam, rsph, f.uniform_phi != 0);
   if constexpr (NDIM == 2)
     if (VISC == 1 || (VISC == 2 && prm.visc)) visc_p_pert<K>(W, lim, dt, T[kTabNu * ts], rc);
 

@@ -27,7 +27,8 @@ struct DiskDev {
   double* cflp;      // this disk's CFL partials
   int ncfl_post;     // partials the post-step reduces into the next dt (fused CFL), or 0
   float2* trig;      // FP32 per-step tables, or null
-  float* vrot;       // the FP32 table's vrot row (stride 1), or null
+  float* tab;        // the FP32 per-slot table (per-step rows rewritten by the pre-step), or null
+  int tstride;
 };
 
 // FP64 path (pn_kernels_fp64.hip); each returns hipSuccess or the launch error
@@ -54,8 +55,9 @@ hipError_t launch_copy(hipStream_t s, const void* a, void* b, size_t nbytes);
 // CFL maxima per block into cflp (no dt); *n receives the partial count
 hipError_t launch_cfl_partials(const LaunchCtx& c, int src, int* n);
 // one workgroup before the sweeps: dt from ncfl partials (0: keep ss->dt), Index_Shift,
-// opening half kick; trig / vrot_f (FP32 tables) may be null
-hipError_t launch_pre_step(const LaunchCtx& c, int ncfl, float2* trig, float* vrot_f, int vrot_stride);
+// opening half kick; trig / tab / tabz (FP32 tables: rows of stride tstride, tabz the 3-D
+// theta-major copy) may be null
+hipError_t launch_pre_step(const LaunchCtx& c, int ncfl, float2* trig, float* tab, float* tabz, int tstride);
 // one workgroup after the sweeps (plnt_flag < 2): roll, time and planet update, then dt
 // from ncfl partials when ncfl > 0
 hipError_t launch_post_step(const LaunchCtx& c, int ncfl);

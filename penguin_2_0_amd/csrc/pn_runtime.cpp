@@ -189,8 +189,9 @@ int launch_step(pn_ctx* c, int& cur) {
     else PN_LAUNCH(c, "cfl", launch_cfl_partials(L, cur, &ncfl));
   }
   float2* trig = is_f32(c) ? c->df.trig : nullptr;
-  float* vrot = is_f32(c) ? c->df.tab + static_cast<long>(kTabVrot) * c->df.tstride : nullptr;
-  PN_LAUNCH(c, "pre step", launch_pre_step(L, ncfl, trig, vrot, 1));
+  float* tab = is_f32(c) ? c->df.tab : nullptr;
+  float* tabz = is_f32(c) ? c->df.tabz : nullptr;
+  PN_LAUNCH(c, "pre step", launch_pre_step(L, ncfl, trig, tab, tabz, c->df.tstride));
   const bool post = c->P.plnt_flag < 2;
   if ((rc = launch_sweeps(c, cur, fused ? 1 : 0, false, !post)) != PN_OK) return rc;
   // damping zones: ring-uniform relaxation, which commutes with the integer roll
@@ -523,6 +524,9 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
       q[kTabLf] = Lf;
       q[kTabIxa3] = 1.0 / (xa0 * xa0 * xa0) / (st_ * st_);
       q[kTabVrot] = sres_v[s] / sradc[s];
+      q[kTabRotC] = 1.0;   // no drift until the first step's pre-step rewrites the rotation rows
+      q[kTabRotPhS] = std::sin(-0.25 * c->h_dy[0]);
+      q[kTabRotPhC] = std::cos(-0.25 * c->h_dy[0]);
       q[kTabDvol0] = sxvol[s];
       q[kTabRc] = sradc[s];
       q[kTabIrc] = 1.0 / sradc[s];

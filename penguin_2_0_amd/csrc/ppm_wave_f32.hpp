@@ -211,8 +211,10 @@ struct WinP {
   // backgrounds (azimuthal sweep: uniform in the window, differences zero); dLb is the
   // difference of L_b along the sweep
   C<float, K> rb, drb, pb, dpb, Lb, dLb, cs, cs2;
-  // geometry; sth / oms: sin theta and 1 - sin theta of the cell centre (3-D planet terms)
-  C<float, K> rad, xa0, dx0, dvol0, vrot, ts, tc, sth, oms;
+  // geometry; sth / oms: sin theta and 1 - sin theta of the cell centre (3-D planet terms);
+  // vrot (azimuthal sweep) or rs / rcs (radial and theta sweeps: sin / cos of the planet's
+  // drift against the slot in half a step, halved) for the planet's azimuth over the step
+  C<float, K> rad, xa0, dx0, dvol0, vrot, ts, tc, sth, oms, rs, rcs;
 };
 
 // the per-slot table entries of this lane's K cells, entry-major (entry e of slot s at
@@ -265,6 +267,16 @@ struct TabLaneS {
   }
 };
 
+// the half angle (s, c) turned by the slot's half-step rotation (rs, rc), and by twice that
+// (a full step)
+__device__ __forceinline__ void rot_by(float s, float c, float rs, float rc, float& so, float& co) {
+  so = s * rc + c * rs;
+  co = c * rc - s * rs;
+}
+__device__ __forceinline__ void rot_by2(float s, float c, float rs, float rc, float& so, float& co) {
+  rot_by(s, c, 2.0f * rs * rc, 1.0f - 2.0f * rs * rs, so, co);
+}
+
 // cos(a + d) - cos(a) from sin a, cos a and a small d (|d| < ~0.3), without cancellation
 __device__ __forceinline__ float dcos(float sa, float ca, float d) {
   const float d2 = d * d;
@@ -285,7 +297,8 @@ __device__ __forceinline__ void rot_theta(float st, float ct, float oms, float e
 }
 
 // The window update.  G: sweep geometry; NDIM: 2 or 3 (the azimuthal sweep carries w and uses
-// the spherical planet terms in 3-D).  tx: the per-slot table of this lane's cells (TabLane
+// the spherical planet terms in 3-D); phi_rows: the azimuthal sweep's W.rs / W.rcs hold the
+// rotation from a column's left edge over half a step (uniform azimuthal grid).  tx: the per-slot table of this lane's cells (TabLane
 // or TabLaneS; radial and theta sweeps read their balance terms where they are used).  P0: paraset of the
 // Eulerian grid (its values outside the window's paraset ranges never reach a parabola the
 // sweep keeps).  wres: the azimuthal grid's uniform angular rate.  rsph: the spherical
@@ -293,7 +306,7 @@ __device__ __forceinline__ void rot_theta(float st, float ct, float oms, float e
 template <int K, int G, int NDIM, class Tab>
 __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para<float, K>& P0, int nmax,
                                            float dt, float wres, const PlanetF& pl, bool planet, bool flat_on,
-                                           float sqrt_gam, float rsph = 0.0f) {
+                                           float sqrt_gam, float rsph = 0.0f, bool phi_rows = false) {
   constexpr bool RAD = (G == kGeoRcyl || G == kGeoRsph);
   const float hdt = 0.5f * dt;
   const float fourthd = 1.3333333333333333f;
@@ -352,7 +365,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
       }
       if (planet) {
         float s, c;
-        rot_half(W.ts[h], W.tc[h], hdt * (W.vrot[h] - pl.vy), s, c);
+        rot_by(W.ts[h], W.tc[h], W.rs[h], W.rcs[h], s, c);
         if constexpr (G == kGeoRcyl) {
           f += planet_fr(pl, W.xa0[h], s, c);
         } else {
@@ -368,7 +381,10 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
       float f = 0.0f;
       if (planet) {
         float s, c;
-        rot_half(W.ts[h], W.tc[h], -0.5f * W.dx0[h] + hdt * (W.vrot[h] - pl.vy), s, c);
+        // from the left edge of the column to the planet half a step on: the ring's own
+        // rotation rows on a uniform grid, else the series
+        if (phi_rows) rot_by(W.ts[h], W.tc[h], W.rs[h], W.rcs[h], s, c);
+        else rot_half(W.ts[h], W.tc[h], -0.5f * W.dx0[h] + hdt * (W.vrot[h] - pl.vy), s, c);
         if constexpr (NDIM == 2) f = planet_fphi(pl, W.rad[h], s, c);
         else f = planet_fphi_sph(pl, rsph, W.sth[h], W.oms[h], s, c);
       }
@@ -389,7 +405,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
       float f = ffb[h] + dlf * (2.0f * Lf[h] + dlf) * Gf[h];
       if (planet) {
         float s, c;
-        rot_half(W.ts[h], W.tc[h], hdt * (W.vrot[h] - pl.vy), s, c);
+        rot_by(W.ts[h], W.tc[h], W.rs[h], W.rcs[h], s, c);
         f += planet_fth_sph(pl, W.rad[h], szl[h], czl[h], 1.0f - szl[h], s, c);
       }
       fhalf[h] = hdt * f;
@@ -545,7 +561,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
         float fdiff = -(ixc * ixc) * g2 + (B + q1) * g3;
         if (planet) {
           float s, cc;
-          rot_half(W.ts[h], W.tc[h], dt * (W.vrot[h] - pl.vy), s, cc);
+          rot_by2(W.ts[h], W.tc[h], W.rs[h], W.rcs[h], s, cc);
           if constexpr (G == kGeoRcyl) {
             fdiff += planet_fr(pl, W.rad[h], W.ts[h], W.tc[h]) + planet_fr(pl, W.rad[h] + dxc, s, cc);
           } else {
@@ -639,7 +655,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
         float fdiff = L2 * e * (2.0f * G1[h] + e * G2[h]);
         if (planet) {
           float s, cc, so, co, omso;
-          rot_half(W.ts[h], W.tc[h], dt * (W.vrot[h] - pl.vy), s, cc);
+          rot_by2(W.ts[h], W.tc[h], W.rs[h], W.rcs[h], s, cc);
           rot_theta(W.sth[h], cth[h], W.oms[h], e, so, co, omso);
           fdiff += 2.0f * (planet_fth_sph(pl, W.rad[h], W.sth[h], cth[h], W.oms[h], W.ts[h], W.tc[h]) +
                            planet_fth_sph(pl, W.rad[h], so, co, omso, s, cc));
