@@ -21,11 +21,13 @@ namespace pn {
 // ---------------------------------------------------------------------------------------
 // lane exchange
 
+// wave_shr:1 / wave_shl:1 with bound_ctrl: the lane without a source (0, resp. 63) reads 0,
+// and the destination needs no initial value (one instruction per shift)
 __device__ __forceinline__ int lane_prev_i(int x) {   // lane l receives lane l-1's value
-  return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false);
+  return __builtin_amdgcn_mov_dpp(x, 0x138, 0xf, 0xf, true);
 }
 __device__ __forceinline__ int lane_next_i(int x) {   // lane l receives lane l+1's value
-  return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false);
+  return __builtin_amdgcn_mov_dpp(x, 0x130, 0xf, 0xf, true);
 }
 __device__ __forceinline__ float lane_prev(float x) { return __int_as_float(lane_prev_i(__float_as_int(x))); }
 __device__ __forceinline__ float lane_next(float x) { return __int_as_float(lane_next_i(__float_as_int(x))); }

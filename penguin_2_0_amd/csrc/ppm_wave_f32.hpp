@@ -323,24 +323,22 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
     C<float, K> steep;
 #pragma unroll
     for (int h = 0; h < K; ++h) {
+      // evaluated in every lane and selected (no divergent branch around it)
       const int c = cell_of(K, h);
-      float s = 0.0f;
-      if (c >= 2 && c < nmax - 2) {
-        const float dp1 = pp1[h] - pm1[h];
-        float dp2 = pp2[h] - pm2[h];
-        if (fabsf(dp2) < 1.0e-15f) dp2 = 1.0e-15f;
-        const float ddp = fabsf(dp1) / fminf(pp1[h], pm1[h]);
-        bool shock = ddp > 0.33f;
-        if (um1[h] < up1[h]) shock = false;
-        if (shock) s = fmaxf(0.0f, (dp1 / dp2 - 0.75f) * 5.0f);
-      }
-      steep[h] = s;
+      const float dp1 = pp1[h] - pm1[h];
+      float dp2 = pp2[h] - pm2[h];
+      dp2 = (fabsf(dp2) < 1.0e-15f) ? 1.0e-15f : dp2;
+      const float ddp = fabsf(dp1) / fminf(pp1[h], pm1[h]);
+      const bool shock = (c >= 2 && c < nmax - 2) && ddp > 0.33f && !(um1[h] < up1[h]);
+      const float st = fmaxf(0.0f, (dp1 / dp2 - 0.75f) * 5.0f);
+      steep[h] = shock ? st : 0.0f;
     }
     C<float, K> sm = prev1(steep), sp = next1(steep);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
       const int c = cell_of(K, h);
-      flat[h] = (c >= 2 && c < nmax - 2) ? fminf(0.5f, max3(sm[h], steep[h], sp[h])) : 0.0f;
+      const float fl = fminf(0.5f, fmaxf(sm[h], fmaxf(steep[h], sp[h])));
+      flat[h] = (c >= 2 && c < nmax - 2) ? fl : 0.0f;
     }
   }
 
