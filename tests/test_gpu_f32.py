@@ -249,3 +249,30 @@ def test_f32_3d_every_variant_and_graph():
     e, _ = run(w, abi.PN_FP32, 6, variants=(0, 3), graph=False)
     g, _ = run(w, abi.PN_FP32, 6, variants=(0, 3), graph=True)
     assert np.array_equal(e.view(np.uint8), g.view(np.uint8))
+
+
+def flags_case(suffix, **over):
+    w = configs.small(96, 288, **over)
+    w.name += suffix
+    return w
+
+
+# every flag combination the FP32 path accepts, against the FP64 path on the same state
+FLAG_CASES = [flags_case("_fargo2", fargo_flag=2), flags_case("_inertial", frrot_flag=0),
+              flags_case("_noflat", flat_flag=0), flags_case("_noplanet", plnt_flag=0),
+              flags_case("_nofargo", fargo_flag=0), flags_case("_growing_planet", t_growth=5.0),
+              flags_case("_strong_planet", M_p=1e-3)]
+
+
+@pytest.mark.parametrize("w", FLAG_CASES, ids=lambda w: w.name)
+def test_f32_flags_vs_fp64(w):
+    n = 50
+    say("case: fp32 flags", w.name, n, "steps")
+    a, sa = run(w, abi.PN_FP32, n)
+    b, sb = run(w, abi.PN_FP64, n)
+    assert np.array_equal(a["rot_j"], b["rot_j"])
+    assert abs(sa.dt - sb.dt) <= 1e-5 * sb.dt
+    for f, t1, tinf in (("r", 1e-6, 1e-4), ("v", 1e-6, 1e-5)):
+        l1, li = rel(lab(a, f), lab(b, f))
+        say(f"  {f}: L1 {l1:.3e} Linf {li:.3e}")
+        assert l1 <= t1 and li <= tinf, (f, l1, li)
