@@ -65,6 +65,7 @@ struct StepScalars {
   int bad;            // non-finite dt seen
   int trig_ok;        // FP32 planet trig table holds the planet azimuth trig_y
   double trig_y;
+  double fr_shift;    // FrRot before this step's opening half kick (what Index_Shift reads)
 };
 
 __host__ __device__ inline int slot_of(const DevGrid& g, int i, int k) {

@@ -321,6 +321,10 @@ __global__ void k_planet_forces_kick(DevParams prm, StepScalars* ss, int do_kick
 }
 
 constexpr int kStepThreads = 1024;
+// a disk with more slots than this runs the per-ring work of the pre- and post-step over the
+// whole grid (k_pre_rings, k_rotate) rather than in their single workgroup
+constexpr int kSplitSlots = 2 * kStepThreads;
+inline bool split_rings(const DevGrid& g) { return g.sr * g.sk > kSplitSlots; }
 
 // max over n partials by the whole (single) workgroup; the result is valid in thread 0
 __device__ double block_max_partials(const double* __restrict__ cflp, int n, double* sh) {
@@ -360,29 +364,36 @@ __device__ void index_shift_ring(DevGrid& g, const DevParams& prm, int n, double
   g.rot_v[s] = rot_v;
 }
 
-// the per-step rotation rows of the FP32 table, every slot (ghost slots keep res_v = 0):
+// the per-step rotation rows of the FP32 table for slot s (ghost slots keep res_v = 0):
 // sin / cos of dt (res_v / r_cyl - v_y) / 4 with the planet velocity the sweeps will read
 // (3-D: also into the theta-major copy tabz, where slot (ir, kz) sits at kz + sk ir)
-__device__ __forceinline__ void write_rot_rows(const DevGrid& g, const StepScalars* ss, float* tab, float* tabz,
-                                               int tstride, double dt) {
-  const double vy = ss->planet.vy;
-  const int nslot = g.sr * g.sk;
-  // one workgroup per disk (the batched pre-step runs one per disk of the grid)
-  for (int s = threadIdx.x; s < nslot; s += blockDim.x) {
-    const double h = 0.25 * dt * (g.res_v[s] / g.radc[s] - vy);
-    const float sh = (float)sin(h), ch = (float)cos(h);
-    tab[(long)kTabRotS * tstride + s] = sh;
-    tab[(long)kTabRotC * tstride + s] = ch;
-    const double hp = h - 0.25 * g.dy[0];
-    tab[(long)kTabRotPhS * tstride + s] = (float)sin(hp);
-    tab[(long)kTabRotPhC * tstride + s] = (float)cos(hp);
-    if (tabz) {
-      const long sz = s / g.sr + (long)g.sk * (s % g.sr);
-      tabz[(long)kTabRotS * tstride + sz] = sh;
-      tabz[(long)kTabRotC * tstride + sz] = ch;
-      tabz[(long)kTabVrot * tstride + sz] = tab[(long)kTabVrot * tstride + s];
-    }
+__device__ __forceinline__ void rot_rows_slot(const DevGrid& g, double vy, float* tab, float* tabz, int tstride,
+                                              double dt, int s) {
+  const double h = 0.25 * dt * (g.res_v[s] / g.radc[s] - vy);
+  const float sh = (float)sin(h), ch = (float)cos(h);
+  tab[(long)kTabRotS * tstride + s] = sh;
+  tab[(long)kTabRotC * tstride + s] = ch;
+  const double hp = h - 0.25 * g.dy[0];
+  tab[(long)kTabRotPhS * tstride + s] = (float)sin(hp);
+  tab[(long)kTabRotPhC * tstride + s] = (float)cos(hp);
+  if (tabz) {
+    const long sz = s / g.sr + (long)g.sk * (s % g.sr);
+    tabz[(long)kTabRotS * tstride + sz] = sh;
+    tabz[(long)kTabRotC * tstride + sz] = ch;
+    tabz[(long)kTabVrot * tstride + sz] = tab[(long)kTabVrot * tstride + s];
   }
+}
+
+// Index_Shift of the ring in slot s (if s holds one) with its vrot row, then the slot's
+// rotation rows: everything per slot reads and writes that slot only
+__device__ __forceinline__ void ring_slot(DevGrid& g, const DevParams& prm, double dt, double fr, double vy,
+                                          float* tab, float* tabz, int tstride, int s) {
+  const int i = s % g.sr - kNpad, k = s / g.sr - g.k0;
+  if (i >= 0 && i < g.imax && k >= 0 && k < g.kmax) {
+    index_shift_ring(g, prm, i + g.imax * k, dt, fr);
+    if (tab) tab[(long)kTabVrot * tstride + s] = (float)(g.res_v[s] / g.radc[s]);
+  }
+  if (tab) rot_rows_slot(g, vy, tab, tabz, tstride, dt, s);
 }
 
 // Everything of a step before the sweeps, in one workgroup (main.cu:345-370): dt from ncfl
@@ -390,10 +401,12 @@ __device__ __forceinline__ void write_rot_rows(const DevGrid& g, const StepScala
 // planet's opening half kick.  For FP32 contexts also the per-step tables: sin / cos of half
 // the angle from the planet to each column centre (trig), and per slot of the table tab
 // vrot = res_v / r_cyl and the planet's azimuthal drift against the slot in half a step
-// (after the kick: the sweeps see that planet velocity).
+// (after the kick: the sweeps see that planet velocity).  With rings false the per-slot part
+// (Index_Shift, vrot and the rotation rows) is left to k_pre_rings, which reads the FrRot
+// Index_Shift needs from ss->fr_shift.
 __device__ __forceinline__ void pre_step_body(DevGrid g, const DevParams& prm, StepScalars* ss,
                                               const double* __restrict__ cflp, int ncfl, float2* trig, float* tab,
-                                              float* tabz, int tstride) {
+                                              float* tabz, int tstride, bool rings) {
   __shared__ double sh[kStepThreads / 64];
   __shared__ double s_dt, s_fr, s_py;
   __shared__ int s_trig;
@@ -404,6 +417,7 @@ __device__ __forceinline__ void pre_step_body(DevGrid g, const DevParams& prm, S
   if (threadIdx.x == 0) {
     s_dt = ss->dt;
     s_fr = ss->FrRot;
+    ss->fr_shift = s_fr;
     s_py = ss->planet.y;   // the half kick below changes velocities only
     // the trig table depends on the planet's azimuth alone: refresh it when that moved
     s_trig = (trig != nullptr) && !(ss->trig_ok && ss->trig_y == s_py);
@@ -414,12 +428,14 @@ __device__ __forceinline__ void pre_step_body(DevGrid g, const DevParams& prm, S
   }
   __syncthreads();
   const double dt = s_dt, FrRot = s_fr;
-  const int nring = g.imax * g.kmax;
-  for (int n = threadIdx.x; n < nring; n += blockDim.x) {
-    index_shift_ring(g, prm, n, dt, FrRot);
-    if (tab) {
-      const int s = slot_of(g, n % g.imax, n / g.imax);
-      tab[(long)kTabVrot * tstride + s] = (float)(g.res_v[s] / g.radc[s]);
+  if (rings) {
+    const int nring = g.imax * g.kmax;
+    for (int n = threadIdx.x; n < nring; n += blockDim.x) {
+      index_shift_ring(g, prm, n, dt, FrRot);
+      if (tab) {
+        const int s = slot_of(g, n % g.imax, n / g.imax);
+        tab[(long)kTabVrot * tstride + s] = (float)(g.res_v[s] / g.radc[s]);
+      }
     }
   }
   if (s_trig) {
@@ -438,22 +454,33 @@ __device__ __forceinline__ void pre_step_body(DevGrid g, const DevParams& prm, S
     ss->FrRot = (prm.frrot == 1) ? b.vy : 0.0;
     b.vy -= ss->FrRot;
   }
-  if (tab) {
+  if (tab && rings) {
     __syncthreads();
-    write_rot_rows(g, ss, tab, tabz, tstride, dt);
+    const double vy = ss->planet.vy;
+    // one workgroup per disk (the batched pre-step runs one per disk of the grid)
+    for (int s = threadIdx.x; s < g.sr * g.sk; s += blockDim.x) rot_rows_slot(g, vy, tab, tabz, tstride, dt, s);
   }
 }
 
 __global__ __launch_bounds__(kStepThreads) void k_pre_step(DevGrid g, DevParams prm, StepScalars* ss,
                                                            const double* __restrict__ cflp, int ncfl, float2* trig,
-                                                           float* tab, float* tabz, int tstride) {
-  pre_step_body(g, prm, ss, cflp, ncfl, trig, tab, tabz, tstride);
+                                                           float* tab, float* tabz, int tstride, int rings) {
+  pre_step_body(g, prm, ss, cflp, ncfl, trig, tab, tabz, tstride, rings != 0);
+}
+
+// the per-slot part of the pre-step over the whole grid, one thread per slot, after
+// k_pre_step with rings false (it has set dt, ss->fr_shift and the kicked planet velocity)
+__global__ __launch_bounds__(256) void k_pre_rings(DevGrid g, DevParams prm, const StepScalars* ss, float* tab,
+                                                   float* tabz, int tstride) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= g.sr * g.sk) return;
+  ring_slot(g, prm, ss->dt, ss->fr_shift, ss->planet.vy, tab, tabz, tstride, s);
 }
 
 // one workgroup per disk of a batch (the fused CFL leaves dt already set: ncfl 0)
 __global__ __launch_bounds__(kStepThreads) void k_pre_step_batch(const DiskDev* __restrict__ dd) {
   const DiskDev& D = dd[blockIdx.x];
-  pre_step_body(D.g, D.p, D.ss, D.cflp, 0, D.trig, D.tab, nullptr, D.tstride);
+  pre_step_body(D.g, D.p, D.ss, D.cflp, 0, D.trig, D.tab, nullptr, D.tstride, true);
 }
 
 // Everything of a step after the sweeps for a planet that does not feel the disk
@@ -461,9 +488,9 @@ __global__ __launch_bounds__(kStepThreads) void k_pre_step_batch(const DiskDev* 
 // advance, drift, forces and closing half kick (main.cu:378-387, planet.cu:54-93), then,
 // with ncfl > 0, the next step's dt from the CFL partials the last sweep wrote.
 __device__ __forceinline__ void post_step_body(DevGrid g, const DevParams& prm, StepScalars* ss,
-                                               const double* __restrict__ cflp, int ncfl) {
+                                               const double* __restrict__ cflp, int ncfl, bool roll) {
   __shared__ double sh[kStepThreads / 64];
-  if (prm.fargo > 0) {
+  if (roll && prm.fargo > 0) {
     const int nring = g.imax * g.kmax;
     for (int n = threadIdx.x; n < nring; n += blockDim.x) {
       const int s = slot_of(g, n % g.imax, n / g.imax);
@@ -501,13 +528,13 @@ __device__ __forceinline__ void post_step_body(DevGrid g, const DevParams& prm, 
 }
 
 __global__ __launch_bounds__(kStepThreads) void k_post_step(DevGrid g, DevParams prm, StepScalars* ss,
-                                                            const double* __restrict__ cflp, int ncfl) {
-  post_step_body(g, prm, ss, cflp, ncfl);
+                                                            const double* __restrict__ cflp, int ncfl, int roll) {
+  post_step_body(g, prm, ss, cflp, ncfl, roll != 0);
 }
 
 __global__ __launch_bounds__(kStepThreads) void k_post_step_batch(const DiskDev* __restrict__ dd) {
   const DiskDev& D = dd[blockIdx.x];
-  post_step_body(D.g, D.p, D.ss, D.cflp, D.ncfl_post);
+  post_step_body(D.g, D.p, D.ss, D.cflp, D.ncfl_post, true);
 }
 
 int red_blocks(const DevGrid& g) {
@@ -570,13 +597,23 @@ hipError_t launch_totals(const LaunchCtx& c, int src, double* out2) {
 
 hipError_t launch_pre_step(const LaunchCtx& c, int ncfl, float2* trig, float* tab, float* tabz, int tstride) {
   if (ncfl < 0 || ncfl > c.ncflp) return hipErrorInvalidValue;
-  k_pre_step<<<1, kStepThreads, 0, c.stream>>>(c.g, c.p, c.ss, c.cflp, ncfl, trig, tab, tabz, tstride);
+  const bool split = split_rings(c.g);
+  k_pre_step<<<1, kStepThreads, 0, c.stream>>>(c.g, c.p, c.ss, c.cflp, ncfl, trig, tab, tabz, tstride, split ? 0 : 1);
+  if (split) {
+    const int n = c.g.sr * c.g.sk;
+    k_pre_rings<<<(n + 255) / 256, 256, 0, c.stream>>>(c.g, c.p, c.ss, tab, tabz, tstride);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_post_step(const LaunchCtx& c, int ncfl) {
   if (ncfl < 0 || ncfl > c.ncflp || c.p.plnt == 2) return hipErrorInvalidValue;
-  k_post_step<<<1, kStepThreads, 0, c.stream>>>(c.g, c.p, c.ss, c.cflp, ncfl);
+  const bool split = split_rings(c.g);
+  if (split && c.p.fargo > 0) {
+    const int n = c.g.imax * c.g.kmax;
+    k_rotate<<<(n + 255) / 256, 256, 0, c.stream>>>(c.g);
+  }
+  k_post_step<<<1, kStepThreads, 0, c.stream>>>(c.g, c.p, c.ss, c.cflp, ncfl, split ? 0 : 1);
   return hipGetLastError();
 }
 

@@ -56,10 +56,11 @@ hipError_t launch_copy(hipStream_t s, const void* a, void* b, size_t nbytes);
 hipError_t launch_cfl_partials(const LaunchCtx& c, int src, int* n);
 // one workgroup before the sweeps: dt from ncfl partials (0: keep ss->dt), Index_Shift,
 // opening half kick; trig / tab / tabz (FP32 tables: rows of stride tstride, tabz the 3-D
-// theta-major copy) may be null
+// theta-major copy) may be null.  Disks of many slots get Index_Shift and the table rows from
+// a second, grid-wide kernel.
 hipError_t launch_pre_step(const LaunchCtx& c, int ncfl, float2* trig, float* tab, float* tabz, int tstride);
 // one workgroup after the sweeps (plnt_flag < 2): roll, time and planet update, then dt
-// from ncfl partials when ncfl > 0
+// from ncfl partials when ncfl > 0 (disks of many slots: the roll from a grid-wide kernel)
 hipError_t launch_post_step(const LaunchCtx& c, int ncfl);
 // the same for every disk of a batch, one workgroup per disk (pre-step with ncfl 0; post-step
 // with each disk's ncfl_post)

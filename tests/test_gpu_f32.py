@@ -261,7 +261,9 @@ def flags_case(suffix, **over):
 FLAG_CASES = [flags_case("_fargo2", fargo_flag=2), flags_case("_inertial", frrot_flag=0),
               flags_case("_noflat", flat_flag=0), flags_case("_noplanet", plnt_flag=0),
               flags_case("_nofargo", fargo_flag=0), flags_case("_growing_planet", t_growth=5.0),
-              flags_case("_strong_planet", M_p=1e-3)]
+              flags_case("_strong_planet", M_p=1e-3),
+              # more than 2048 slots: Index_Shift, the table rows and the roll over the whole grid
+              configs.small(2048, 116, M_p=1e-3)]
 
 
 @pytest.mark.parametrize("w", FLAG_CASES, ids=lambda w: w.name)

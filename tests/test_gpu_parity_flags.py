@@ -100,6 +100,16 @@ CASES = [
     # (evolve.cu:110-124, remap.cu:50-54): the last-ulp difference of the device and host cos
     # is amplified by that cancellation, most visibly in v_theta, which starts at zero
     (disk3d(24, 72, 12), ("r", "u", "v", "w"), (1e-12, 1e-9), (1e-10, 1e-7)),
+    # disks of more than 2048 slots (radial and theta cells with their ghosts) run Index_Shift
+    # and the roll over the whole grid instead of in the pre- and post-step's one workgroup.
+    # The theta range keeps dtheta near the 24 x 72 x 12 case's: on finer theta grids the
+    # cancellation above makes a single theta sweep's v_theta more sensitive to last-ulp cos
+    # differences (one ulp added to every cos of the oracle moves v_theta by Linf 5.6e-7 on
+    # 56 x 96 x 24 over the same +-0.15, by 3.9e-10 on this grid)
+    (named(reparam(disk3d(48, 72, 36), nlft=1, nrgh=0, zmin=HPI - 0.45, zmax=HPI + 0.45), "_grid_wide_rings"),
+     ("r", "u", "v", "w"), (1e-12, 1e-9), (1e-10, 1e-7)),
+    (named(configs.small(2048, 96, M_p=1e-3), "_grid_wide_rings"), ("r", "u", "v"), (1e-13, 1e-11),
+     (1e-11, 1e-8)),
 ]
 
 
