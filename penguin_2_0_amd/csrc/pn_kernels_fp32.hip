@@ -109,7 +109,7 @@ template <int K, int W, int Q, int NF>
 struct XTile {
   static constexpr int WIN = 64 * K, NC = W * Q, LD = WIN + 2;
   float tile[NF][NC][LD];   // [field][column][cell]; LD even keeps K-float reads aligned
-  int srot[WIN];
+  int srot[WIN];            // rot_j of the window's slots
   float red[W];
 };
 
@@ -124,28 +124,49 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
   constexpr int NF = (G == kGeoRcyl) ? 3 : 4;
   constexpr int WIN = 64 * K, REAL = WIN - 2 * kNpad, NC = W * Q;
   constexpr bool TH = (G == kGeoTheta);
+  // staging: thread t moves the NIT cells c = t / NC + it (64 W / NC) of column t % NC, the
+  // same cells on the way in and out
+  constexpr int TPB = 64 * W, NIT = WIN * NC / TPB, CSTEP = TPB / NC;
+  static_assert(TPB % NC == 0 && (WIN * NC) % TPB == 0, "staging map");
   auto& tile = T.tile;
-  auto& srot = T.srot;
   const int jmax = g.jmax;
-  __syncthreads();
   const int jb = grp * NC;
   const int lim = TH ? win_len_k<K>(g.kmax, f.nwin_z, l) : win_len_k<K>(g.imax, f.nwin_x, l);
   const int s0 = TH ? slot_of(g, row, REAL * l - kNpad) : slot_of(g, REAL * l - kNpad, row);
   const int step = TH ? g.sr : 1;   // slots between neighbouring cells of the pencil
-  for (int c = threadIdx.x; c < WIN; c += 64 * W) srot[c] = g.rot_j[s0 + step * (c < lim ? c : lim - 1)];
+  const int qs = threadIdx.x % NC, cs = threadIdx.x / NC;
+  const int jl = jb + qs;
+  __syncthreads();   // the previous item's stores have read srot and the tile
+  for (int c = threadIdx.x; c < WIN; c += 64 * W) T.srot[c] = g.rot_j[s0 + step * (c < lim ? c : lim - 1)];
   __syncthreads();
   // the fields as buffer views with 32-bit byte offsets (a state buffer is below 2 GiB)
   const unsigned fbytes = 4u * (unsigned)(g.sr * g.sk) * (unsigned)jmax;
   Buf fin[NF];
 #pragma unroll
   for (int fi = 0; fi < NF; ++fi) fin[fi] = Buf(f.f[src][fi], fbytes);
-  for (int e = threadIdx.x; e < WIN * NC; e += 64 * W) {
-    const int q = e % NC, c = e / NC;
-    const int jl = jb + q;
-    if (jl < jmax) {
-      const unsigned o = 4u * (unsigned)((s0 + step * (c < lim ? c : lim - 1)) * jmax + wrapj(jl - srot[c], jmax));
+  // the tile's loads go out in chunks of CH cells per thread, each chunk's loads issued
+  // before its first wait (at most 12 in flight: more raised the register count of the
+  // pencils below and cost a wave of occupancy); columns past jmax (the last column group)
+  // read column jmax - 1, which no wave computes and no store writes
+  constexpr int CH = NF == 3 ? (NIT < 4 ? NIT : 4) : 2;
+  static_assert(NIT % CH == 0, "staging chunks");
+  const int jlc = jl < jmax ? jl : jmax - 1;
 #pragma unroll
-      for (int fi = 0; fi < NF; ++fi) tile[fi][q][c] = fin[fi].f(o);
+  for (int i0 = 0; i0 < NIT; i0 += CH) {
+    float v[CH][NF];
+#pragma unroll
+    for (int it = 0; it < CH; ++it) {
+      const int c = cs + CSTEP * (i0 + it);
+      const unsigned o =
+          4u * (unsigned)((s0 + step * (c < lim ? c : lim - 1)) * jmax + wrapj(jlc - T.srot[c], jmax));
+#pragma unroll
+      for (int fi = 0; fi < NF; ++fi) v[it][fi] = fin[fi].f(o);
+    }
+#pragma unroll
+    for (int it = 0; it < CH; ++it) {
+      const int c = cs + CSTEP * (i0 + it);
+#pragma unroll
+      for (int fi = 0; fi < NF; ++fi) tile[fi][qs][c] = v[it][fi];
     }
   }
   __syncthreads();
@@ -256,12 +277,12 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
   Buf fout[NF];
 #pragma unroll
   for (int fi = 0; fi < NF; ++fi) fout[fi] = Buf(f.f[src ^ 1][fi], fbytes);
-  for (int e = threadIdx.x; e < WIN * NC; e += 64 * W) {
-    const int q = e % NC, c = e / NC;
-    const int jl = jb + q;
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int q = qs, c = cs + CSTEP * it;
     if (jl < jmax && c >= kNpad && c < lim - kNpad) {
       const int sl = s0 + step * c;
-      const int j = wrapj(jl - srot[c], jmax);
+      const int j = wrapj(jl - T.srot[c], jmax);
       const unsigned o = 4u * (unsigned)(sl * jmax + j);
 #pragma unroll
       for (int fi = 0; fi < NF; ++fi) fout[fi].put(tile[fi][q][c], o);
