@@ -843,6 +843,15 @@ static void flatten(win_t* W, int nmax, const double* p, const double* u, double
   }
 }
 
+/* the energy a cell carries into the Lagrangian step: EOS 2 as ppm.cu:106-108 forms it (v is
+   the first transverse slot, already r_cyl v_phi in the radial sweep); EOS 3 the internal
+   energy plus the sweep direction's kinetic energy; unused (0) otherwise */
+static double orc_energy(const pn_params* P, double p, double r, double u, double v) {
+  if (P->eos == 2) return p / (r * (P->gam - 1.0)) + 0.5 * ((u * u) + (v * v));
+  if (P->eos == 3) return p / (r * (P->gam - 1.0)) + 0.5 * (u * u);
+  return 0.0;
+}
+
 /* riemann, riemann.cu:3-115 */
 void orc_riemann(int eos, double gam, double pl, double rl, double ul, double pr, double rr, double ur,
                  double* umid, double* pmid) {
@@ -853,7 +862,7 @@ void orc_riemann(int eos, double gam, double pl, double rl, double ul, double pr
     double pm = sqrt(pl * pr) * exp(t2);
     *umid = ul - t1 * log(pm / pl);
     *pmid = fmax(smallp, pm);
-  } else if (eos == 2) {
+  } else if (eos == 2 || eos == 3) {
     double cs_l = sqrt(gam * pl / rl);
     double cs_r = sqrt(gam * pr / rr);
     double pm = pow((cs_l + cs_r - (gamm / 2.0) * (ur - ul)) / (cs_l * pow(pl, -gamz) + cs_r * pow(pr, -gamz)),
@@ -1067,6 +1076,12 @@ static void ppmlr_w(win_t* W, const wargs* A) {
       else
         q[n] = ef - 0.5 * (u[n] * u[n]) - orc_potential(P, A->pl, A->rad[n], A->azi[n], xcl);
       q[n] = fmax(gamm * q[n], smallp);
+    } else if (P->eos == 3) {
+      /* e = p / (gamm rho) + u^2/2 with the pressure work and the time-centred work of the
+         forces that moved u (the commented line of evolve.cu:146); p from what remains */
+      double ef = e[n] - W->dtdm[n] * (vfac[n + 1] * umid[n + 1] - vfac[n] * umid[n]) +
+                  0.5 * dt * (dum * W->fto1[n] + u[n] * W->fto2[n]);
+      q[n] = fmax(gamm * (ef - 0.5 * (u[n] * u[n])), smallp);
     }
   }
 
@@ -1123,7 +1138,7 @@ static void ppmlr_w(win_t* W, const wargs* A) {
   }
   /* u, v, w and q: parabola, face flux = face value * mass flux, update */
   double* fld[4] = {u, v, w, q};
-  int nf = (P->eos <= 2) ? 4 : 3;
+  int nf = 4; /* q is remapped for every EOS (remap.cu:144-176) */
   for (int f = 0; f < nf; f++) {
     double* x = fld[f];
     parabola(W, 3, nmax - 3, W->flat, x, al, da, a6);
@@ -1375,8 +1390,7 @@ void orc_sweepx(const pn_grid* g, const pn_params* P, void* rings, const void* l
             w[i] = pr[5 * c + 4];
             if (P->ngeomy > 2) v[i] *= radc[i];
             if (P->ngeomz == 5) w[i] *= rad[i];
-            e[i] = 0.0;
-            if (P->eos == 2) e[i] = p[i] / (r[i] * (P->gam - 1.0)) + 0.5 * ((u[i] * u[i]) + (v[i] * v[i]));
+            e[i] = orc_energy(P, p[i], r[i], u[i], v[i]);
             vrot[i] = *R.res_v / radc[i];
           }
           window_update(W, P, 0, lim, dt, FrRot, planet, rad, azi, pol, vrot, dvol0, xa0, dx0, r, p, u, v, w, e);
@@ -1456,8 +1470,7 @@ void orc_sweepy(const pn_grid* g, const pn_params* P, void* rings, double dt, co
           u[i] = pr[5 * j0 + 2];
           v[i] = pr[5 * j0 + 3];
           w[i] = pr[5 * j0 + 4];
-          e[i] = 0.0;
-          if (P->eos == 2) e[i] = p[i] / (r[i] * (P->gam - 1.0)) + 0.5 * ((u[i] * u[i]) + (v[i] * v[i]));
+          e[i] = orc_energy(P, p[i], r[i], u[i], v[i]);
           radv[i] = rad_cyl;
           polv[i] = pol;
           vrot[i] = *R.res_v / rad_cyl;
@@ -1547,8 +1560,7 @@ void orc_sweepz(const pn_grid* g, const pn_params* P, void* rings, const void* u
             u[kk] = pr[5 * c + 4];
             v[kk] = pr[5 * c + 2];
             w[kk] = pr[5 * c + 3] * radc[kk];
-            e[kk] = 0.0;
-            if (P->eos == 2) e[kk] = p[kk] / (r[kk] * (P->gam - 1.0)) + 0.5 * ((u[kk] * u[kk]) + (v[kk] * v[kk]));
+            e[kk] = orc_energy(P, p[kk], r[kk], u[kk], v[kk]);
             vrot[kk] = *R.res_v / radc[kk];
           }
           window_update(W, P, 2, lim, dt, FrRot, planet, rad, azi, pol, vrot, dvol0, xa0, dx0, r, p, u, v, w, e);

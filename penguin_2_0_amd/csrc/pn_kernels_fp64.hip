@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void k_sweep_x(DevGrid g, DevParams prm, const
     if (CF::GY > 2) v *= rc;
     if (CF::GZ == 5) w *= rad;
     W.r[h] = r; W.p[h] = p; W.u[h] = u; W.v[h] = v; W.w[h] = w;
-    W.e[h] = (CF::EOS == 2) ? p / (r * (prm.gam - 1.0)) + 0.5 * ((u * u) + (v * v)) : 0.0;
+    W.e[h] = energy_of<CF>(p, r, u, v, prm.gam);
   }
 
   ppmlr_wave<double, kK, CF>(W, lim, dt, FrRot, pl, rf, prm.gam);
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void k_sweep_y(DevGrid g, DevParams prm, const
     const double r = fr[j], p = fp[j], u = fv[j];
     const double v = prm.has_w ? fw[j] : 0.0;
     W.r[h] = r; W.p[h] = p; W.u[h] = u; W.v[h] = v; W.w[h] = fu[j];
-    W.e[h] = (CF::EOS == 2) ? p / (r * (prm.gam - 1.0)) + 0.5 * ((u * u) + (v * v)) : 0.0;
+    W.e[h] = energy_of<CF>(p, r, u, v, prm.gam);
   }
 
   ppmlr_wave<double, kK, CF>(W, lim, dt, FrRot, pl, rf, prm.gam);
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void k_sweep_z(DevGrid g, DevParams prm, const
     const double u = g.f[src][FW][a], v = g.f[src][FU][a];
     W.r[h] = r; W.p[h] = p; W.u[h] = u; W.v[h] = v;
     W.w[h] = g.f[src][FV][a] * rc;
-    W.e[h] = (CF::EOS == 2) ? p / (r * (prm.gam - 1.0)) + 0.5 * ((u * u) + (v * v)) : 0.0;
+    W.e[h] = energy_of<CF>(p, r, u, v, prm.gam);
   }
 
   ppmlr_wave<double, kK, CF>(W, lim, dt, FrRot, pl, rf, prm.gam);
@@ -323,6 +323,8 @@ hipError_t dispatch(const LaunchCtx& c, int src, long waves) {
       case 3: PN_LAUNCH(1, 3, 0, 1, 1); break;
       case 4: PN_LAUNCH(1, 3, 0, 2, 0); break;
       case 5: PN_LAUNCH(1, 3, 0, 2, 1); break;
+      case 6: PN_LAUNCH(1, 3, 0, 3, 0); break;
+      case 7: PN_LAUNCH(1, 3, 0, 3, 1); break;
       default: return hipErrorInvalidValue;
     }
   } else if (p.ngeomx == 2 && p.ngeomy == 4 && p.ngeomz == 5) {

@@ -94,13 +94,13 @@ int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision)
   if (g->imax < 1 || g->jmax < 2 * kNpad || g->kmax < 1) return fail(c, PN_ERR_ARG, "bad grid %d x %d x %d", g->imax, g->jmax, g->kmax);
   if (p->ndim != 2 && p->ndim != 3) return fail(c, PN_ERR_ARG, "ndim must be 2 or 3");
   if ((p->ndim == 2) != (g->kmax == 1)) return fail(c, PN_ERR_ARG, "ndim %d with kmax %d", p->ndim, g->kmax);
-  if (p->eos < 0 || p->eos > 2) return fail(c, PN_ERR_ARG, "eos %d", p->eos);
+  if (p->eos < 0 || p->eos > 3) return fail(c, PN_ERR_ARG, "eos %d", p->eos);
   const bool cyl = (p->ngeomx == 1 && p->ngeomy == 3 && p->ngeomz == 0);
   const bool sph = (p->ngeomx == 2 && p->ngeomy == 4 && p->ngeomz == 5);
   if (p->ndim == 2 && !cyl) return fail(c, PN_ERR_UNSUPPORTED, "2-D needs cylindrical geometry (1,3,0)");
   if (p->ndim == 3 && !sph) return fail(c, PN_ERR_UNSUPPORTED, "3-D needs spherical geometry (2,4,5)");
   if (p->ndim == 3 && p->visc_flag) return fail(c, PN_ERR_UNSUPPORTED, "3-D viscosity is not built");
-  if (p->ndim == 3 && p->eos == 1) return fail(c, PN_ERR_UNSUPPORTED, "3-D EOS 1 is not built");
+  if (p->ndim == 3 && (p->eos == 1 || p->eos == 3)) return fail(c, PN_ERR_UNSUPPORTED, "3-D EOS %d is not built", p->eos);
   if (p->fargo_flag < 0 || p->fargo_flag > 2) return fail(c, PN_ERR_ARG, "fargo_flag %d", p->fargo_flag);
   if (p->plnt_flag < 0 || p->plnt_flag > 2) return fail(c, PN_ERR_ARG, "plnt_flag %d", p->plnt_flag);
   if (p->plnt_flag == 2 && p->ndim == 3) return fail(c, PN_ERR_UNSUPPORTED, "moving planet needs the 2-D force reductions");

@@ -268,6 +268,16 @@ struct Cfg {
   static constexpr int AXIS = AXIS_, GX = GX_, GY = GY_, GZ = GZ_, EOS = EOS_, VISC = VISC_;
 };
 
+// the energy a cell carries into the Lagrangian step from the window's p, rho, u and first
+// transverse slot v: EOS 2 as ppm.cu:106-108 has it (v already scaled by r_cyl in the radial
+// sweep), EOS 3 the internal energy plus the sweep direction's kinetic energy; 0 otherwise
+template <class CF>
+__device__ __forceinline__ double energy_of(double p, double r, double u, double v, double gam) {
+  if constexpr (CF::EOS == 2) return p / (r * (gam - 1.0)) + 0.5 * ((u * u) + (v * v));
+  else if constexpr (CF::EOS == 3) return p / (r * (gam - 1.0)) + 0.5 * (u * u);
+  else return 0.0;
+}
+
 struct RunFlags {
   int flat;      // flattening on
   int planet;    // plnt_flag > 0
@@ -588,7 +598,7 @@ __device__ __forceinline__ void ppmlr_wave(Win<T, K>& W, int nmax, T dt, T FrRot
       T t2 = T(-0.5) * (ur - ul) / t1;
       pm = sqrt(pl_ * pr) * exp(t2);
       um = ul - t1 * log(pm / pl_);
-    } else if constexpr (CF::EOS == 2) {
+    } else if constexpr (CF::EOS == 2 || CF::EOS == 3) {
       const T gamfac2 = gam + T(1.0), gamz = gamm / gam / T(2.0);
       T cs_l = sqrt(gam * pl_ / rl);
       T cs_r = sqrt(gam * pr / rr);
@@ -728,6 +738,13 @@ __device__ __forceinline__ void ppmlr_wave(Win<T, K>& W, int nmax, T dt, T FrRot
         else
           qh = ef - T(0.5) * (un * un) - potential<T>(rf.planet, pl, W.rad[h], W.azi[h], x_);
         qh = fmax(gamm * qh, smallp);
+      } else if constexpr (CF::EOS == 3) {
+        // the energy-consistent form (the commented line of evolve.cu:146): e = p / (gamm rho)
+        // + u^2 / 2 gains the pressure work and the time-centred work of the forces that move
+        // u; the internal energy is what remains after the new u^2 / 2
+        T ef = W.e[h] - dtdm[h] * (vfp[h] * ump[h] - vf[h] * umidp[h]) +
+               T(0.5) * dt * (W.u[h] * fto1[h] + un * fto2[h]);
+        qh = fmax(gamm * (ef - T(0.5) * (un * un)), smallp);
       }
       q[h] = qh;
       if (in) {

@@ -1,7 +1,8 @@
 """FP64 HIP path against the FP64 CPU oracle for the flag combinations beyond the plain
-isothermal disk: viscosity, the isentropic (EOS 1) and adiabatic (EOS 2) closures, a planet
-that moves under the disk's torque, FARGO frame mode 2, a non-rotating frame, no
-flattening, a growing planet, and the 3-D r-phi-theta disk with its theta sweep.
+isothermal disk: viscosity, the isentropic (EOS 1) and adiabatic (EOS 2) closures, the
+energy-consistent adiabatic step (EOS 3), a planet that moves under the disk's torque,
+FARGO frame mode 2, a non-rotating frame, no flattening, a growing planet, and the 3-D
+r-phi-theta disk with its theta sweep.
 
 Bars as in test_gpu_parity.py: integers (rot_j) bit-exact, reals within the written
 relative L1 / Linf tolerances after each single sweep and after 10 full steps.  The
@@ -67,6 +68,11 @@ CASES = [
     (named(configs.small(48, 144, eos=2, gam=1.4), "_eos2_adiabatic_ic"), ("r", "p", "u", "v"),
      (1e-12, 1e-10), (1e-10, 1e-8)),
     (named(configs.small(48, 144, eos=1, gam=1.4), "_eos1"), ("r", "p", "u", "v"), (1e-9, 1e-7), (1e-8, 1e-6)),
+    # EOS 3: EOS 2's Riemann solver with the energy-consistent Lagrangian step
+    (named(configs.small(48, 144, eos=3, gam=1.4, init_p_iso=1, M_p=1e-3), "_eos3"), ("r", "p", "u", "v"),
+     (1e-12, 1e-10), (1e-10, 1e-8)),
+    (named(configs.small(48, 144, eos=3, gam=1.4, M_p=1e-3, visc_flag=1, ss_alpha=1e-2, vis_nu=2.5e-5),
+           "_eos3_visc_adiabatic_ic"), ("r", "p", "u", "v"), (1e-12, 1e-10), (1e-10, 1e-8)),
     (named(configs.small(48, 144, plnt_flag=2, M_p=1e-3), "_moving_planet"), ("r", "u", "v"),
      (1e-13, 1e-11), (1e-11, 1e-8)),
     (named(configs.small(48, 144, fargo_flag=2), "_fargo2"), ("r", "u", "v"), (1e-13, 1e-11), (1e-11, 1e-8)),

@@ -164,9 +164,20 @@ def test_batch_argument_errors_before_any_device_call():
 
 def test_fp32_parameter_errors_before_any_device_call():
     # the FP32 path is built for the isothermal closure only
+    for eos in (1, 2, 3):
+        with pytest.raises(abi.PnError) as e:
+            abi.Context(PnGrid(32, 48, 1), configs.disk2d(32, 48, eos=eos, gam=1.4), abi.PN_FP32)
+        assert e.value.code == abi.PN_ERR_UNSUPPORTED
+
+
+def test_eos3_is_two_dimensional():
+    p3 = configs.c3().params.replace(eos=3, gam=1.4)
     with pytest.raises(abi.PnError) as e:
-        abi.Context(PnGrid(32, 48, 1), configs.disk2d(32, 48, eos=2, gam=1.4), abi.PN_FP32)
+        abi.Context(PnGrid(24, 72, 12), p3)
     assert e.value.code == abi.PN_ERR_UNSUPPORTED
+    with pytest.raises(abi.PnError) as e:
+        abi.Context(PnGrid(32, 48, 1), configs.disk2d(32, 48, eos=4, gam=1.4))
+    assert e.value.code == abi.PN_ERR_ARG
 
 
 @pytest.mark.skipif(_gpu_visible(), reason="checks the no-device error path")

@@ -497,6 +497,125 @@ def test_radial_sweep_equilibrium_residual_converges():
     assert res[0] / res[1] > 3.5 and res[1] / res[2] > 3.5
 
 
+# ---------------------------------------------------------------------------------------
+# EOS 3: the energy-consistent adiabatic Lagrangian step (the line evolve.cu:146 leaves
+# commented out).  The shock tube pins the pressure-work term (a flipped sign there moves
+# the mean |rho - exact| at 200 cells from 0.0026 to 0.080), the cold infall the work of the
+# body forces (a flipped sign there drives p to the floor), the disk without a planet the
+# energy each cell carries in.
+
+def exact_gamma_law(rl, ul, pl, rr, ur, pr, gam, s):
+    """Exact Riemann solution of the gamma-law gas (two-shock / two-rarefaction pressure
+    function solved by bisection, then sampled at s = x / t): rho, u, p."""
+    cl, cr = math.sqrt(gam * pl / rl), math.sqrt(gam * pr / rr)
+    g1, g2 = (gam - 1.0) / (gam + 1.0), (gam - 1.0) / (2.0 * gam)
+
+    def fk(p, rk, pk, ck):
+        if p > pk:
+            return (p - pk) * math.sqrt(2.0 / ((gam + 1.0) * rk) / (p + g1 * pk))
+        return 2.0 * ck / (gam - 1.0) * ((p / pk) ** g2 - 1.0)
+
+    lo, hi = 1e-12, 10.0 * max(pl, pr)
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if fk(mid, rl, pl, cl) + fk(mid, rr, pr, cr) + ur - ul > 0.0:
+            hi = mid
+        else:
+            lo = mid
+    ps = 0.5 * (lo + hi)
+    us = 0.5 * (ul + ur) + 0.5 * (fk(ps, rr, pr, cr) - fk(ps, rl, pl, cl))
+    out = np.zeros((3, len(s)))
+    for n, x in enumerate(s):
+        side = -1.0 if x < us else 1.0
+        rk, uk, pk, ck = (rl, ul, pl, cl) if side < 0 else (rr, ur, pr, cr)
+        if ps > pk:   # shock
+            sk = uk + side * ck * math.sqrt((gam + 1.0) / (2.0 * gam) * ps / pk + g2)
+            inside = (x > sk) if side < 0 else (x < sk)
+            out[:, n] = (rk * (ps / pk + g1) / (g1 * ps / pk + 1.0), us, ps) if inside else (rk, uk, pk)
+        else:         # rarefaction: head at uk -+ ck, tail at us -+ c*
+            cs = ck * (ps / pk) ** g2
+            head, tail = uk + side * ck, us + side * cs
+            if (x < head) if side < 0 else (x > head):
+                out[:, n] = (rk, uk, pk)
+            elif (x > tail) if side < 0 else (x < tail):
+                out[:, n] = (rk * (ps / pk) ** (1.0 / gam), us, ps)
+            else:
+                u = 2.0 / (gam + 1.0) * (-side * ck + 0.5 * (gam - 1.0) * uk + x)
+                c = 2.0 / (gam + 1.0) * (ck - side * 0.5 * (gam - 1.0) * (uk - x))
+                out[:, n] = (rk * (c / ck) ** (2.0 / (gam - 1.0)), u, pk * (c / ck) ** (2.0 * gam / (gam - 1.0)))
+    return out
+
+
+def test_exact_gamma_law_solution_satisfies_jump_conditions():
+    """The test's own exact solution: Sod's problem has p* = 0.30313, u* = 0.92745 (Toro,
+    table 4.2), and the shock state obeys Rankine-Hugoniot mass and momentum balance."""
+    s = np.array([-2.0, 0.5, 1.5, 2.0])
+    ex = exact_gamma_law(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4, s)
+    assert abs(ex[2, 1] - 0.30313) < 5e-5 and abs(ex[1, 1] - 0.92745) < 5e-5
+    r2, u2, p2 = ex[:, 2]
+    S = (r2 * u2 - 0.125 * 0.0) / (r2 - 0.125)
+    assert 1.5 < S < 2.0
+    assert abs((r2 * u2 * (u2 - S) + p2) - (0.1 - 0.125 * 0.0 * S)) < 1e-9
+
+
+def _sod_eos3(n, gam=1.4):
+    """Sod's shock tube along the radial sweep in planar geometry (ngeomx 0, no gravity)
+    with EOS 3; mean |rho - exact| and |p - exact| at t = 0.15."""
+    P = configs.disk2d(n, 12).replace(ngeomx=0, plnt_flag=0, fargo_flag=0, visc_flag=0, frrot_flag=0,
+                                      grspx=0, xmin=0.0, xmax=1.0, eos=3, gam=gam, init_p_iso=1)
+    o = oracle.OracleDisk(PnGrid(n, 12, 1), P, 1, 0.0)
+    for A in (o.rings, o.lft, o.rgh):
+        left = A["xc"] < 0.5
+        A["r"] = np.where(left, 1.0, 0.125)[:, None] * np.ones(12)
+        A["p"] = np.where(left, 1.0, 0.1)[:, None] * np.ones(12)
+        A["u"] = 0.0
+        A["v"] = 0.0
+        A["rot_j"] = 0
+        A["res_v"] = 0.0
+    t_end = 0.15
+    nstep = int(t_end / (0.3 / n / 1.8)) + 1
+    for _ in range(nstep):
+        o.sweepx(t_end / nstep, PnBody(), 0.0)
+    ex = exact_gamma_law(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, gam, (o.rings["xc"] - 0.5) / t_end)
+    return (np.abs(o.rings["r"][:, 0] - ex[0]).mean(), np.abs(o.rings["p"][:, 0] - ex[2]).mean())
+
+
+def test_eos3_shock_tube_converges_to_exact_solution():
+    (r1, p1), (r2, p2) = _sod_eos3(100), _sod_eos3(200)
+    assert r2 < 4e-3 and p2 < 4e-3
+    assert r2 < 0.75 * r1 and p2 < 0.75 * p1
+
+
+def test_eos3_cold_infall_keeps_the_adiabatic_invariant():
+    """A cold ring at rest falls towards the star (no rotation, no planet): the work of
+    gravity goes into u^2 / 2 (2.7 times the internal energy here), so p / rho^gamma stays
+    fixed only if that work is booked with the right sign."""
+    n, p0, gam = 128, 1e-4, 1.4
+    P = configs.disk2d(n, 12).replace(plnt_flag=0, fargo_flag=0, frrot_flag=0, grspx=0, xmin=1.0, xmax=2.0,
+                                      eos=3, gam=gam, init_p_iso=1)
+    o = oracle.OracleDisk(PnGrid(n, 12, 1), P, 1, 0.0)
+    for A in (o.rings, o.lft, o.rgh):
+        A["r"] = 1.0
+        A["p"] = p0
+        A["u"] = 0.0
+        A["v"] = 0.0
+    for _ in range(5):
+        o.sweepx(0.01, PnBody(), 0.0)
+    r, p, u = (o.rings[f][20:n - 20, 0] for f in ("r", "p", "u"))
+    assert 0.5 * (u * u).max() > 2.0 * p0 / (gam - 1.0)
+    np.testing.assert_allclose((p / p0) / r ** gam, 1.0, atol=1e-3)
+
+
+def test_eos3_disk_without_planet_stays_in_equilibrium():
+    w = configs.small(64, 192, eos=3, gam=1.4, init_p_iso=1, plnt_flag=0, M_p=0.0)
+    o = oracle.OracleDisk(w.grid, w.params, w.seed, w.noise)
+    p0 = o.rings["p"].copy()
+    st = o.run_begin()
+    o.run(100, st)
+    assert np.abs(o.rings["p"] / p0 - 1.0).max() < 1e-3
+    assert np.abs(o.rings["u"]).max() < 1e-4
+
+
 def test_damping_zones_relax_towards_background_only_near_edges():
     """kill_wave: a state equal to its background is a fixed point; otherwise every field's
     distance to the background shrinks by the same factor 1 - f per cell, f growing linearly
