@@ -9,8 +9,9 @@ the K timed steps of the measured path, with the state resident in HBM.
 
 With the default FP32 path the plain FP64 path is run beside it on the same initial state
 for the same W + K steps: its throughput gives the speedup, and its final state is the
-reference for the relative L1 / Linf errors of Sigma, v_r and v_phi (compared in the lab
-frame) and for the mass and angular-momentum drift.  The "batched" object times the
+reference for the relative L1 / Linf errors of Sigma, v_r and v_phi (and p for an adiabatic
+disk, e.g. --config c5e3: the 4096 x 12288 disk with the energy-consistent EOS 3), compared
+in the lab frame, and for the mass and angular-momentum drift.  The "batched" object times the
 parameter sweep of 48 uneven viscous disks (N_r 128..512) advanced as one FP32 batch against
 the same disks run one by one on the plain FP64 path, with each disk's error set beside its
 sensitivity floor (the drift of the FP64 run from the FP32-rounded initial state).
@@ -229,7 +230,7 @@ def main(argv=None) -> int:
     }
     if prec == abi.PN_FP32 and not a.no_ref:
         ref = timed_run(w, abi.PN_FP64, a.steps, a.warmup, graph=not a.eager)
-        errs = {f: rel_err(lab(run["state"], f), lab(ref["state"], f)) for f in ("r", "u", "v")}
+        errs = {f: rel_err(lab(run["state"], f), lab(ref["state"], f)) for f in ("r", "u", "v", "p")}
         out["fp64_plain"] = {"value": ref["value"], "ms_per_step": ref["wall"] / a.steps * 1e3,
                              "speedup": value / ref["value"], "kernels_ms": ref["km"],
                              "mass_drift": ref["dmass"], "angmom_drift": ref["dangmom"]}
@@ -238,6 +239,7 @@ def main(argv=None) -> int:
             "l1_rho": errs["r"][0], "linf_rho": errs["r"][1],
             "l1_vr": errs["u"][0], "linf_vr": errs["u"][1],
             "l1_vphi": errs["v"][0], "linf_vphi": errs["v"][1],
+            **({"l1_p": errs["p"][0], "linf_p": errs["p"][1]} if w.params.eos != 0 else {}),
             "rot_j_equal": bool(np.array_equal(run["state"]["rot_j"], ref["state"]["rot_j"])),
             "dt_rel_diff": abs(run["st"].dt - ref["st"].dt) / ref["st"].dt,
         }

@@ -83,6 +83,12 @@ def c5() -> Workload:
     return Workload("c5_disk2d_adiabatic_4096x12288", PnGrid(4096, 12288, 1), p, seed=5, steps=5000)
 
 
+def c5e3() -> Workload:
+    """The C5 disk with the energy-consistent adiabatic step (EOS 3) in place of EOS 2."""
+    w = c5()
+    return Workload("c5_disk2d_adiabatic_eos3_4096x12288", w.grid, w.params.replace(eos=3), seed=5, steps=5000)
+
+
 def small(nr: int = 48, nphi: int = 144, **over) -> Workload:
     """A small disk of the same physics for fast parity tests."""
     return Workload(f"disk2d_{nr}x{nphi}", PnGrid(nr, nphi, 1), disk2d(nr, nphi, **over), seed=7, steps=10)

@@ -174,6 +174,8 @@ int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int pr
       return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches are built for 2-D disks", d);
     if (params[d].plnt_flag == 2)
       return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: a moving planet needs its force reductions between sweeps", d);
+    if (params[d].eos != 0)
+      return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches are built for the isothermal closure (EOS 0)", d);
     pn_ctx* c = nullptr;
     const int rc = pn_create(&grids[d], &params[d], precision, &c);
     if (c) b->disks.push_back(c);

@@ -31,7 +31,7 @@ struct pn_ctx {
   int cur = 0;                      // buffer holding the state
   bool loaded = false;
   // host copies of the per-ring values the device never changes (for packing)
-  std::vector<double> h_x, h_dx, h_xc, h_xvol, h_z, h_dz, h_zc, h_zvol, h_vbg, h_rbg, h_cs2;
+  std::vector<double> h_x, h_dx, h_xc, h_xvol, h_z, h_dz, h_zc, h_zvol, h_vbg, h_rbg, h_cs2, h_qbg;
   std::vector<double> h_y, h_dy, h_yc, h_yvol;
   // graph of one step (2-D) or two steps (3-D, the buffers alternate)
   hipGraphExec_t gexec = nullptr;

@@ -6,7 +6,8 @@
 namespace pn {
 
 // per-slot constants of the FP32 sweeps, entry-major: entry e of slot s at tab[e*stride + s]
-// (FP64 values rounded once; rho_b is Sigma_b, p_b = c_s^2 Sigma_b, L_b = r v_bg):
+// (FP64 values rounded once; rho_b is Sigma_b, p_b = c_s^2 Sigma_b (EOS 3: the initial
+// pressure of the background record), L_b = r v_bg):
 //   rad (cell centre), xa0 (left edge), dx0 (width), 1/rad,
 //   rho_b, rho_b(s+1) - rho_b(s), p_b, p_b(s+1) - p_b(s),
 //   c_s^2, c_s, L_b, L_b(s+1) - L_b(s),
@@ -48,12 +49,15 @@ enum TabEntry {
   // per step, azimuthal sweep on a uniform grid: the same from the column's left edge,
   // sin / cos of (dt (vrot - v_y) / 2 - dphi / 2) / 2
   kTabRotPhS, kTabRotPhC,
+  // EOS 3 (2-D): q_b = p_b / rho_b and q_b(s+1) - q_b(s)
+  kTabQb, kTabDqb,
   kTabEntries
 };
 constexpr int kTabOverhang = 256;   // >= the longest radial window
 
 // FP32 state as deviations from per-slot backgrounds, and the per-slot constants.
-// Fields: 0 Sigma - Sigma_b(slot), 1 v_r, 2 v_phi - v_bg(slot), 3 v_theta (3-D only).
+// Fields: 0 Sigma - Sigma_b(slot), 1 v_r, 2 v_phi - v_bg(slot), 3 v_theta (3-D) or
+// p / Sigma - q_b(slot) (2-D, EOS 3).
 struct DevF32 {
   float* f[2][4];
   float* tab;                  // kTabEntries x tstride, entry-major

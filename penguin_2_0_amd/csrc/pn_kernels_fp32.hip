@@ -117,11 +117,11 @@ struct XTile {
 // column group grp.  Workgroup-uniform; starts with a barrier so that a workgroup can run
 // items back to back on the same LDS.  With CFL, returns in thread 0 the largest CFL term of
 // the cells it stored (the last sweep of a 3-D step).
-template <int K, int W, int Q, int G, int VISC, bool CFL>
+template <int K, int W, int Q, int G, int VISC, bool CFL, bool ADIA = false>
 __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, const DevParams& prm,
                                              const StepScalars* __restrict__ ss, int src, int grp, int l, int row,
-                                             XTile<K, W, Q, (G == kGeoRcyl ? 3 : 4)>& T) {
-  constexpr int NF = (G == kGeoRcyl) ? 3 : 4;
+                                             XTile<K, W, Q, (G == kGeoRcyl && !ADIA ? 3 : 4)>& T) {
+  constexpr int NF = (G == kGeoRcyl && !ADIA) ? 3 : 4;
   constexpr int WIN = 64 * K, REAL = WIN - 2 * kNpad, NC = W * Q;
   constexpr bool TH = (G == kGeoTheta);
   // staging: thread t moves the NIT cells c = t / NC + it (64 W / NC) of column t % NC, the
@@ -216,6 +216,10 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
       Wp.sth = tx.get(kTabSth);
       Wp.oms = tx.get(kTabOms);
     }
+    if constexpr (ADIA) {
+      Wp.qb = tx.get(kTabQb);
+      Wp.dqb = tx.get(kTabDqb);
+    }
     const float2 tr = f.trig[j0];
     Wp.ts = splat<float, K>(tr.x);
     Wp.tc = splat<float, K>(tr.y);
@@ -236,6 +240,7 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
         Wp.u[h] = f1[h];
         Wp.v[h] = rc[h] * f2[h];
         if constexpr (G == kGeoRsph) Wp.w[h] = Wp.rad[h] * f3[h];
+        if constexpr (ADIA) Wp.q[h] = f3[h];
       }
     }
     Para<float, K> P0;
@@ -248,7 +253,8 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
       P0.c3 = tx.get(kTabP3);
       P0.c4 = tx.get(kTabP4);
     }
-    ppmlr_pert<K, G, (G == kGeoRcyl ? 2 : 3)>(Wp, tx, P0, lim, dt, 0.0f, pl, prm.plnt > 0, prm.flat != 0, sqrt_gam);
+    ppmlr_pert<K, G, (G == kGeoRcyl ? 2 : 3), TabLane<K>, ADIA>(Wp, tx, P0, lim, dt, 0.0f, pl, prm.plnt > 0,
+                                                                 prm.flat != 0, sqrt_gam, 0.0f, false, (float)prm.gam);
     if constexpr (G == kGeoRcyl)
       if (VISC == 1 || (VISC == 2 && prm.visc)) visc_r_pert<K>(Wp, tx, lim, dt);
     const C<float, K> irc = tx.get(kTabIrc);
@@ -267,6 +273,7 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
           tile[1][q][c] = Wp.u[h];
           tile[2][q][c] = Wp.v[h] * irc[h];
           if constexpr (G == kGeoRsph) tile[3][q][c] = Wp.w[h] * ixc[h];
+          if constexpr (ADIA) tile[3][q][c] = Wp.q[h];
         }
       }
     }
@@ -310,15 +317,15 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
 }
 
 // one disk: one workgroup per item, XCD-contiguous
-template <int K, int W, int Q, int VISC>
+template <int K, int W, int Q, int VISC, bool ADIA>
 __global__ __launch_bounds__(64 * W) void k_sweep_x_f32(DevGrid g, DevF32 f, DevParams prm,
                                                         const StepScalars* __restrict__ ss, int src) {
-  __shared__ XTile<K, W, Q, 3> T;
+  __shared__ XTile<K, W, Q, ADIA ? 4 : 3> T;
   const int ngrp = (g.jmax + W * Q - 1) / (W * Q);
   const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
   const int grp = bid % ngrp, l = bid / ngrp;
   if (l >= f.nwin_x) return;   // workgroup-uniform, before any barrier
-  pencil_item<K, W, Q, kGeoRcyl, VISC, false>(g, f, prm, ss, src, grp, l, 0, T);
+  pencil_item<K, W, Q, kGeoRcyl, VISC, false, ADIA>(g, f, prm, ss, src, grp, l, 0, T);
 }
 
 // 3-D: the spherical radial sweep (item = theta row k, radial window, column group) and the
@@ -378,7 +385,7 @@ __global__ __launch_bounds__(64 * W) void k_sweep_x_f32_batch(const DiskDev* __r
 // Azimuthal sweep of window l of interior ring n (3-D: ring n % imax of theta row n / imax)
 // by one wavefront; returns the wave's maximum CFL term of the cells it stored (2-D).  The
 // sweep's radius is r_cyl; in 3-D it also carries v_theta and the planet's spherical forms.
-template <int K, int VISC, int NDIM>
+template <int K, int VISC, int NDIM, bool ADIA = false>
 __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, const DevParams& prm,
                                           const StepScalars* __restrict__ ss, int src, int n, int l) {
   constexpr int REAL = 64 * K - 2 * kNpad;
@@ -401,7 +408,8 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
   const float rc = T[kTabRc * ts], dxr = T[kTabDx0 * ts], rb = T[kTabRb * ts], pb = T[kTabPb * ts],
               cs2 = T[kTabCs2 * ts], cs = T[kTabCs * ts], vrot = T[kTabVrot * ts], rps = T[kTabRotPhS * ts],
               rpc = T[kTabRotPhC * ts];
-  float rsph = 0.0f, sth = 1.0f, oms = 0.0f;
+  float rsph = 0.0f, sth = 1.0f, oms = 0.0f, qb = 0.0f;
+  if constexpr (ADIA) qb = T[kTabQb * ts];
   if constexpr (NDIM == 3) {
     rsph = T[kTabRad * ts];
     sth = T[kTabSth * ts];
@@ -447,16 +455,19 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
     W.dLb[h] = 0.0f;
     W.sth[h] = sth;
     W.oms[h] = oms;
+    W.qb[h] = qb;
+    W.dqb[h] = 0.0f;
     dyc[h] = bdy.f(o);
     W.r[h] = bfr.f(o);
     W.u[h] = bfv.f(o);   // v_phi - v_bg
     W.v[h] = bfu.f(o);   // v_r
     if constexpr (NDIM == 3) W.w[h] = Buf(f.f[src][3] + base, rb4).f(o);
+    if constexpr (ADIA) W.q[h] = Buf(f.f[src][3] + base, rb4).f(o);
   }
   const float sqrt_gam = sqrtf((float)prm.gam);
   const Para<float, K> P0 = f.uniform_phi ? paraset_uniform<K>() : paraset(0, lim, W.dx0);
-  ppmlr_pert<K, kGeoPhi, NDIM>(W, TabLane<K>{}, P0, lim, dt, wres, pl, prm.plnt > 0, prm.flat != 0,
-                               sqrt_gam, rsph, f.uniform_phi != 0);
+  ppmlr_pert<K, kGeoPhi, NDIM, TabLane<K>, ADIA>(W, TabLane<K>{}, P0, lim, dt, wres, pl, prm.plnt > 0,
+                                                 prm.flat != 0, sqrt_gam, rsph, f.uniform_phi != 0, (float)prm.gam);
   if constexpr (NDIM == 2)
     if (VISC == 1 || (VISC == 2 && prm.visc)) visc_p_pert<K>(W, lim, dt, T[kTabNu * ts], rc);
 
@@ -473,9 +484,12 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
       du.put(W.v[h], o);
       dv.put(W.u[h], o);
       if constexpr (NDIM == 3) Buf(f.f[src ^ 1][3] + base, rb4).put(W.w[h], o);
-      // the next step's CFL term of this cell (timestep.cu:48-56)
-      const float t0 = (fabsf(W.v[h]) + csnd) * idx;
-      const float t1 = (fabsf(W.u[h] + cfl_off) + csnd) / (dyc[h] * rc);
+      if constexpr (ADIA) Buf(f.f[src ^ 1][3] + base, rb4).put(W.q[h], o);
+      // the next step's CFL term of this cell (timestep.cu:48-56); EOS 3: c_s^2 = gam q
+      float cn = csnd;
+      if constexpr (ADIA) cn = sqrtf((float)prm.gam * (qb + W.q[h]));
+      const float t0 = (fabsf(W.v[h]) + cn) * idx;
+      const float t1 = (fabsf(W.u[h] + cfl_off) + cn) / (dyc[h] * rc);
       m = fmaxf(m, fmaxf(t0, t1));
     }
   }
@@ -484,7 +498,7 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
 
 // one wavefront per (ring, window); with do_cfl each workgroup writes the maximum CFL term
 // of its waves to cflp[workgroup] (2-D)
-template <int K, int WPB, int VISC, int NDIM>
+template <int K, int WPB, int VISC, int NDIM, bool ADIA>
 __global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, DevParams prm,
                                                           const StepScalars* __restrict__ ss, double* cflp, int src,
                                                           int do_cfl) {
@@ -494,7 +508,7 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, D
   const int l = gw % f.nwin_y;
   const int n = gw / f.nwin_y;
   float m = 0.0f;
-  if (n < g.imax * g.kmax) m = y_pencil<K, VISC, NDIM>(g, f, prm, ss, src, n, l);   // wave-uniform
+  if (n < g.imax * g.kmax) m = y_pencil<K, VISC, NDIM, ADIA>(g, f, prm, ss, src, n, l);   // wave-uniform
   if (do_cfl) {   // kernel-uniform: every wave reaches the barrier
     if ((threadIdx.x & 63) == 0) shm[wv] = m;
     __syncthreads();
@@ -557,7 +571,9 @@ __global__ __launch_bounds__(256) void k_cfl_f32(DevGrid g, DevF32 f, DevParams 
     const long a = (long)s * g.jmax + j;
     const float* T = f.tab + s;
     const long ts = f.tstride;
-    const float csnd = sqrt_gam * T[kTabCs * ts];
+    // EOS 3: c_s^2 = gam q (2-D; field 3 carries q - q_b)
+    const float csnd = (prm.eos == 3) ? sqrtf((float)prm.gam * (T[kTabQb * ts] + f.f[src][3][a]))
+                                      : sqrt_gam * T[kTabCs * ts];
     const float cfl_off = (float)(f.vbg[s] - g.rot_v[s] - FrRot * g.radc[s]);
     const float t0 = (fabsf(f.f[src][1][a]) + csnd) / T[kTabDx0 * ts];
     const float t1 = (fabsf(f.f[src][2][a] + cfl_off) + csnd) / (f.dy[j] * T[kTabRc * ts]);
@@ -660,7 +676,8 @@ hipError_t sweep_x_k(const LaunchCtx& c, const DevF32& f0, int src) {
   dim3 grid((unsigned)(ngrp * f.nwin_x)), block(64 * W);
   // the same body as the batched kernels (viscosity switched by the parameters), so a disk
   // computes the same bits alone or in a batch
-  k_sweep_x_f32<K, W, Q, 2><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+  if (c.p.eos == 3) k_sweep_x_f32<K, W, Q, 0, true><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+  else k_sweep_x_f32<K, W, Q, 2, false><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
   return hipGetLastError();
 }
 
@@ -672,8 +689,10 @@ hipError_t sweep_y_k(const LaunchCtx& c, const DevF32& f0, int src, int do_cfl) 
   const long nblk = (waves + WPB - 1) / WPB;
   if (do_cfl && (nblk > c.ncflp || c.g.ndim == 3)) return hipErrorInvalidValue;
   dim3 grid((unsigned)nblk), block(64 * WPB);
-  if (c.g.ndim == 3) k_sweep_y_f32<K, WPB, 0, 3><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, 0);
-  else k_sweep_y_f32<K, WPB, 2, 2><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  if (c.g.ndim == 3) k_sweep_y_f32<K, WPB, 0, 3, false><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, 0);
+  else if (c.p.eos == 3)
+    k_sweep_y_f32<K, WPB, 0, 2, true><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  else k_sweep_y_f32<K, WPB, 2, 2, false><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
   return hipGetLastError();
 }
 

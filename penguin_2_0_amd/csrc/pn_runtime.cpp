@@ -110,8 +110,8 @@ int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision)
   for (int code : {p->nlft, p->nrgh})
     if ((code == 0 || code == 2) && g->imax < kNpad + 1)
       return fail(c, PN_ERR_ARG, "radial boundary code %d needs imax > %d", code, kNpad);
-  if (precision == PN_FP32 && p->eos != 0)
-    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for the isothermal disk (EOS 0)");
+  if (precision == PN_FP32 && p->eos != 0 && !(p->eos == 3 && p->ndim == 2 && !p->visc_flag))
+    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for EOS 0 and the inviscid 2-D EOS 3 disk");
   if (p->bary_flag != 0 && p->bary_flag != 1) return fail(c, PN_ERR_ARG, "bary_flag %d", p->bary_flag);
   if (precision == PN_FP32 && (p->nlft != 3 || p->nrgh != 3 || p->kill_flag))
     return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path keeps fixed radial ghost rings and no damping zones");
@@ -252,7 +252,7 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
       for (int f = 0; f < kNField && r2 == PN_OK; ++f) r2 = dalloc(c, &d.f[b][f], ncell);
   } else {
     for (int b = 0; b < 2 && r2 == PN_OK; ++b)
-      for (int f = 0; f < (p->ndim == 3 ? 4 : 3) && r2 == PN_OK; ++f) r2 = dalloc(c, &c->df.f[b][f], ncell);
+      for (int f = 0; f < (p->ndim == 3 || p->eos == 3 ? 4 : 3) && r2 == PN_OK; ++f) r2 = dalloc(c, &c->df.f[b][f], ncell);
   }
   double *x, *dx, *xc, *xvol, *z, *dz, *zc, *zvol, *radc, *cs2, *nu, *y, *dy, *yc, *yvol;
   pn_symdisk* val;
@@ -454,13 +454,22 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
         PN_HIP(c, hipMemcpyAsync(d.f[b][f], F[f].data(), cells * sizeof(double), hipMemcpyHostToDevice, st));
   } else {
     const bool d3 = (c->P.ndim == 3);
-    const int nf = d3 ? 4 : 3;
+    const bool adia = (c->P.eos == 3);
+    const int nf = (d3 || adia) ? 4 : 3;
     // backgrounds: the SymDisk record of an interior ring, a ghost ring's own (uniform) values
     std::vector<double> rbg(nslot);
     for (long s = 0; s < nslot; ++s) rbg[s] = F[FR][static_cast<size_t>(s) * J];
     for (int k = 0; k < K; ++k)
       for (int i = 0; i < I; ++i) rbg[slot_of(d, i, k)] = val[i + static_cast<long>(I) * k].r;
     c->h_rbg = rbg;
+    // EOS 3: the background pressure of the same records, and q_b = p_b / Sigma_b
+    std::vector<double> pbg(nslot), qbg(nslot);
+    for (long s = 0; s < nslot; ++s) pbg[s] = adia ? F[FP][static_cast<size_t>(s) * J] : scs2[s] * rbg[s];
+    if (adia)
+      for (int k = 0; k < K; ++k)
+        for (int i = 0; i < I; ++i) pbg[slot_of(d, i, k)] = val[i + static_cast<long>(I) * k].p;
+    for (long s = 0; s < nslot; ++s) qbg[s] = pbg[s] / rbg[s];
+    c->h_qbg = qbg;
     // Sigma - Sigma_b, v_r, v_phi - v_bg and (3-D) v_theta, rounded once
     std::vector<float> G[4];
     for (int f = 0; f < nf; ++f) G[f].resize(cells);
@@ -471,6 +480,7 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
         G[1][a] = static_cast<float>(F[FU][a]);
         G[2][a] = static_cast<float>(F[FV][a] - svbg[s]);
         if (d3) G[3][a] = static_cast<float>(F[FW][a]);
+        if (adia) G[3][a] = static_cast<float>(F[FP][a] / F[FR][a] - qbg[s]);
       }
     for (int b = 0; b < 2; ++b)
       for (int f = 0; f < nf; ++f)
@@ -483,7 +493,7 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
     std::vector<float> tab(static_cast<size_t>(kTabEntries) * ts, 0.0f);
     std::vector<double> pb(nslot), Lb(nslot);
     for (long s = 0; s < nslot; ++s) {
-      pb[s] = scs2[s] * rbg[s];
+      pb[s] = pbg[s];
       Lb[s] = sradc[s] * svbg[s];
     }
     for (long s = 0; s < nslot; ++s) {
@@ -528,6 +538,8 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
       q[kTabRotPhS] = std::sin(-0.25 * c->h_dy[0]);
       q[kTabRotPhC] = std::cos(-0.25 * c->h_dy[0]);
       q[kTabDvol0] = sxvol[s];
+      q[kTabQb] = qbg[s];
+      q[kTabDqb] = hn ? qbg[s + 1] - qbg[s] : 0.0;
       q[kTabRc] = sradc[s];
       q[kTabIrc] = 1.0 / sradc[s];
       q[kTabIs2] = 1.0 / (st_ * st_);
@@ -696,7 +708,7 @@ int pn_store_rings(pn_ctx* c, void* rings, size_t rings_bytes) {
   std::vector<double> F[kNField];
   for (int f = 0; f < kNField; ++f) F[f].assign(cells, 0.0);
   std::vector<float> G[4];
-  const int nf = (c->P.ndim == 3) ? 4 : 3;
+  const int nf = (c->P.ndim == 3 || c->P.eos == 3) ? 4 : 3;
   if (!is_f32(c)) {
     for (int f = 0; f < kNField; ++f)
       PN_HIP(c, hipMemcpyAsync(F[f].data(), d.f[c->cur][f], cells * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -720,10 +732,11 @@ int pn_store_rings(pn_ctx* c, void* rings, size_t rings_bytes) {
         const size_t a = static_cast<size_t>(s) * J + j;
         const double r = c->h_rbg[s] + static_cast<double>(G[0][a]);
         F[FR][a] = r;
-        F[FP][a] = r * c->h_cs2[s];   // isothermal closure (ppm.cu:139)
+        // isothermal closure (ppm.cu:139); EOS 3 carries q = p / Sigma
+        F[FP][a] = (c->P.eos == 3) ? r * (c->h_qbg[s] + static_cast<double>(G[3][a])) : r * c->h_cs2[s];
         F[FU][a] = static_cast<double>(G[1][a]);
         F[FV][a] = c->h_vbg[s] + static_cast<double>(G[2][a]);
-        if (nf == 4) F[FW][a] = static_cast<double>(G[3][a]);
+        if (c->P.ndim == 3) F[FW][a] = static_cast<double>(G[3][a]);
       }
   std::memset(rings, 0, rb * I * K);
   for (int k = 0; k < K; ++k)

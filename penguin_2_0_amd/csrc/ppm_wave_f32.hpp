@@ -208,9 +208,11 @@ struct WinP {
   C<float, K> u;   // sweep-normal velocity: v_r (radial), v_phi - v_bg (azimuthal), v_theta (theta)
   C<float, K> v;   // radial: l - L_b (l = r_cyl v_phi); azimuthal and theta: v_r
   C<float, K> w;   // 3-D only -- spherical radial: r v_theta; azimuthal: v_theta; theta: l - L_b
+  C<float, K> q;   // adiabatic (EOS 3, 2-D): p / rho - q_b
   // backgrounds (azimuthal sweep: uniform in the window, differences zero); dLb is the
   // difference of L_b along the sweep
   C<float, K> rb, drb, pb, dpb, Lb, dLb, cs, cs2;
+  C<float, K> qb, dqb;   // EOS 3: q_b = p_b / rho_b and its difference along the sweep
   // geometry; sth / oms: sin theta and 1 - sin theta of the cell centre (3-D planet terms);
   // vrot (azimuthal sweep) or rs / rcs (radial and theta sweeps: sin / cos of the planet's
   // drift against the slot in half a step, halved) for the planet's azimuth over the step
@@ -303,21 +305,33 @@ __device__ __forceinline__ void rot_theta(float st, float ct, float oms, float e
 // Eulerian grid (its values outside the window's paraset ranges never reach a parabola the
 // sweep keeps).  wres: the azimuthal grid's uniform angular rate.  rsph: the spherical
 // radius of an azimuthal 3-D ring.  On return cells [6, nmax-6) hold the update.
-template <int K, int G, int NDIM, class Tab>
+template <int K, int G, int NDIM, class Tab, bool ADIA = false>
 __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para<float, K>& P0, int nmax,
                                            float dt, float wres, const PlanetF& pl, bool planet, bool flat_on,
-                                           float sqrt_gam, float rsph = 0.0f, bool phi_rows = false) {
+                                           float sqrt_gam, float rsph = 0.0f, bool phi_rows = false,
+                                           float gam = 1.0f) {
   constexpr bool RAD = (G == kGeoRcyl || G == kGeoRsph);
+  static_assert(!ADIA || (NDIM == 2 && (G == kGeoRcyl || G == kGeoPhi)), "EOS 3 is built for 2-D disks");
   const float hdt = 0.5f * dt;
   const float fourthd = 1.3333333333333333f;
   const float smallr = 1.0e-15f;
+  // EOS 3 carries q = p / rho as its deviation dq from q_b; the pressure deviation is
+  // p - p_b = rho_b dq + q_b drho + drho dq (p_b = rho_b q_b)
+  C<float, K> dpc;
+  if constexpr (ADIA) {
+#pragma unroll
+    for (int h = 0; h < K; ++h) dpc[h] = W.rb[h] * W.q[h] + W.qb[h] * W.r[h] + W.r[h] * W.q[h];
+  }
 
   // ---- flatten (flatten.cu:7-40), on absolute pressure (relative-jump thresholds)
   C<float, K> flat = splat<float, K>(0.0f);
   if (flat_on) {
     C<float, K> pabs;
 #pragma unroll
-    for (int h = 0; h < K; ++h) pabs[h] = W.pb[h] + W.r[h] * W.cs2[h];
+    for (int h = 0; h < K; ++h) {
+      if constexpr (ADIA) pabs[h] = W.pb[h] + dpc[h];
+      else pabs[h] = W.pb[h] + W.r[h] * W.cs2[h];
+    }
     C<float, K> pm1 = prev1(pabs), pp1 = next1(pabs), pm2 = prev2(pabs), pp2 = next2(pabs);
     C<float, K> um1 = prev1(W.u), up1 = next1(W.u);
     C<float, K> steep;
@@ -412,8 +426,10 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
   C<float, K> csn, csfac;
 #pragma unroll
   for (int h = 0; h < K; ++h) {
-    // sqrt(gam p / rho) = sqrt(gam) c_s for the isothermal closure
-    float s = hdt * sqrt_gam * W.cs[h] / W.dx0[h];
+    // sqrt(gam p / rho) = sqrt(gam) c_s for the isothermal closure, sqrt(gam q) for EOS 3
+    float s;
+    if constexpr (ADIA) s = hdt * sqrtf(gam * (W.qb[h] + W.q[h])) / W.dx0[h];
+    else s = hdt * sqrt_gam * W.cs[h] / W.dx0[h];
     if constexpr (G == kGeoPhi || G == kGeoTheta) s = s / W.rad[h];
     csn[h] = s;
     csfac[h] = 1.0f - fourthd * s;
@@ -429,11 +445,16 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
   // interface c: left state = right edge of cell c-1, right state = left edge of cell c, as
   // relative jumps against the backgrounds of cell c-1
   C<float, K> epl, epr, erl, err, ul, ur;
-  if constexpr (G != kGeoPhi) {
+  if constexpr (G != kGeoPhi || ADIA) {
     C<float, K> pp, re, le;
 #pragma unroll
-    for (int h = 0; h < K; ++h) pp[h] = W.r[h] * W.cs2[h];
-    edges(parabola_off(0, nmax, flat, pp, W.dpb, P0), re, le);
+    for (int h = 0; h < K; ++h) {
+      if constexpr (ADIA) pp[h] = dpc[h];
+      else pp[h] = W.r[h] * W.cs2[h];
+    }
+    // along a ring p_b is one constant
+    if constexpr (G == kGeoPhi) edges(parabola(0, nmax, flat, pp, P0), re, le);
+    else edges(parabola_off(0, nmax, flat, pp, W.dpb, P0), re, le);
     C<float, K> q;
 #pragma unroll
     for (int h = 0; h < K; ++h) q[h] = re[h] / W.pb[h];
@@ -472,7 +493,35 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
   //   t1 = (sqrt(pl/rl) + sqrt(pr/rr))/2, t2 = -(ur-ul)/(2 t1), p* = sqrt(pl pr) e^t2,
   //   u* = ul - t1 log(p*/pl); here with pl = p_b(1+epl), ..., p* = p_b(1+em)
   C<float, K> umid, em;
-  if constexpr (G != kGeoPhi) {
+  if constexpr (ADIA) {
+    // gamma-law interface (riemann.cu:55-71, EOS 2's solver): the two-rarefaction estimate
+    // of p*, then u* from each side's shock or rarefaction curve.  Every quantity is taken
+    // relative to the background of cell c-1 (p_b, rho_b, c0^2 = gam q_b): a_s = c_s / c0,
+    // N / (2 c0) = (a_l + a_r)/2 - (gam-1)(ur-ul)/(4 c0), D / (2 c0 p_b^-z) =
+    // (a_l (1+epl)^-z + a_r (1+epr)^-z)/2, p* / p_b = (N / D)^(1/z) with every factor formed
+    // as 1 + (a small number)
+    const float z = 0.5f * (gam - 1.0f) / gam, iz = 1.0f / z, gm1 = gam - 1.0f, gp1 = gam + 1.0f;
+    const C<float, K> qbm = prev1(W.qb);
+#pragma unroll
+    for (int h = 0; h < K; ++h) {
+      const float lpl = log1p_acc(epl[h]), lpr = log1p_acc(epr[h]);
+      const float lrl = log1p_acc(erl[h]), lrr = log1p_acc(err[h]);
+      const float c0 = sqrtf(gam * qbm[h]);
+      const float hl = 0.5f * (lpl - lrl), hr = 0.5f * (lpr - lrr);   // log(a_l), log(a_r)
+      const float al1 = expm1_acc(hl), ar1 = expm1_acc(hr);
+      const float nm1 = 0.5f * (al1 + ar1) - 0.25f * gm1 * (ur[h] - ul[h]) / c0;
+      const float dm1 = 0.5f * (expm1_acc(hl - z * lpl) + expm1_acc(hr - z * lpr));
+      const float lm = iz * (log1p_acc(nm1) - log1p_acc(dm1));   // log(p* / p_b)
+      const float e_ = expm1_acc(lm);
+      float t1, t2;   // right and left wave terms of u* (riemann.cu:64-70)
+      if (e_ > epr[h]) t1 = (e_ - epr[h]) * sqrtf(2.0f * qbm[h] / ((1.0f + err[h]) * (gp1 * (1.0f + e_) + gm1 * (1.0f + epr[h]))));
+      else t1 = 2.0f * c0 * (1.0f + ar1) * expm1_acc(z * (lm - lpr)) / gm1;
+      if (e_ > epl[h]) t2 = (e_ - epl[h]) * sqrtf(2.0f * qbm[h] / ((1.0f + erl[h]) * (gp1 * (1.0f + e_) + gm1 * (1.0f + epl[h]))));
+      else t2 = 2.0f * c0 * (1.0f + al1) * expm1_acc(z * (lm - lpl)) / gm1;
+      em[h] = e_;
+      umid[h] = 0.5f * (ur[h] + ul[h]) + 0.5f * (t1 - t2);
+    }
+  } else if constexpr (G != kGeoPhi) {
     C<float, K> csm = prev1(W.cs);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
@@ -537,6 +586,13 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
                         dpbm = tx.get(kTabDpbm);
       C<float, K> is2;
       if constexpr (G == kGeoRsph) is2 = tx.get(kTabIs2);
+      // EOS 3: the pressure work through the left edge, (area of the moved edge) u* p*
+      C<float, K> Fl, Flp;
+      if constexpr (ADIA) {
+#pragma unroll
+        for (int h = 0; h < K; ++h) Fl[h] = (W.xa0[h] + 0.5f * dltx[h]) * umid[h] * (pbm[h] + pbm[h] * em[h]);
+        Flp = next1(Fl);
+      }
 #pragma unroll
       for (int h = 0; h < K; ++h) {
         const int c = cell_of(K, h);
@@ -557,11 +613,14 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
         const float g3 = -e * (3.0f - e * (6.0f - 10.0f * e));   // (1+e)^-3 - 1
         const float B = W.Lb[h] * W.Lb[h] * il;
         float fdiff = -(ixc * ixc) * g2 + (B + q1) * g3;
+        float f21 = fdiff;   // force at the moved centre - at the centre (EOS 3)
         if (planet) {
           float s, cc;
           rot_by2(W.ts[h], W.tc[h], W.rs[h], W.rcs[h], s, cc);
           if constexpr (G == kGeoRcyl) {
-            fdiff += planet_fr(pl, W.rad[h], W.ts[h], W.tc[h]) + planet_fr(pl, W.rad[h] + dxc, s, cc);
+            const float p1 = planet_fr(pl, W.rad[h], W.ts[h], W.tc[h]), p2 = planet_fr(pl, W.rad[h] + dxc, s, cc);
+            fdiff += p1 + p2;
+            f21 += p2 - p1;
           } else {
             // the star pulls with 1/(1+m): -1/r^2 + plm/r^2 at both centres
             fdiff += pl.m * ixc * ixc * (2.0f + g2) + planet_fr_sph(pl, W.rad[h], W.sth[h], W.oms[h], W.ts[h], W.tc[h]) +
@@ -597,20 +656,41 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
         }
         const float dvol = W.dvol0[h] + ddv;
         rL[h] = in ? (W.r[h] * W.dvol0[h] - W.rb[h] * ddv) / dvol : W.r[h];
+        if constexpr (ADIA) {
+          // EOS 3 (the line evolve.cu:146 leaves commented out): e = eps + u^2/2 gains
+          // -dt/dm (F_r - F_l) + dt (u f1 + u' f2)/2, eps' = e - u'^2/2.  With u' = u + du and
+          // du = dt (f1 + f2)/2 - dt/dm dp A the body forces' work cancels against the kinetic
+          // energy exactly, leaving
+          //   eps' - eps = -dt/dm ((F_r - F_l) - (u + du/2) dp A) + dt du (f2 - f1) / 4
+          const float du = dt * acc;
+          const float dpA = dpbm[h] * V0 + (dpbm[h] * dV + dp * (V0 + dV));
+          const float de = -dt / (rho * W.dvol0[h]) * ((Flp[h] - Fl[h]) - (W.u[h] + 0.5f * du) * dpA) +
+                           0.25f * dt * du * f21;
+          if (in) W.q[h] += (gam - 1.0f) * de;
+        }
         if (in) W.u[h] = W.u[h] + dt * acc;
       }
     } else if constexpr (G == kGeoPhi) {
+      // EOS 3: the pressure work through the left edge, u* p* (unit area factor)
+      C<float, K> Fl, Flp;
+      if constexpr (ADIA) {
+#pragma unroll
+        for (int h = 0; h < K; ++h) Fl[h] = umid[h] * (W.pb[h] + W.pb[h] * em[h]);
+        Flp = next1(Fl);
+      }
 #pragma unroll
       for (int h = 0; h < K; ++h) {
         const int c = cell_of(K, h);
         const bool in = (c >= 3 && c < nmax - 3);
-        float f = 0.0f;
+        float f = 0.0f, f21 = 0.0f;
         if (planet) {
           const float off = dltx[h] + 0.5f * Dl[h] + dt * (W.vrot[h] - pl.vy);
           float s, cc;
           rot_half(W.ts[h], W.tc[h], off, s, cc);
           if constexpr (NDIM == 2) {
-            f = planet_fphi(pl, W.rad[h], W.ts[h], W.tc[h]) + planet_fphi(pl, W.rad[h], s, cc);
+            const float p1 = planet_fphi(pl, W.rad[h], W.ts[h], W.tc[h]), p2 = planet_fphi(pl, W.rad[h], s, cc);
+            f = p1 + p2;
+            f21 = p2 - p1;
           } else {
             f = planet_fphi_sph(pl, rsph, W.sth[h], W.oms[h], W.ts[h], W.tc[h]) +
                 planet_fphi_sph(pl, rsph, W.sth[h], W.oms[h], s, cc);
@@ -622,6 +702,12 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
         const float ddv = Dl[h] * W.rad[h];
         const float dvol = W.dvol0[h] + ddv;
         rL[h] = in ? (W.r[h] * W.dvol0[h] - W.rb[h] * ddv) / dvol : W.r[h];
+        if constexpr (ADIA) {   // as in the radial sweep, with unit area factors
+          const float du = un - W.u[h];
+          const float de = -dt / (rho * W.dvol0[h]) * ((Flp[h] - Fl[h]) - (W.u[h] + 0.5f * du) * dpm) +
+                           0.25f * dt * du * f21;
+          if (in) W.q[h] += (gam - 1.0f) * de;
+        }
         if (in) W.u[h] = un;
       }
     } else {
@@ -749,26 +835,32 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
       if (c >= 6 && c < nmax - 6) X[h] = (X[h] * mass[h] + (fl[h] - flp[h])) * invm[h];
     }
   };
-  // l - L_b: fluxes relative to the cell's own L_b; the right edge's flux comes from the
-  // next cell's frame, L_b(c+1) = L_b(c) + dL_b
-  auto remap_bg = [&](C<float, K>& X) {
-    Parab<float, K> Q = parabola_off(3, nmax - 3, flat, X, W.dLb, P1);
+  // deviations from a background that varies along the sweep (l - L_b; EOS 3's q - q_b):
+  // fluxes relative to the cell's own background; the right edge's flux comes from the next
+  // cell's frame, b(c+1) = b(c) + dbg
+  auto remap_bg = [&](C<float, K>& X, const C<float, K>& dbg) {
+    Parab<float, K> Q = parabola_off(3, nmax - 3, flat, X, dbg, P1);
     C<float, K> fl;
-    face(Q, W.dLb, fl);
+    face(Q, dbg, fl);
 #pragma unroll
     for (int h = 0; h < K; ++h) fl[h] = fl[h] * fr[h];
     C<float, K> flp = next1(fl), frp = next1(fr);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
       const int c = cell_of(K, h);
-      if (c >= 6 && c < nmax - 6) X[h] = (X[h] * mass[h] + (fl[h] - (flp[h] + W.dLb[h] * frp[h]))) * invm[h];
+      if (c >= 6 && c < nmax - 6) X[h] = (X[h] * mass[h] + (fl[h] - (flp[h] + dbg[h] * frp[h]))) * invm[h];
     }
   };
   remap_plain(W.u);
-  if constexpr (RAD) remap_bg(W.v);
+  if constexpr (RAD) remap_bg(W.v, W.dLb);
   else remap_plain(W.v);
   if constexpr (G == kGeoRsph || (G == kGeoPhi && NDIM == 3)) remap_plain(W.w);
-  if constexpr (G == kGeoTheta) remap_bg(W.w);
+  if constexpr (G == kGeoTheta) remap_bg(W.w, W.dLb);
+  // EOS 3: q = p / rho, mass-weighted (remap.cu:171-176); q_b is uniform along a ring
+  if constexpr (ADIA) {
+    if constexpr (G == kGeoPhi) remap_plain(W.q);
+    else remap_bg(W.q, W.dqb);
+  }
 }
 
 // Explicit viscosity after the radial remap (device_viscosity_r, viscosity.cu:370-421) on

@@ -278,3 +278,40 @@ def test_f32_flags_vs_fp64(w):
         l1, li = rel(lab(a, f), lab(b, f))
         say(f"  {f}: L1 {l1:.3e} Linf {li:.3e}")
         assert l1 <= t1 and li <= tinf, (f, l1, li)
+
+
+# EOS 3 (the energy-consistent adiabatic step): Sigma, v_r, v_phi and q = p / Sigma in FP32
+# against the FP64 path; p is judged like Sigma, v_r (noise-driven, small) against its own
+# largest value
+EOS3_CASES = [flags_case("_eos3", eos=3, gam=1.4, init_p_iso=1),
+              flags_case("_eos3_adiabatic_ic", eos=3, gam=1.4),
+              flags_case("_eos3_strong_planet", eos=3, gam=1.4, init_p_iso=1, M_p=1e-3)]
+
+
+@pytest.mark.parametrize("w", EOS3_CASES, ids=lambda w: w.name)
+def test_f32_eos3_vs_fp64(w):
+    n = 50
+    say("case: fp32 eos3", w.name, n, "steps")
+    a, sa = run(w, abi.PN_FP32, n)
+    b, sb = run(w, abi.PN_FP64, n)
+    assert np.array_equal(a["rot_j"], b["rot_j"])
+    assert abs(sa.dt - sb.dt) <= 1e-5 * sb.dt
+    for f, t1, tinf in (("r", 1e-6, 1e-4), ("p", 1e-6, 1e-4), ("v", 1e-6, 1e-5), ("u", 1e-3, 1e-3)):
+        l1, li = rel(lab(a, f), lab(b, f))
+        say(f"  {f}: L1 {l1:.3e} Linf {li:.3e}")
+        assert l1 <= t1 and li <= tinf, (f, l1, li)
+
+
+def test_f32_eos3_every_variant_and_graph():
+    w = EOS3_CASES[2]
+    say("case: fp32 eos3 variants", w.name)
+    b, _ = run(w, abi.PN_FP64, 30)
+    for vx in range(4):
+        for vy in range(4):
+            a, _ = run(w, abi.PN_FP32, 30, variants=(vx, vy))
+            l1, _ = rel(lab(a, "p"), lab(b, "p"))
+            say(f"  variants ({vx}, {vy}): p L1 {l1:.3e}")
+            assert l1 <= 1e-7
+    e, _ = run(w, abi.PN_FP32, 6, graph=False)
+    g, _ = run(w, abi.PN_FP32, 6, graph=True)
+    assert np.array_equal(e.view(np.uint8), g.view(np.uint8))
