@@ -174,6 +174,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 leg of an FP32 run")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched parameter-sweep leg")
     ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--floor", action="store_true",
+                    help="also run the FP64 path from the FP32-rounded initial state: the flow's own "
+                         "sensitivity, the error floor no FP32 arithmetic can go below")
     ap.add_argument("--copy-probe", action="store_true", help="only run a plain device copy of known size")
     ap.add_argument("--eager", action="store_true",
                     help="launch every step's kernels directly instead of replaying a captured HIP graph "
@@ -243,6 +246,18 @@ def main(argv=None) -> int:
             "rot_j_equal": bool(np.array_equal(run["state"]["rot_j"], ref["state"]["rot_j"])),
             "dt_rel_diff": abs(run["st"].dt - ref["st"].dt) / ref["st"].dt,
         }
+        if a.floor:
+            d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+            for f in ("r", "p", "u", "v"):
+                d.rings[f] = d.rings[f].astype(np.float32).astype(np.float64)
+            with abi.Context(w.grid, w.params, abi.PN_FP64) as ctx:
+                ctx.load(d)
+                s = ctx.run_begin()
+                ctx.run(a.warmup + a.steps, s, graph=not a.eager)
+                fl = ctx.store()
+            out["accuracy_vs_fp64"]["floor"] = {
+                f"l1_{k}": rel_err(lab(fl, f), lab(ref["state"], f))[0]
+                for k, f in (("rho", "r"), ("vr", "u"), ("vphi", "v"), ("p", "p"))}
     if prec == abi.PN_FP32 and not a.no_batch:
         out["batched"] = batched_run(a.batch_steps, a.warmup, graph=not a.eager)
     if rank == 0 and not a.no_cpu:

@@ -532,6 +532,20 @@ __global__ __launch_bounds__(kStepThreads) void k_post_step(DevGrid g, DevParams
   post_step_body(g, prm, ss, cflp, ncfl, roll != 0);
 }
 
+// the largest of n CFL partials per workgroup, grid-strided: a disk whose fused CFL leaves
+// more partials than kSplitPartials has them folded into kCflChunks maxima first, so the
+// single-workgroup post-step reads few
+constexpr int kSplitPartials = 4096, kCflChunks = 256;
+__global__ __launch_bounds__(256) void k_max_chunks(const double* __restrict__ in, int n, double* __restrict__ out) {
+  __shared__ double sh[4];
+  double m = 0.0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) m = fmax(m, in[t]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
 __global__ __launch_bounds__(kStepThreads) void k_post_step_batch(const DiskDev* __restrict__ dd) {
   const DiskDev& D = dd[blockIdx.x];
   post_step_body(D.g, D.p, D.ss, D.cflp, D.ncfl_post, true);
@@ -613,7 +627,14 @@ hipError_t launch_post_step(const LaunchCtx& c, int ncfl) {
     const int n = c.g.imax * c.g.kmax;
     k_rotate<<<(n + 255) / 256, 256, 0, c.stream>>>(c.g);
   }
-  k_post_step<<<1, kStepThreads, 0, c.stream>>>(c.g, c.p, c.ss, c.cflp, ncfl, split ? 0 : 1);
+  const double* parts = c.cflp;
+  int np = ncfl;
+  if (ncfl > kSplitPartials) {   // the reduction scratch is free while a step runs
+    k_max_chunks<<<kCflChunks, 256, 0, c.stream>>>(c.cflp, ncfl, c.partials);
+    parts = c.partials;
+    np = kCflChunks;
+  }
+  k_post_step<<<1, kStepThreads, 0, c.stream>>>(c.g, c.p, c.ss, parts, np, split ? 0 : 1);
   return hipGetLastError();
 }
 

@@ -60,7 +60,8 @@ hipError_t launch_cfl_partials(const LaunchCtx& c, int src, int* n);
 // a second, grid-wide kernel.
 hipError_t launch_pre_step(const LaunchCtx& c, int ncfl, float2* trig, float* tab, float* tabz, int tstride);
 // one workgroup after the sweeps (plnt_flag < 2): roll, time and planet update, then dt
-// from ncfl partials when ncfl > 0 (disks of many slots: the roll from a grid-wide kernel)
+// from ncfl partials when ncfl > 0 (disks of many slots: the roll from a grid-wide kernel;
+// many partials: folded first by a grid-wide kernel into the reduction scratch)
 hipError_t launch_post_step(const LaunchCtx& c, int ncfl);
 // the same for every disk of a batch, one workgroup per disk (pre-step with ncfl 0; post-step
 // with each disk's ncfl_post)
