@@ -7,7 +7,8 @@ These cases put the pencil lengths where that decomposition changes shape:
   * a radial length of exactly 116 (one full window; the reference would skip it,
     ppm.cu:34, 48 -- this build and the oracle run it) and of 117 (a second window
     holding one kept cell);
-  * the smallest ring the halos allow (12 cells), and a 3-D disk only 4 cells high in theta.
+  * the smallest ring the halos allow (12 cells), and a 3-D disk only 4 cells high in theta;
+  * two of them again with the energy-consistent adiabatic step (EOS 3), P compared too.
 Bars as in the other parity files: 1e-11 relative L1 / 1e-8 Linf after 10 steps for FP64
 (device transcendentals against the host's), FP32 as in the FP32 tests.
 """
@@ -49,8 +50,15 @@ def disk3d(nr, nphi, nth):
 
 # (7 x 12 is smaller still, but there the method itself fails: the oracle's density hits
 # its floor and v_r turns non-finite by the second step)
+def eos3(w):
+    w.name += "_eos3"
+    return w
+
+
 EDGES = [configs.small(30, 60), configs.small(116, 144), configs.small(117, 233), configs.small(8, 12),
-         configs.small(232, 96, M_p=1e-3), disk3d(20, 40, 4)]
+         configs.small(232, 96, M_p=1e-3), disk3d(20, 40, 4),
+         eos3(configs.small(30, 60, eos=3, gam=1.4, init_p_iso=1)),
+         eos3(configs.small(117, 233, eos=3, gam=1.4, M_p=1e-3))]
 
 
 @pytest.mark.parametrize("w", EDGES, ids=lambda w: w.name)
@@ -70,6 +78,8 @@ def test_fp64_edges_vs_oracle(w):
     assert np.array_equal(g["rot_j"], o.rings["rot_j"])
     assert abs(gst.dt - st.dt) <= 1e-12 * st.dt
     fields = ("r", "u", "v", "w") if w.params.ndim == 3 else ("r", "u", "v")
+    if w.params.eos != 0:
+        fields += ("p",)
     tol = (1e-10, 1e-7) if w.params.ndim == 3 else (1e-11, 1e-8)
     for f in fields:
         l1, li = rel(g[f], o.rings[f])
@@ -96,10 +106,11 @@ def test_fp32_edges_vs_fp64(w):
             ctx.run(n, s32)
             a = ctx.store()
         assert np.array_equal(a["rot_j"], ref["rot_j"])
-        l1, li = rel(lab(a, "r"), lab(ref, "r"))
-        say(f"  variants ({vx}, {vy}): rho L1 {l1:.3e} Linf {li:.3e}")
         # FP32 rounding is relative to each cell's deviation from its ring background; on the
         # 8 x 12 grid the median deviation after 10 steps is 1.8e-2 of the background (2e-4 on
         # 30 x 60), so its bar is ten times wider
         bar = 1e-5 if w.grid.imax * w.grid.jmax < 200 else 1e-6
-        assert l1 <= bar and li <= 1e-4
+        for f in ("r", "p") if w.params.eos != 0 else ("r",):
+            l1, li = rel(lab(a, f), lab(ref, f))
+            say(f"  variants ({vx}, {vy}): {f} L1 {l1:.3e} Linf {li:.3e}")
+            assert l1 <= bar and li <= 1e-4, (f, l1, li)
