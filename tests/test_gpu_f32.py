@@ -285,7 +285,10 @@ def test_f32_flags_vs_fp64(w):
 # largest value
 EOS3_CASES = [flags_case("_eos3", eos=3, gam=1.4, init_p_iso=1),
               flags_case("_eos3_adiabatic_ic", eos=3, gam=1.4),
-              flags_case("_eos3_strong_planet", eos=3, gam=1.4, init_p_iso=1, M_p=1e-3)]
+              flags_case("_eos3_strong_planet", eos=3, gam=1.4, init_p_iso=1, M_p=1e-3),
+              flags_case("_eos3_fargo2", eos=3, gam=1.4, init_p_iso=1, fargo_flag=2),
+              flags_case("_eos3_inertial", eos=3, gam=1.4, init_p_iso=1, frrot_flag=0),
+              flags_case("_eos3_noflat", eos=3, gam=1.4, init_p_iso=1, flat_flag=0)]
 
 
 @pytest.mark.parametrize("w", EOS3_CASES, ids=lambda w: w.name)
