@@ -60,6 +60,23 @@ def test_f32_load_store_roundtrip():
     assert np.allclose(out["p"], out["r"] * cs2, rtol=1e-14, atol=0)
 
 
+def test_f32_eos3_load_store_roundtrip():
+    """EOS 3 carries q = P / Sigma as its deviation from q_b = P_b / Sigma_b: the round trip
+    rounds that deviation once, so P comes back within one FP32 ulp of Sigma (q - q_b)."""
+    w = configs.small(96, 288, eos=3, gam=1.4, M_p=1e-3)
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    with abi.Context(w.grid, w.params, abi.PN_FP32) as ctx:
+        ctx.load(d)
+        out = ctx.store()
+    val = d.val_array()
+    q, qb = d.rings["p"] / d.rings["r"], (val[:, 1] / val[:, 0])[:, None]
+    dr = np.abs(d.rings["r"] - val[:, 0:1])
+    assert np.all(np.abs(out["r"] - d.rings["r"]) <= 2.0 ** -24 * dr + 4e-16 * d.rings["r"]), "r"
+    # P' = Sigma' (q_b + dq'): the Sigma rounding enters through q, the dq rounding through Sigma
+    bound = out["r"] * 2.0 ** -24 * np.abs(q - qb) + q * 2.0 ** -24 * dr + 1e-15 * d.rings["p"]
+    assert np.all(np.abs(out["p"] - d.rings["p"]) <= bound), "p"
+
+
 @pytest.mark.parametrize("w,n,tol", [(configs.small(130, 250), 50, 1e-6), (configs.c1(), 100, 1e-6)],
                          ids=["disk2d_130x250", "c1"])
 def test_f32_vs_fp64(w, n, tol):
