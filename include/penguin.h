@@ -280,7 +280,7 @@ int pn_batch_set_variants(pn_batch* b, int var_x, int var_y);
  * batch, timed with HIP events on the batch's stream.  The state advances. */
 int pn_batch_time_kernels(pn_batch* b, int nrep, double* ms4);
 
-/* ---- snapshots (output.cu:48-120, global.cu:27-38, main.cu:246-262) ----- */
+/* ---- snapshots and output files (output.cu:48-120, global.cu:27-38, main.cu:113-124, 246-262) ---- */
 
 /* Write the imax*kmax rings of ctx as one raw binary_<label>_<NNNNN> file
  * (no header). */
@@ -288,6 +288,20 @@ int pn_save_snapshot(pn_ctx* ctx, const char* path);
 /* Read a raw snapshot back into ctx (restart, main.cu:246-262); ghost rings
  * and val must already be loaded.  PN_ERR_IO on a missing or short file. */
 int pn_load_snapshot(pn_ctx* ctx, const char* path);
+/* The run label (create_label, output.cu:57-88) that names every output file: the grid
+ * sizes, the c_s scale, the planet, the radial bounds, viscosity, EOS and frame tags, each
+ * number through the reference's int conversion.  EOS 3 is tagged _ad3.  Writes the
+ * NUL-terminated label into buf[n]; PN_ERR_ARG if it does not fit. */
+int pn_label(const pn_grid* g, const pn_params* p, char* buf, size_t n);
+/* frame_num (global.cu:27-38): num as five zero-padded digits ("00042"), the suffix of
+ * binary_<label>_ snapshot names (main.cu:124, 313, 403).  PN_ERR_ARG outside 0..99999,
+ * where the reference returns an empty suffix. */
+int pn_frame_name(int num, char* buf, size_t n);
+/* write_para_file (output.cu:90-107): the para_<label>.dat header the reader parses
+ * (read/post.cu:65-103), one value per line in default stream formatting; nDev is 1. */
+int pn_write_para(const pn_grid* g, const pn_params* p, const char* mainp, const char* label, const char* path);
+/* write_initial_file (output.cu:109-120): "xa dx" of each radial cell, one per line. */
+int pn_write_initial(const pn_grid* g, const pn_params* p, const char* path);
 
 #ifdef __cplusplus
 }

@@ -121,6 +121,10 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_damp_edges": (i, [vp, d]),
         "pn_save_snapshot": (i, [vp, C.c_char_p]),
         "pn_load_snapshot": (i, [vp, C.c_char_p]),
+        "pn_label": (i, [P(PnGrid), P(PnParams), C.c_char_p, sz]),
+        "pn_frame_name": (i, [i, C.c_char_p, sz]),
+        "pn_write_para": (i, [P(PnGrid), P(PnParams), C.c_char_p, C.c_char_p, C.c_char_p]),
+        "pn_write_initial": (i, [P(PnGrid), P(PnParams), C.c_char_p]),
         "pn_batch_create": (i, [i, P(PnGrid), P(PnParams), i, P(vp)]),
         "pn_batch_destroy": (i, [vp]),
         "pn_batch_last_error": (C.c_char_p, [vp]),
@@ -208,6 +212,37 @@ def lab_fields(rings: np.ndarray, grid: PnGrid) -> dict:
     if rc != PN_OK:
         raise PnError(rc, "pn_lab_fields")
     return out
+
+
+def label(grid: PnGrid, params: PnParams) -> str:
+    """The run label of every output file name (pn_label, create_label)."""
+    buf = C.create_string_buffer(256)
+    rc = load_library().pn_label(C.byref(grid), C.byref(params), buf, len(buf))
+    if rc != PN_OK:
+        raise PnError(rc, "pn_label")
+    return buf.value.decode()
+
+
+def frame_name(num: int) -> str:
+    """The five-digit snapshot suffix (pn_frame_name, frame_num)."""
+    buf = C.create_string_buffer(8)
+    rc = load_library().pn_frame_name(num, buf, len(buf))
+    if rc != PN_OK:
+        raise PnError(rc, "pn_frame_name")
+    return buf.value.decode()
+
+
+def write_para(grid: PnGrid, params: PnParams, mainp: str, label_: str, path) -> None:
+    rc = load_library().pn_write_para(C.byref(grid), C.byref(params), mainp.encode(), label_.encode(),
+                                      str(path).encode())
+    if rc != PN_OK:
+        raise PnError(rc, "pn_write_para")
+
+
+def write_initial(grid: PnGrid, params: PnParams, path) -> None:
+    rc = load_library().pn_write_initial(C.byref(grid), C.byref(params), str(path).encode())
+    if rc != PN_OK:
+        raise PnError(rc, "pn_write_initial")
 
 
 class Context:

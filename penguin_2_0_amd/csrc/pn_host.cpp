@@ -1,8 +1,12 @@
 // pn_host.cpp -- host-side pieces of the C ABI that touch no device: parameter defaults,
-// the hydr_ring byte layout, synthetic disks (init.cpp / disk_profile.cu) and the planet's
-// kick-drift-kick (planet.cu).
+// the hydr_ring byte layout, synthetic disks (init.cpp / disk_profile.cu), the planet's
+// kick-drift-kick (planet.cu) and the output files' names and headers (output.cu,
+// global.cu).
 #include <cmath>
 #include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "penguin.h"
@@ -430,6 +434,71 @@ void pn_kick_drift_kick(const pn_params* p, pn_body* b, double* FrRot, double dt
     *FrRot = (p->frrot_flag == 1) ? b->vy : 0.0;
     b->vy -= *FrRot;
   }
+}
+
+// ---- output names and headers (output.cu:57-120, global.cu:20-38, main.cu:113-124) ------
+
+// copies s with its terminating NUL into buf[n]; PN_ERR_ARG when it does not fit
+static int put_string(const std::string& s, char* buf, size_t n) {
+  if (!buf || s.size() + 1 > n) return PN_ERR_ARG;
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return PN_OK;
+}
+
+// the reference's int_to_string takes an int: a double argument is truncated towards zero
+static std::string istr(double v) { return std::to_string(static_cast<int>(v)); }
+
+int pn_label(const pn_grid* g, const pn_params* p, char* buf, size_t n) {
+  if (!g || !p) return PN_ERR_ARG;
+  std::string l = istr(g->imax) + "x" + istr(g->jmax) + "x" + istr(g->kmax) + "_c" + istr(std::round(p->sc_h / 0.001));
+  // (the opacity tags of opac_flag have no counterpart: that module is not part of this build)
+  if (p->plnt_flag == 2) l += "_pm" + istr(std::round(p->M_p / 0.000003)) + "_s" + istr(std::round(100.0 * p->rs_fac));
+  else if (p->plnt_flag == 1) l += "_p" + istr(std::round(p->M_p / 0.000003)) + "_s" + istr(std::round(100.0 * p->rs_fac));
+  l += "_i" + istr(p->xmin * 100.0) + "_o" + istr(p->xmax * 100.0);
+  if (p->visc_flag == 1) l += "_a" + istr(std::round(p->ss_alpha * 10000.0));
+  if (p->eos == 1) {
+    l += "_g" + istr(std::round(p->gam * 10.0));
+  } else if (p->eos == 0) {
+    if (p->p_alpha + 0.5 * p->p_beta - 1.5 == 1.5) l += "_flatvor";
+    if (p->p_beta == 0.0) l += "_iso";
+    else l += "_b" + istr(p->p_beta * 10.0);
+  } else if (p->eos == 2) {
+    l += "_ad";
+  } else {
+    l += "_ad3";   // EOS 3 has no tag in the reference
+  }
+  if (p->frrot_flag == 1) l += "_rot";
+  if (p->fargo_flag == 1) l += "_fargo";
+  return put_string(l, buf, n);
+}
+
+int pn_frame_name(int num, char* buf, size_t n) {
+  // five digits, zero padded; the reference returns an empty name past 99999 (and for a
+  // negative number), which would overwrite one file: refused here
+  if (num < 0 || num > 99999) return PN_ERR_ARG;
+  std::string c = std::to_string(num);
+  return put_string(std::string(5 - c.size(), '0') + c, buf, n);
+}
+
+int pn_write_para(const pn_grid* g, const pn_params* p, const char* mainp, const char* label, const char* path) {
+  if (!g || !p || !mainp || !label || !path) return PN_ERR_ARG;
+  std::ofstream f(path);
+  if (!f) return PN_ERR_IO;
+  // one GPU holds the whole disk: nDev = 1; default stream formatting, as ofstream << gives
+  f << g->imax << "\n" << g->jmax << "\n" << g->kmax << "\n" << 1 << "\n" << p->gam << "\n" << p->sc_h << "\n"
+    << p->tmovie << "\n" << mainp << "\n" << label << "\n" << p->grspx << "\n" << p->grspy << "\n" << p->grspz << "\n"
+    << p->xmin << " " << p->xmax << "\n" << p->ymin << " " << p->ymax << "\n" << p->zmin << " " << p->zmax << "\n";
+  return f.good() ? PN_OK : PN_ERR_IO;
+}
+
+int pn_write_initial(const pn_grid* g, const pn_params* p, const char* path) {
+  if (!g || !p || !path || g->imax < 1) return PN_ERR_ARG;
+  std::vector<double> xa(g->imax), xc(g->imax), dx(g->imax);
+  pn::make_grid(g->imax, p->xmin, p->xmax, p->grspx, xa.data(), xc.data(), dx.data());
+  std::ofstream f(path);
+  if (!f) return PN_ERR_IO;
+  for (int i = 0; i < g->imax; ++i) f << xa[i] << " " << dx[i] << "\n";
+  return f.good() ? PN_OK : PN_ERR_IO;
 }
 
 }  // extern "C"

@@ -221,3 +221,50 @@ def test_lab_fields_unrotates_like_the_reader():
             abi.lab_fields(bad, grid)
         with pytest.raises(abi.PnError):
             abi.lab_fields(d.rings[:-1], grid)
+
+
+def test_output_names_follow_the_reference_formulas():
+    """create_label (output.cu:57-88) and frame_num (global.cu:27-38), evaluated by hand."""
+    c1 = configs.c1()
+    # 256x768x1; round(0.05/0.001) = 50; round(1e-4/3e-6) = 33; round(100*0.6) = 60;
+    # int(0.4*100) = 40; int(2.5*100) = 250; EOS 0 with p_beta 1: int(10) -> _b10
+    assert abi.label(c1.grid, c1.params) == "256x768x1_c50_p33_s60_i40_o250_b10_rot_fargo"
+    # int_to_string takes an int: 0.29*100 = 28.999999999999996 is truncated to 28
+    assert "_i28_" in abi.label(c1.grid, c1.params.replace(xmin=0.29))
+    p = c1.params.replace(plnt_flag=2, visc_flag=1, ss_alpha=1e-3, eos=1, gam=1.4, frrot_flag=0, fargo_flag=2)
+    assert abi.label(c1.grid, p) == "256x768x1_c50_pm33_s60_i40_o250_a10_g14"
+    # p_alpha + p_beta/2 - 3/2 == 3/2 and p_beta == 0
+    assert abi.label(c1.grid, c1.params.replace(p_alpha=3.0, p_beta=0.0, plnt_flag=0)) == \
+        "256x768x1_c50_i40_o250_flatvor_iso_rot_fargo"
+    assert abi.label(c1.grid, c1.params.replace(eos=2, gam=1.4)).endswith("_o250_ad_rot_fargo")
+    assert abi.label(c1.grid, c1.params.replace(eos=3, gam=1.4)).endswith("_o250_ad3_rot_fargo")
+    assert [abi.frame_name(n) for n in (0, 7, 42, 1234, 99999)] == ["00000", "00007", "00042", "01234", "99999"]
+    for bad in (-1, 100000):
+        with pytest.raises(abi.PnError):
+            abi.frame_name(bad)
+
+
+def test_para_and_initial_files_read_back_as_the_reader_reads_them(tmp_path):
+    """write_para_file (output.cu:90-107) parsed in read_para's token order (read/post.cu:72-89);
+    write_initial_file (output.cu:109-120) against the oracle's radial grid."""
+    w = configs.c1()
+    lab_ = abi.label(w.grid, w.params)
+    abi.write_para(w.grid, w.params, "/work/run", lab_, tmp_path / "para.dat")
+    t = (tmp_path / "para.dat").read_text().split()
+    assert len(t) == 18
+    assert [int(x) for x in t[:4]] == [w.grid.imax, w.grid.jmax, w.grid.kmax, 1]
+    P = w.params
+    for got, want in zip((t[4], t[5], t[6]), (P.gam, P.sc_h, P.tmovie)):
+        assert abs(float(got) - want) <= 5e-6 * abs(want)   # 6 significant digits
+    assert t[7] == "/work/run" and t[8] == lab_
+    assert [int(x) for x in t[9:12]] == [P.grspx, P.grspy, P.grspz]
+    for got, want in zip(t[12:], (P.xmin, P.xmax, P.ymin, P.ymax, P.zmin, P.zmax)):
+        assert abs(float(got) - want) <= 5e-6 * abs(want)
+    abi.write_initial(w.grid, w.params, tmp_path / "initial.dat")
+    rows = np.loadtxt(tmp_path / "initial.dat")
+    n = w.grid.imax
+    xa, xc, dx = np.zeros(n), np.zeros(n), np.zeros(n)
+    oracle.lib().orc_grid(n, P.xmin, P.xmax, P.grspx, oracle.dptr(xa), oracle.dptr(xc), oracle.dptr(dx))
+    assert rows.shape == (n, 2)
+    np.testing.assert_allclose(rows[:, 0], xa, rtol=5e-6)
+    np.testing.assert_allclose(rows[:, 1], dx, rtol=5e-6)
