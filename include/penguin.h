@@ -113,6 +113,10 @@ typedef struct pn_run_state {
   double FrRot;            /* frame rotation rate */
   pn_body planet;
   int64_t ncycle;
+  double timem;            /* time since the last picture (main.cu:273) */
+  int32_t land;            /* in: 1 applies the endtime / tmovie landing of main.cu:352-366 */
+  int32_t event;           /* out: 0, or the landing the last step made: 1 tmovie (a picture
+                              is due), 2 endtime (the run is over) */
 } pn_run_state;
 
 typedef struct pn_ctx pn_ctx;
@@ -207,9 +211,15 @@ void pn_kick_drift_kick(const pn_params* p, pn_body* planet, double* FrRot, doub
  * (main.cu:270, 325-330). */
 int pn_run_begin(pn_ctx* ctx, pn_run_state* st);
 
-/* Advance nsteps steps of main.cu:343-387 (no endtime / tmovie landing, no
- * output), keeping dt, the planet and FrRot on the device between steps.
- * The step is launched as a captured HIP graph when graph != 0. */
+/* Advance nsteps steps of main.cu:343-387 (no output), keeping dt, the planet and FrRot on
+ * the device between steps.  The step is launched as a captured HIP graph when graph != 0.
+ * With st->land set, each step applies the landing of main.cu:352-366 on the device (a step
+ * that would pass endtime or the next picture time, timem + dt >= tmovie, is cut to land on
+ * it; after it timem restarts and dt returns to the CFL value), and pn_run returns right
+ * after such a step with st->event = 1 (picture due) or 2 (endtime reached) and st->ncycle
+ * counting the steps done.  To stop there without a device-side halt, the host enqueues at
+ * most as many steps as cannot reach a landing (dt grows by at most 1.1 per step) and reads
+ * the scalars between those batches. */
 int pn_run(pn_ctx* ctx, int nsteps, int graph, pn_run_state* st);
 
 /* Time the kernels of one step with HIP events on the context's stream: each of

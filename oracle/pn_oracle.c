@@ -1806,10 +1806,28 @@ void orc_damp(const pn_grid* g, const pn_params* P, void* rings, const pn_symdis
     }
 }
 
+/* the step loop of main.cu:343-387; with st->land, the endtime / tmovie landing of
+   main.cu:352-366 (a cut step, then timem = 0 and dt = old_dt, main.cu:408-411), stopping
+   after a step that landed with st->event = 1 (tmovie) or 2 (endtime) */
 void orc_run(const pn_grid* g, const pn_params* P, void* rings, const void* lft, const void* rgh,
              const void* udr, const void* top, const pn_symdisk* val, int nsteps, pn_run_state* st) {
+  st->event = 0;
   for (int s = 0; s < nsteps; s++) {
     double dt = orc_get_dt(g, P, rings, st->dt, st->FrRot);
+    double old_dt = dt;
+    int event = 0;
+    if (st->land) {
+      if (st->simtime + dt >= P->endtime) {
+        old_dt = dt;
+        dt = P->endtime - st->simtime;
+        event = 2;
+      }
+      if (st->timem + dt >= P->tmovie) {
+        old_dt = dt;
+        dt = P->tmovie - st->timem;
+        if (event == 0) event = 1;
+      }
+    }
     st->dt = dt;
     orc_index_shift(g, P, rings, val, dt, st->FrRot);
     if (P->plnt_flag > 0) orc_kick_drift_kick(P, &st->planet, &st->FrRot, 0.5 * dt, 1);
@@ -1819,11 +1837,18 @@ void orc_run(const pn_grid* g, const pn_params* P, void* rings, const void* lft,
     if (P->fargo_flag > 0) orc_rotate(g, rings);
     if (P->kill_flag) orc_damp(g, P, rings, val, dt);
     st->simtime += dt;
+    st->timem += dt;
     st->ncycle++;
     if (P->plnt_flag > 0) {
       orc_kick_drift_kick(P, &st->planet, &st->FrRot, dt, 0);
       orc_planet_forces(g, P, rings, &st->planet, st->FrRot, st->simtime);
       orc_kick_drift_kick(P, &st->planet, &st->FrRot, 0.5 * dt, 1);
+    }
+    if (event != 0) {
+      st->timem = 0.0;
+      st->dt = old_dt;
+      st->event = event;
+      return;
     }
   }
 }

@@ -57,7 +57,8 @@ class PnParams(C.Structure):
 
 class PnRunState(C.Structure):
     _fields_ = [("dt", C.c_double), ("simtime", C.c_double), ("FrRot", C.c_double),
-                ("planet", PnBody), ("ncycle", C.c_int64)]
+                ("planet", PnBody), ("ncycle", C.c_int64), ("timem", C.c_double), ("land", C.c_int32),
+                ("event", C.c_int32)]
 
 
 class PnError(RuntimeError):

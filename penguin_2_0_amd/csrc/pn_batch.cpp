@@ -275,6 +275,9 @@ int pn_batch_run(pn_batch* b, int nsteps, int graph, pn_run_state* states) {
   if (!b || !states || nsteps < 0) return PN_ERR_ARG;
   int cur, rc;
   if ((rc = common_cur(b, &cur)) != PN_OK) return rc;
+  for (int d = 0; d < ndisk(b); ++d)
+    if (states[d].land)
+      return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: the endtime / tmovie landing is built for single contexts", d);
   if (!b->tuned && (rc = pn_batch_tune(b, 3, nullptr, nullptr)) != PN_OK) return rc;
   if ((rc = upload_desc(b)) != PN_OK) return rc;
   // every disk's scalars, and its first dt from its state (the fused CFL supplies the rest)

@@ -35,6 +35,7 @@ struct pn_ctx {
   std::vector<double> h_y, h_dy, h_yc, h_yvol;
   // graph of one step (2-D) or two steps (3-D, the buffers alternate)
   hipGraphExec_t gexec = nullptr;
+  int gexec_cur = 0;   // the buffer the captured steps start from (and, whole, end on)
   // host copies of the fixed inputs, so a snapshot can be re-entered (restart)
   std::vector<char> h_lft, h_rgh, h_udr, h_top;
   std::vector<pn_symdisk> h_val;

@@ -50,6 +50,7 @@ struct DevParams {
   int eos, flat, plnt, visc, fargo, frrot, ngeomx, ngeomy, ngeomz, ndim, has_w;
   int nlft, nrgh, kill, bary;
   double xmin, xmax;
+  double endtime, tmovie;
 };
 
 // Scalars one step reads: device-resident so a whole step can be replayed as a graph.
@@ -66,6 +67,10 @@ struct StepScalars {
   int trig_ok;        // FP32 planet trig table holds the planet azimuth trig_y
   double trig_y;
   double fr_shift;    // FrRot before this step's opening half kick (what Index_Shift reads)
+  double timem;       // time since the last picture (main.cu:273)
+  double old_dt;      // the CFL dt a landing step cut (main.cu:354, 362)
+  int land;           // apply the endtime / tmovie landing (main.cu:352-366)
+  int event;          // the landing this step made: 0, 1 tmovie, 2 endtime
 };
 
 __host__ __device__ inline int slot_of(const DevGrid& g, int i, int k) {

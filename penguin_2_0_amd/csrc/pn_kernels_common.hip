@@ -280,6 +280,36 @@ __device__ double set_M_p(const DevParams& prm, double t) {   // planet.cu:28-34
   return prm.M_p;
 }
 
+// thread 0, before a step (main.cu:352-366): a step that would pass endtime or the next
+// picture time is cut to land on it; old_dt keeps the dt it replaced (the second cut, when
+// both apply, keeps the first cut's dt, as the reference does)
+__device__ void land_step(const DevParams& prm, StepScalars* ss) {
+  double dt = ss->dt;
+  int ev = 0;
+  if (ss->simtime + dt >= prm.endtime) {
+    ss->old_dt = dt;
+    dt = prm.endtime - ss->simtime;
+    ev = 2;
+  }
+  if (ss->timem + dt >= prm.tmovie) {
+    ss->old_dt = dt;
+    dt = prm.tmovie - ss->timem;
+    if (ev == 0) ev = 1;
+  }
+  ss->dt = dt;
+  ss->event = ev;
+}
+
+// thread 0, after a step's closing kick (main.cu:379, 408-411): timem advances by the step's
+// dt; after a landing it restarts and dt returns to the value the landing cut
+__device__ void land_close(StepScalars* ss, double dt) {
+  ss->timem += dt;
+  if (ss->event != 0) {
+    ss->timem = 0.0;
+    ss->dt = ss->old_dt;
+  }
+}
+
 // after the sweeps: time advance, drift, forces and the closing half kick
 // (main.cu:378-387, planet.cu:54-93).  red[1], red[2] hold the mode 8 / 9 reductions
 // when the planet moves.
@@ -317,6 +347,7 @@ __global__ void k_planet_forces_kick(DevParams prm, StepScalars* ss, int do_kick
     b.vy += ss->FrRot + dt * b.fy / b.x / b.x;
     ss->FrRot = (prm.frrot == 1) ? b.vy : 0.0;
     b.vy -= ss->FrRot;
+    land_close(ss, ss->dt);   // the end of the step
   }
 }
 
@@ -415,6 +446,7 @@ __device__ __forceinline__ void pre_step_body(DevGrid g, const DevParams& prm, S
     if (threadIdx.x == 0) finalize_dt(prm, ss, mx);
   }
   if (threadIdx.x == 0) {
+    if (ss->land) land_step(prm, ss);
     s_dt = ss->dt;
     s_fr = ss->FrRot;
     ss->fr_shift = s_fr;
@@ -524,6 +556,7 @@ __device__ __forceinline__ void post_step_body(DevGrid g, const DevParams& prm, 
     ss->FrRot = (prm.frrot == 1) ? b.vy : 0.0;
     b.vy -= ss->FrRot;
   }
+  land_close(ss, dt);
   if (ncfl > 0) finalize_dt(prm, ss, mx);
 }
 

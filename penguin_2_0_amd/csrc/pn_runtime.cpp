@@ -328,6 +328,8 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
   q.bary = p->bary_flag;
   q.xmin = p->xmin;
   q.xmax = p->xmax;
+  q.endtime = p->endtime;
+  q.tmovie = p->tmovie;
   StepScalars s0{};
   rc = upload_scalars(c, s0);
   *out = c;
@@ -1061,24 +1063,24 @@ int pn_run_begin(pn_ctx* c, pn_run_state* st) {
   return pn_index_shift(c, st->dt, st->FrRot);
 }
 
-int pn_run(pn_ctx* c, int nsteps, int graph, pn_run_state* st) {
-  if (!c || !st || nsteps < 0) return PN_ERR_ARG;
-  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
-  int rc;
-  if (is_f32(c) && !c->tuned && (rc = pn_tune(c, 3, nullptr, nullptr)) != PN_OK) return rc;
-  StepScalars h{};
-  h.dt = st->dt;
-  h.FrRot = st->FrRot;
-  h.simtime = st->simtime;
-  h.planet = st->planet;
-  h.ncycle = st->ncycle;
-  if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
-  const bool fused = fused_cfl(c);
-  // with the CFL taken in the sweep epilogue, dt of the first step comes from the state
-  if (fused && nsteps > 0 && (rc = launch_cfl_any(c, c->cur)) != PN_OK) return rc;
+}  // extern "C"
+
+namespace pn {
+namespace rt {
+namespace {
+
+// n steps from the current buffer: captured-graph replays for whole units of per_graph steps
+// (a 3-D step flips the buffer, so its graph holds two), the rest launched directly.  A
+// graph is captured from the buffer the state is in and captured again when an odd number of
+// direct 3-D steps has moved the state to the other one.
+int run_steps(pn_ctx* c, int nsteps, int graph) {
   const int per_graph = (c->P.ndim == 3) ? 2 : 1;
-  int done = 0;
+  int done = 0, rc;
   if (graph && nsteps >= per_graph) {
+    if (c->gexec && c->gexec_cur != c->cur) {
+      hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+    }
     if (!c->gexec) {
       hipGraph_t gr;
       PN_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1092,6 +1094,7 @@ int pn_run(pn_ctx* c, int nsteps, int graph, pn_run_state* st) {
       e = hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0);
       hipGraphDestroy(gr);
       if (e != hipSuccess) return fail(c, PN_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      c->gexec_cur = c->cur;
     }
     while (nsteps - done >= per_graph) {
       PN_HIP(c, hipGraphLaunch(c->gexec, c->stream));
@@ -1102,6 +1105,63 @@ int pn_run(pn_ctx* c, int nsteps, int graph, pn_run_state* st) {
   for (; done < nsteps; ++done)
     if ((rc = launch_step(c, cur)) != PN_OK) return rc;
   c->cur = cur;
+  return PN_OK;
+}
+
+// steps that cannot reach a landing from dt0: each dt is at most 1.1 times the one before,
+// so n steps span at most dt0 (1.1 + ... + 1.1^n); room is the time to the nearer of the
+// next picture and endtime (a small margin covers the rounding of the device's sums)
+int steps_before_landing(double dt0, double room, int cap) {
+  int n = 0;
+  double t = 0.0, d = dt0;
+  while (n < cap) {
+    d *= 1.1;
+    t += d;
+    if (!(t < 0.999 * room)) break;
+    ++n;
+  }
+  return n;
+}
+
+}  // namespace
+}  // namespace rt
+}  // namespace pn
+
+extern "C" {
+
+int pn_run(pn_ctx* c, int nsteps, int graph, pn_run_state* st) {
+  if (!c || !st || nsteps < 0) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  int rc;
+  if (is_f32(c) && !c->tuned && (rc = pn_tune(c, 3, nullptr, nullptr)) != PN_OK) return rc;
+  StepScalars h{};
+  h.dt = st->dt;
+  h.FrRot = st->FrRot;
+  h.simtime = st->simtime;
+  h.planet = st->planet;
+  h.ncycle = st->ncycle;
+  h.timem = st->timem;
+  h.land = st->land ? 1 : 0;
+  h.event = 0;
+  if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
+  const bool fused = fused_cfl(c);
+  // with the CFL taken in the sweep epilogue, dt of the first step comes from the state
+  if (fused && nsteps > 0 && (rc = launch_cfl_any(c, c->cur)) != PN_OK) return rc;
+  if (!h.land) {
+    if ((rc = rt::run_steps(c, nsteps, graph)) != PN_OK) return rc;
+  } else {
+    // the landing is applied on the device every step; the host only stops after a step
+    // that landed, reading the scalars between batches that cannot reach one
+    int done = 0;
+    while (done < nsteps) {
+      const double room = std::fmin(c->P.tmovie - h.timem, c->P.endtime - h.simtime);
+      const int n = std::max(1, rt::steps_before_landing(h.dt, room, nsteps - done));
+      if ((rc = rt::run_steps(c, n, graph)) != PN_OK) return rc;
+      done += n;
+      if ((rc = download_scalars(c, &h)) != PN_OK) return rc;
+      if (h.event != 0 || h.bad) break;
+    }
+  }
   if ((rc = download_scalars(c, &h)) != PN_OK) return rc;
   // fused: the record already holds the next step's dt; report the last one used
   st->dt = (fused && nsteps > 0) ? h.dt_prev : h.dt;
@@ -1109,6 +1169,8 @@ int pn_run(pn_ctx* c, int nsteps, int graph, pn_run_state* st) {
   st->simtime = h.simtime;
   st->planet = h.planet;
   st->ncycle = h.ncycle;
+  st->timem = h.timem;
+  st->event = h.event;
   if (h.bad) return fail(c, PN_ERR_NUMERIC, "non-finite or non-positive dt during the run");
   return PN_OK;
 }

@@ -335,3 +335,43 @@ def test_f32_eos3_every_variant_and_graph():
     e, _ = run(w, abi.PN_FP32, 6, graph=False)
     g, _ = run(w, abi.PN_FP32, 6, graph=True)
     assert np.array_equal(e.view(np.uint8), g.view(np.uint8))
+
+
+def test_f32_landing_at_the_fp64_steps():
+    """The landing on picture times takes the same steps on the FP32 path as on the FP64 one."""
+    w = configs.small(96, 288)
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    with abi.Context(w.grid, w.params, abi.PN_FP64) as c:
+        c.load(d)
+        s = c.run_begin()
+        dt0 = c.get_dt(s.dt, s.FrRot)
+    w.params.tmovie = 7.5 * dt0
+    out = {}
+    for prec in (abi.PN_FP64, abi.PN_FP32):
+        with abi.Context(w.grid, w.params, prec) as c:
+            c.load(d)
+            s = c.run_begin()
+            s.land = 1
+            c.run(100, s)
+            out[prec] = (s.event, s.ncycle, s.simtime, c.store())
+    say("case: fp32 landing", out[abi.PN_FP64][:3], out[abi.PN_FP32][:3])
+    assert out[abi.PN_FP32][:2] == out[abi.PN_FP64][:2] == (1, 8)
+    assert abs(out[abi.PN_FP32][2] - w.params.tmovie) <= 1e-12 * w.params.tmovie
+    l1, _ = rel(lab(out[abi.PN_FP32][3], "r"), lab(out[abi.PN_FP64][3], "r"))
+    assert l1 <= 1e-6
+
+
+@pytest.mark.parametrize("prec", [abi.PN_FP64, abi.PN_FP32], ids=["fp64", "fp32"])
+def test_3d_graph_after_an_odd_number_of_steps(prec):
+    """A 3-D step flips the state buffer and a captured graph holds two steps: after a call
+    that ended on an odd step, the next graph replay must start from the other buffer."""
+    w = disk3d(24, 72, 12)
+    a, _ = run(w, prec, 7, graph=False)
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    with abi.Context(w.grid, w.params, prec) as ctx:
+        ctx.load(d)
+        st = ctx.run_begin()
+        ctx.run(3, st, graph=True)
+        ctx.run(4, st, graph=True)
+        b = ctx.store()
+    assert np.array_equal(a.view(np.uint8), b.view(np.uint8))

@@ -201,3 +201,27 @@ def test_roll_invariance_bitwise():
             la = np.roll(a[f][n], sa)
             lb = np.roll(b[f][n], sb)
             assert np.array_equal(la, lb), (n, f)
+
+
+def test_landing_on_picture_times_matches_the_oracle():
+    """The endtime / tmovie landing (main.cu:352-366) applied on the device: the same steps
+    are cut, at the same times, as in the oracle's loop, and pn_run stops after each."""
+    w = configs.small(48, 144)
+    o0 = oracle.OracleDisk(w.grid, w.params, w.seed, w.noise)
+    s0 = o0.run_begin()
+    w.params.tmovie = 3.5 * o0.get_dt(s0.dt, s0.FrRot)
+    ctx, o = pair(w)
+    say("case: landing", w.name, "tmovie", w.params.tmovie)
+    st = o.run_begin()
+    st.land = 1
+    with ctx:
+        gst = ctx.run_begin()
+        gst.land = 1
+        for k in (1, 2):
+            o.run(100, st)
+            ctx.run(100, gst, graph=True)
+            say(f"  picture {k}: ncycle {gst.ncycle} vs {st.ncycle}, simtime {gst.simtime!r} vs {st.simtime!r}")
+            assert gst.event == st.event == 1 and gst.ncycle == st.ncycle
+            assert gst.simtime == st.simtime and gst.timem == st.timem == 0.0
+            assert abs(gst.dt - st.dt) <= 1e-12 * st.dt
+            fields_close(ctx.store(), o.rings, 1e-11, 1e-8, f"picture {k}")

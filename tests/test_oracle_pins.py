@@ -688,3 +688,52 @@ def test_barycentric_forces_are_minus_grad_of_the_potential():
     # the heliocentric radial force is the barycentric one plus the frame's acceleration
     # terms, which do not vanish: the two frames differ
     assert abs(grav(0, 0, 1, 1.3, 0.2, 0.0) - grav(0, 0, 0, 1.3, 0.2, 0.0)) > 1e-6
+
+
+# ---------------------------------------------------------------------------------------
+# the endtime / tmovie landing of the step loop (main.cu:352-366, 408-411)
+
+def _landing_disk(**over):
+    w = configs.small(32, 96, **over)
+    o = oracle.OracleDisk(w.grid, w.params, w.seed, w.noise)
+    st = o.run_begin()
+    return w, o, st
+
+
+def test_landing_cuts_the_step_that_would_pass_a_picture_time():
+    w, o, st = _landing_disk()
+    dt0 = o.get_dt(st.dt, st.FrRot)
+    w.params.tmovie = 3.5 * dt0   # the run's own parameter record, read by orc_run
+    o.params = w.params
+    st.land = 1
+    o.run(100, st)
+    # the CFL dt barely changes over four steps: steps 1-3 are whole, the 4th is cut to land
+    assert st.event == 1 and st.ncycle == 4
+    assert abs(st.simtime - w.params.tmovie) <= 4 * np.spacing(w.params.tmovie)
+    assert st.timem == 0.0
+    assert 0.9 * dt0 < st.dt < 1.2 * dt0   # dt is back at the CFL value the landing cut
+    # the next call runs to the next picture, 2 tmovie
+    o.run(100, st)
+    assert st.event == 1
+    assert abs(st.simtime - 2.0 * w.params.tmovie) <= 8 * np.spacing(w.params.tmovie)
+
+
+def test_landing_on_endtime_wins_over_a_picture_at_the_same_time():
+    w, o, st = _landing_disk()
+    dt0 = o.get_dt(st.dt, st.FrRot)
+    w.params.endtime = 2.5 * dt0
+    w.params.tmovie = 2.5 * dt0
+    o.params = w.params
+    st.land = 1
+    o.run(100, st)
+    assert st.event == 2 and st.ncycle == 3
+    assert abs(st.simtime - w.params.endtime) <= 4 * np.spacing(w.params.endtime)
+
+
+def test_without_landing_the_loop_runs_every_step():
+    w, o, st = _landing_disk()
+    dt0 = o.get_dt(st.dt, st.FrRot)
+    w.params.tmovie = 1.5 * dt0
+    o.params = w.params
+    o.run(5, st)
+    assert st.event == 0 and st.ncycle == 5 and st.simtime > 4.0 * dt0
