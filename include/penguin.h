@@ -17,7 +17,7 @@
  *           save_cells / copy_bfile         include/output.h:26         (src/output/output.cu:48-55, main.cu:37-42)
  *           init_planet / planet_forces /
  *           kick_drift_kick                 include/planet.h:7-9        (src/planet/planet.cu:36-93)
- *   driver  the step loop of kernel()       main.cu:343-441
+ *   loop    the step loop of kernel()       main.cu:343-441
  */
 #ifndef PENGUIN_H
 #define PENGUIN_H

@@ -1498,7 +1498,7 @@ void orc_sweepy(const pn_grid* g, const pn_params* P, void* rings, double dt, co
   }
 }
 
-/* theta sweep (3-D).  The reference has no driver for it (main.cu:376 calls
+/* theta sweep (3-D).  The reference has no caller for it (main.cu:376 calls
    an undefined bundle_sweep3D); it is assembled from the theta pieces the
    reference does hold: get_fz_bound / get_fz (forces.cu:298-372), the theta
    branches of states / evolve / remap (states.cu:29-39, evolve.cu:26-29,

@@ -15,6 +15,21 @@
 
 namespace pn {
 
+// Occupancy.  A SIMD's 512 VGPRs per lane hold eight waves of at most 64 VGPRs, six of 80,
+// five of 96 and four of 128; amdgpu_waves_per_eu asks the allocator to fit a count, spilling
+// what does not fit to scratch.
+// - LDS-staged pencils: the allocator left alone settles them at 99-122 (four waves).  A
+//   workgroup of four waves puts one on each SIMD, so a fifth wave per SIMD is a fifth
+//   workgroup per CU, which the LDS must hold too; for those shapes the kernels ask for five
+//   (pencil_waves).  It costs a few spills per pencil (radial sweep 1024 x 3072: two) and took
+//   that sweep from 75.1 to 73.3 us.  Workgroups of eight or sixteen waves gain nothing short
+//   of six or eight waves per SIMD, so they ask for nothing.
+// - Azimuthal windows of 128 cells: 66 VGPRs for a 2-D isothermal disk (seven waves; eight at
+//   64 for four spills) and 86 for the batch (five waves; six at 80 for six spills).  Longer
+//   windows and the 3-D and EOS 3 forms spill tens of values at the next step down, so they
+//   keep what the allocator gives (a minimum of one wave asks for nothing).
+#define PN_WAVES(n) __attribute__((amdgpu_waves_per_eu(n, 8)))
+
 namespace {
 
 __device__ __forceinline__ int xcd_swizzle(int bid, int nblk) {
@@ -112,6 +127,13 @@ struct XTile {
   int srot[WIN];            // rot_j of the window's slots
   float red[W];
 };
+
+// waves per SIMD a pencil kernel asks for: five where a fifth workgroup of four waves fits the
+// CU's 160 KiB of LDS, else no request
+template <int K, int W, int Q, int NF>
+constexpr int pencil_waves() {
+  return (W == 4 && 5 * sizeof(XTile<K, W, Q, NF>) <= 160 * 1024) ? 5 : 1;
+}
 
 // one work item: window l of the pencils of row `row` (radial: theta row k; theta: ring i),
 // column group grp.  Workgroup-uniform; starts with a barrier so that a workgroup can run
@@ -318,7 +340,7 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
 
 // one disk: one workgroup per item, XCD-contiguous
 template <int K, int W, int Q, int VISC, bool ADIA>
-__global__ __launch_bounds__(64 * W) void k_sweep_x_f32(DevGrid g, DevF32 f, DevParams prm,
+__global__ __launch_bounds__(64 * W) PN_WAVES((pencil_waves<K, W, Q, ADIA ? 4 : 3>())) void k_sweep_x_f32(DevGrid g, DevF32 f, DevParams prm,
                                                         const StepScalars* __restrict__ ss, int src) {
   __shared__ XTile<K, W, Q, ADIA ? 4 : 3> T;
   const int ngrp = (g.jmax + W * Q - 1) / (W * Q);
@@ -332,7 +354,7 @@ __global__ __launch_bounds__(64 * W) void k_sweep_x_f32(DevGrid g, DevF32 f, Dev
 // theta sweep (item = ring i, theta window, column group; with do_cfl it is the step's last
 // sweep and writes one CFL partial per workgroup)
 template <int K, int W, int Q, int G>
-__global__ __launch_bounds__(64 * W) void k_sweep_pencil3_f32(DevGrid g, DevF32 f, DevParams prm,
+__global__ __launch_bounds__(64 * W) PN_WAVES((pencil_waves<K, W, Q, 4>())) void k_sweep_pencil3_f32(DevGrid g, DevF32 f, DevParams prm,
                                                               const StepScalars* __restrict__ ss, double* cflp,
                                                               int src, int do_cfl) {
   __shared__ XTile<K, W, Q, 4> T;
@@ -366,7 +388,7 @@ __device__ __forceinline__ int disk_of(const int* __restrict__ base, int ndisk, 
 // of 8), each mapped through the XCD permutation so that one XCD walks a contiguous run of
 // items (neighbouring column groups share L2 lines).  base: ndisk + 1 item offsets.
 template <int K, int W, int Q>
-__global__ __launch_bounds__(64 * W) void k_sweep_x_f32_batch(const DiskDev* __restrict__ dd,
+__global__ __launch_bounds__(64 * W) PN_WAVES((pencil_waves<K, W, Q, 3>())) void k_sweep_x_f32_batch(const DiskDev* __restrict__ dd,
                                                               const DevF32* __restrict__ dfs,
                                                               const int* __restrict__ base, int ndisk, int src) {
   __shared__ XTile<K, W, Q, 3> T;
@@ -499,7 +521,7 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
 // one wavefront per (ring, window); with do_cfl each workgroup writes the maximum CFL term
 // of its waves to cflp[workgroup] (2-D)
 template <int K, int WPB, int VISC, int NDIM, bool ADIA>
-__global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, DevParams prm,
+__global__ __launch_bounds__(64 * WPB) PN_WAVES((K == 2 && NDIM == 2 && !ADIA) ? 8 : 1) void k_sweep_y_f32(DevGrid g, DevF32 f, DevParams prm,
                                                           const StepScalars* __restrict__ ss, double* cflp, int src,
                                                           int do_cfl) {
   __shared__ float shm[WPB];
@@ -522,7 +544,7 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32(DevGrid g, DevF32 f, D
 // many disks, persistent (as k_sweep_x_f32_batch): item = one workgroup's WPB waves of one
 // disk; each item writes its CFL partial to its disk's cflp[item - base[d]]
 template <int K, int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_sweep_y_f32_batch(const DiskDev* __restrict__ dd,
+__global__ __launch_bounds__(64 * WPB) PN_WAVES(K == 2 ? 6 : 1) void k_sweep_y_f32_batch(const DiskDev* __restrict__ dd,
                                                               const DevF32* __restrict__ dfs,
                                                               const int* __restrict__ base, int ndisk, int src) {
   __shared__ float shm[WPB];

@@ -50,10 +50,19 @@ __device__ __forceinline__ Planet load_planet(const StepScalars* ss, int bary) {
   return p;
 }
 
+// Waves per SIMD the radial sweep asks for: the isothermal cylindrical sweep needs 172 VGPRs
+// (viscous: 176), two waves; at 168 it holds three, with no spill (viscous: four scratch
+// accesses), and runs 1024 x 3072 in 0.248 instead of 0.274 ms.  The other closures and the
+// 3-D sweeps spill tens of values at 168 and ran slower, so they keep what the allocator gives.
+template <class CF>
+constexpr int x_waves() {
+  return (CF::EOS == 0 && CF::GX == 1) ? 3 : 1;
+}
+
 // ---------------------------------------------------------------------------------------
 // radial sweep (sweepx, ppm.cu:26-160): pencil = lab column j0 of theta layer k
 template <class CF>
-__global__ __launch_bounds__(256) void k_sweep_x(DevGrid g, DevParams prm, const StepScalars* __restrict__ ss,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(x_waves<CF>(), 8))) void k_sweep_x(DevGrid g, DevParams prm, const StepScalars* __restrict__ ss,
                                                  int src) {
   const int lane = threadIdx.x & 63;
   const int gw = xcd_swizzle(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
@@ -229,7 +238,7 @@ __global__ __launch_bounds__(256) void k_sweep_y(DevGrid g, DevParams prm, const
 
 // ---------------------------------------------------------------------------------------
 // theta sweep (3-D): pencil = (ring i, lab column j0) over theta layers.  The reference has
-// no driver for it (main.cu:376); it is assembled from the theta pieces it holds
+// no caller for it (main.cu:376); it is assembled from the theta pieces it holds
 // (forces.cu:298-372, states.cu:29-39, evolve.cu:26-29, 63-67, 110-124, remap.cu:50-54,
 // init.cpp:415-541).  Sweep components: u <- v_theta (w), v <- v_r (u),
 // w <- v_phi * r sin(theta) (the angular momentum get_fz's centrifugal term expects).
