@@ -50,13 +50,13 @@ __device__ __forceinline__ Planet load_planet(const StepScalars* ss, int bary) {
   return p;
 }
 
-// Waves per SIMD the radial sweep asks for: the isothermal cylindrical sweep needs 172 VGPRs
-// (viscous: 176), two waves; at 168 it holds three, with no spill (viscous: four scratch
-// accesses), and runs 1024 x 3072 in 0.248 instead of 0.274 ms.  The other closures and the
-// 3-D sweeps spill tens of values at 168 and ran slower, so they keep what the allocator gives.
+// Waves per SIMD the radial sweep asks for: the inviscid isothermal cylindrical sweep needs
+// 172 VGPRs, two waves; at 168 it holds three with no spill and runs 1024 x 3072 in 0.248
+// instead of 0.274 ms.  Every other form spills at 168 (the viscous one four accesses, the
+// other closures and the 3-D sweeps tens), so they keep what the allocator gives.
 template <class CF>
 constexpr int x_waves() {
-  return (CF::EOS == 0 && CF::GX == 1) ? 3 : 1;
+  return (CF::EOS == 0 && CF::GX == 1 && CF::VISC == 0) ? 3 : 1;
 }
 
 // ---------------------------------------------------------------------------------------
