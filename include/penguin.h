@@ -210,6 +210,12 @@ void pn_kick_drift_kick(const pn_params* p, pn_body* planet, double* FrRot, doub
 /* Prime a run: st->dt = endtime, planet initialised, FARGO primed
  * (main.cu:270, 325-330). */
 int pn_run_begin(pn_ctx* ctx, pn_run_state* st);
+/* Prime a run resumed from picture startat (restart, main.cu:244-330): st->dt = endtime,
+ * simtime = tmovie * startat, timem = 0, the planet initialised afresh with its mass ramp
+ * at that simtime (the snapshots hold no planet), and no priming Index_Shift: the rings
+ * keep the FARGO state they were saved with (main.cu:330).  Load the picture with
+ * pn_load_snapshot first.  PN_ERR_ARG for startat < 0. */
+int pn_run_restart(pn_ctx* ctx, int startat, pn_run_state* st);
 
 /* Advance nsteps steps of main.cu:343-387 (no output), keeping dt, the planet and FrRot on
  * the device between steps.  The step is launched as a captured HIP graph when graph != 0.

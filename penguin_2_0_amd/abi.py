@@ -115,6 +115,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_planet_forces": (i, [vp, P(PnBody), d, d]),
         "pn_kick_drift_kick": (None, [P(PnParams), P(PnBody), P(d), d, i]),
         "pn_run_begin": (i, [vp, P(PnRunState)]),
+        "pn_run_restart": (i, [vp, i, P(PnRunState)]),
         "pn_run": (i, [vp, i, i, P(PnRunState)]),
         "pn_totals": (i, [vp, P(d), P(d)]),
         "pn_copy_bandwidth": (i, [sz, i, P(d)]),
@@ -341,6 +342,12 @@ class Context:
     def run_begin(self) -> PnRunState:
         st = PnRunState()
         self._chk(self.lib.pn_run_begin(self.h, C.byref(st)), "pn_run_begin")
+        return st
+
+    def run_restart(self, startat: int) -> PnRunState:
+        """Prime a run resumed from picture startat (simtime = tmovie * startat)."""
+        st = PnRunState()
+        self._chk(self.lib.pn_run_restart(self.h, startat, C.byref(st)), "pn_run_restart")
         return st
 
     def run(self, nsteps: int, st: PnRunState, graph: bool = True) -> PnRunState:

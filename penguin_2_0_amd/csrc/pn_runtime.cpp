@@ -1051,16 +1051,37 @@ int pn_planet_forces(pn_ctx* c, pn_body* b, double FrRot, double simtime) {
   return PN_OK;
 }
 
-int pn_run_begin(pn_ctx* c, pn_run_state* st) {
-  if (!c || !st) return PN_ERR_ARG;
+}  // extern "C"
+
+namespace {
+
+// main.cu:270-330: dt = endtime, simtime, timem = 0, a fresh planet and its forces at simtime,
+// and on a fresh start the priming Index_Shift (a restart keeps the FARGO state of the rings
+// it read back)
+int prime_run(pn_ctx* c, double simtime, bool restart, pn_run_state* st) {
   if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
   std::memset(st, 0, sizeof *st);
   st->dt = c->P.endtime;
-  st->simtime = 0.0;
+  st->simtime = simtime;
   pn_init_planet(&c->P, &st->planet, &st->FrRot);
   int rc = pn_planet_forces(c, &st->planet, st->FrRot, st->simtime);
-  if (rc != PN_OK) return rc;
+  if (rc != PN_OK || restart) return rc;
   return pn_index_shift(c, st->dt, st->FrRot);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pn_run_begin(pn_ctx* c, pn_run_state* st) {
+  if (!c || !st) return PN_ERR_ARG;
+  return prime_run(c, 0.0, false, st);
+}
+
+int pn_run_restart(pn_ctx* c, int startat, pn_run_state* st) {
+  if (!c || !st) return PN_ERR_ARG;
+  if (startat < 0) return fail(c, PN_ERR_ARG, "startat %d", startat);
+  return prime_run(c, c->P.tmovie * (double)startat, true, st);
 }
 
 }  // extern "C"

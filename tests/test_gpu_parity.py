@@ -225,3 +225,51 @@ def test_landing_on_picture_times_matches_the_oracle():
             assert gst.simtime == st.simtime and gst.timem == st.timem == 0.0
             assert abs(gst.dt - st.dt) <= 1e-12 * st.dt
             fields_close(ctx.store(), o.rings, 1e-11, 1e-8, f"picture {k}")
+
+
+def test_restart_from_a_picture_resumes_the_run(tmp_path):
+    """pn_run_restart (main.cu:244-330) after pn_load_snapshot: the resumed run continues the
+    unbroken one -- the same ring bytes step for step (the first resumed dt is the CFL value,
+    as the unbroken run's is once dt has settled) -- with simtime counted from tmovie *
+    startat, no priming Index_Shift, and the planet's mass ramp taken at the resumed time."""
+    w = configs.small(48, 144)
+    w.params.tmovie = 0.25
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    snap = str(tmp_path / "binary_restart_00003")
+    say("case: restart", w.name, "startat 3, tmovie", w.params.tmovie)
+    with abi.Context(w.grid, w.params, abi.PN_FP64) as a:
+        a.load(d)
+        sa = a.run_begin()
+        a.run(5, sa, graph=True)
+        a.save_snapshot(snap)
+        t5 = sa.simtime
+        a.run(4, sa, graph=True)
+        ra = a.store()
+    with abi.Context(w.grid, w.params, abi.PN_FP64) as b:
+        b.load(d)   # ghost rings and the background record; the rings come from the picture
+        b.load_snapshot(snap)
+        sb = b.run_restart(3)
+        assert sb.simtime == 3 * w.params.tmovie and sb.timem == 0.0 and sb.dt == w.params.endtime
+        assert sb.ncycle == 0 and sb.FrRot == sa.FrRot
+        b.run(4, sb, graph=True)
+        rb = b.store()
+        say(f"  simtime {sb.simtime!r}, unbroken run shifted {3 * w.params.tmovie + (sa.simtime - t5)!r}")
+        assert np.array_equal(ra.view(np.uint8), rb.view(np.uint8))
+        assert abs(sb.simtime - (3 * w.params.tmovie + (sa.simtime - t5))) <= 1e-12 * sb.simtime
+        with pytest.raises(abi.PnError) as e:
+            b.run_restart(-1)
+        assert e.value.code == abi.PN_ERR_ARG
+    # a growing planet: the mass is the ramp's at tmovie * startat, not at 0
+    w.params.t_growth = 10.0
+    with abi.Context(w.grid, w.params, abi.PN_FP64) as c:
+        c.load(d)
+        st = c.run_restart(2)
+        want = []
+        for t in (0.0, 2 * w.params.tmovie):
+            body, frot = abi.PnBody(), C.c_double()
+            c.lib.pn_init_planet(C.byref(w.params), C.byref(body), C.byref(frot))
+            assert c.lib.pn_planet_forces(c.h, C.byref(body), frot.value, t) == abi.PN_OK
+            want.append(body.m)
+        say(f"  planet mass at restart {st.planet.m!r}, ramp at 0 / at 0.5: {want}")
+        assert st.planet.m == want[1] and want[1] > want[0]
+

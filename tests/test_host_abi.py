@@ -127,6 +127,9 @@ def test_argument_errors_before_any_device_call():
                           d.rings.ctypes.data_as(C.c_void_p), d.lft.ctypes.data_as(C.c_void_p),
                           d.rgh.ctypes.data_as(C.c_void_p), None, None, d.val)
     assert rc == abi.PN_ERR_ARG
+    # priming a run needs a context and a state record
+    assert lib.pn_run_begin(None, None) == abi.PN_ERR_ARG
+    assert lib.pn_run_restart(None, 2, None) == abi.PN_ERR_ARG
 
 
 def _gpu_visible() -> bool:
