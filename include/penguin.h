@@ -225,7 +225,8 @@ int pn_run_restart(pn_ctx* ctx, int startat, pn_run_state* st);
  * after such a step with st->event = 1 (picture due) or 2 (endtime reached) and st->ncycle
  * counting the steps done.  To stop there without a device-side halt, the host enqueues at
  * most as many steps as cannot reach a landing (dt grows by at most 1.1 per step) and reads
- * the scalars between those batches. */
+ * the scalars between those batches.  PN_ERR_ARG when st->land is set and no time is left
+ * (timem >= tmovie or simtime >= endtime): that step would be cut to dt <= 0. */
 int pn_run(pn_ctx* ctx, int nsteps, int graph, pn_run_state* st);
 
 /* Time the kernels of one step with HIP events on the context's stream: each of

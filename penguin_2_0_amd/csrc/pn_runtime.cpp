@@ -1153,6 +1153,10 @@ extern "C" {
 int pn_run(pn_ctx* c, int nsteps, int graph, pn_run_state* st) {
   if (!c || !st || nsteps < 0) return PN_ERR_ARG;
   if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  // a landing needs time left before both marks, or its step would be cut to dt <= 0
+  if (st->land && !(st->timem < c->P.tmovie && st->simtime < c->P.endtime))
+    return fail(c, PN_ERR_ARG, "landing with no time left: timem %g of tmovie %g, simtime %g of endtime %g",
+                st->timem, c->P.tmovie, st->simtime, c->P.endtime);
   int rc;
   if (is_f32(c) && !c->tuned && (rc = pn_tune(c, 3, nullptr, nullptr)) != PN_OK) return rc;
   StepScalars h{};

@@ -225,6 +225,11 @@ def test_landing_on_picture_times_matches_the_oracle():
             assert gst.simtime == st.simtime and gst.timem == st.timem == 0.0
             assert abs(gst.dt - st.dt) <= 1e-12 * st.dt
             fields_close(ctx.store(), o.rings, 1e-11, 1e-8, f"picture {k}")
+        # no time left before endtime: the step would be cut to dt <= 0, so the call refuses
+        gst.simtime = w.params.endtime
+        with pytest.raises(abi.PnError) as e:
+            ctx.run(1, gst)
+        assert e.value.code == abi.PN_ERR_ARG
 
 
 def test_restart_from_a_picture_resumes_the_run(tmp_path):
