@@ -716,12 +716,12 @@ hipError_t launch_sum_partials(const LaunchCtx& c, int nb, int stride, double* o
   return hipGetLastError();
 }
 
-hipError_t launch_planet_after_sweep(const LaunchCtx& c, int src) {
+hipError_t launch_planet_after_sweep(const LaunchCtx& c, int src, const DevF32* f32) {
   k_planet_drift<<<1, 64, 0, c.stream>>>(c.p, c.ss);
   if (c.p.plnt == 2) {
-    hipError_t e = launch_reduce(c, src, 8, 0.0, 1);
+    hipError_t e = f32 ? f32_reduce(c, *f32, src, 8, 0.0, 1) : launch_reduce(c, src, 8, 0.0, 1);
     if (e != hipSuccess) return e;
-    e = launch_reduce(c, src, 9, 0.0, 2);
+    e = f32 ? f32_reduce(c, *f32, src, 9, 0.0, 2) : launch_reduce(c, src, 9, 0.0, 2);
     if (e != hipSuccess) return e;
   }
   k_planet_forces_kick<<<1, 64, 0, c.stream>>>(c.p, c.ss, 1);

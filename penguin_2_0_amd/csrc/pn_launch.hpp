@@ -42,7 +42,10 @@ hipError_t launch_rotate(const LaunchCtx& c);
 hipError_t launch_cfl(const LaunchCtx& c, int src);              // partials into cflp, then ss->dt
 hipError_t launch_reduce(const LaunchCtx& c, int src, int mode, double exclude, int slot);  // -> ss->red[slot]
 hipError_t launch_totals(const LaunchCtx& c, int src, double* out2);  // out2: device, mass and angmom
-hipError_t launch_planet_after_sweep(const LaunchCtx& c, int src);  // simtime, drift, forces, half kick
+// simtime, drift, the disk's forces on the planet (modes 8 / 9; from the FP32 state when f32 is
+// given), closing half kick
+struct DevF32;
+hipError_t launch_planet_after_sweep(const LaunchCtx& c, int src, const DevF32* f32 = nullptr);
 hipError_t launch_dt_finalize(const LaunchCtx& c, int n);           // dt = min(1.1 dt, courant / max of n cflp)
 hipError_t launch_sum_partials(const LaunchCtx& c, int nb, int stride, double* out);
 // radial ghost rings of buffer src from the interior under the boundary codes (FP64 state;

@@ -182,7 +182,6 @@ int launch_sweeps(pn_ctx* c, int& cur, int do_cfl, bool prep, bool roll) {
 int launch_step(pn_ctx* c, int& cur) {
   LaunchCtx L = lctx(c);
   const bool fused = fused_cfl(c);
-  if (is_f32(c) && c->P.plnt_flag == 2) return fail(c, PN_ERR_UNSUPPORTED, "moving planet on the FP32 path");
   int ncfl = 0, rc;
   if (!fused) {
     if (is_f32(c)) PN_LAUNCH(c, "cfl", f32_cfl(L, c->df, cur, &ncfl));
@@ -197,7 +196,7 @@ int launch_step(pn_ctx* c, int& cur) {
   // damping zones: ring-uniform relaxation, which commutes with the integer roll
   if (c->P.kill_flag) PN_LAUNCH(c, "damp", launch_damp(L, cur));
   if (post) PN_LAUNCH(c, "post step", launch_post_step(L, fused ? f32_fused_partials(c) : 0));
-  else PN_LAUNCH(c, "planet update", launch_planet_after_sweep(L, cur));
+  else PN_LAUNCH(c, "planet update", launch_planet_after_sweep(L, cur, is_f32(c) ? &c->df : nullptr));
   return PN_OK;
 }
 

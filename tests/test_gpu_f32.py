@@ -279,6 +279,8 @@ FLAG_CASES = [flags_case("_fargo2", fargo_flag=2), flags_case("_inertial", frrot
               flags_case("_noflat", flat_flag=0), flags_case("_noplanet", plnt_flag=0),
               flags_case("_nofargo", fargo_flag=0), flags_case("_growing_planet", t_growth=5.0),
               flags_case("_strong_planet", M_p=1e-3),
+              # a planet that moves under the disk's forces, reduced from the FP32 state
+              flags_case("_moving_planet", plnt_flag=2, M_p=1e-3),
               # more than 2048 slots: Index_Shift, the table rows and the roll over the whole grid
               configs.small(2048, 116, M_p=1e-3)]
 
@@ -295,6 +297,11 @@ def test_f32_flags_vs_fp64(w):
         l1, li = rel(lab(a, f), lab(b, f))
         say(f"  {f}: L1 {l1:.3e} Linf {li:.3e}")
         assert l1 <= t1 and li <= tinf, (f, l1, li)
+    if w.params.plnt_flag == 2:
+        dp = max(abs(sa.planet.x - sb.planet.x), abs(sa.planet.y - sb.planet.y),
+                 abs(sa.planet.vx - sb.planet.vx), abs(sa.planet.vy - sb.planet.vy))
+        say(f"  planet: x {sb.planet.x!r} y {sb.planet.y!r}, largest difference {dp:.3e}")
+        assert sb.planet.vx != 0.0 and dp <= 5e-9   # the disk has moved the planet; both paths alike
 
 
 # EOS 3 (the energy-consistent adiabatic step): Sigma, v_r, v_phi and q = p / Sigma in FP32
@@ -305,7 +312,8 @@ EOS3_CASES = [flags_case("_eos3", eos=3, gam=1.4, init_p_iso=1),
               flags_case("_eos3_strong_planet", eos=3, gam=1.4, init_p_iso=1, M_p=1e-3),
               flags_case("_eos3_fargo2", eos=3, gam=1.4, init_p_iso=1, fargo_flag=2),
               flags_case("_eos3_inertial", eos=3, gam=1.4, init_p_iso=1, frrot_flag=0),
-              flags_case("_eos3_noflat", eos=3, gam=1.4, init_p_iso=1, flat_flag=0)]
+              flags_case("_eos3_noflat", eos=3, gam=1.4, init_p_iso=1, flat_flag=0),
+              flags_case("_eos3_moving_planet", eos=3, gam=1.4, init_p_iso=1, plnt_flag=2, M_p=1e-3)]
 
 
 @pytest.mark.parametrize("w", EOS3_CASES, ids=lambda w: w.name)
