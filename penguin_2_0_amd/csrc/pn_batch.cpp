@@ -176,6 +176,8 @@ int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int pr
       return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: a moving planet needs its force reductions between sweeps", d);
     if (params[d].eos != 0)
       return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches are built for the isothermal closure (EOS 0)", d);
+    if (params[d].kill_flag)
+      return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches run no damping zones", d);
     pn_ctx* c = nullptr;
     const int rc = pn_create(&grids[d], &params[d], precision, &c);
     if (c) b->disks.push_back(c);

@@ -116,5 +116,7 @@ hipError_t f32_sweep_y_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs
 hipError_t f32_cfl(const LaunchCtx& c, const DevF32& f, int src, int* n);
 hipError_t f32_reduce(const LaunchCtx& c, const DevF32& f, int src, int mode, double exclude, int slot);
 hipError_t f32_totals(const LaunchCtx& c, const DevF32& f, int src, double* out2);
+// damping zones at the radial edges with ss->dt (the FP64 path's launch_damp on the FP32 state)
+hipError_t f32_damp(const LaunchCtx& c, const DevF32& f, int src);
 
 }  // namespace pn

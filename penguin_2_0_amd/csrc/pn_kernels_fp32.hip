@@ -647,6 +647,47 @@ __global__ __launch_bounds__(256) void k_reduce_f32(DevGrid g, DevF32 f, DevPara
   if (threadIdx.x == 0) partials[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
+// Damping zones (kill_wave.cu:3-43) on the FP32 state: the rings within sc_h/2 of xmin or xmax
+// relax towards their background record by the fraction k = dt r^-1.5 (distance into the zone) /
+// (sc_h/2), as the FP64 path does.  The background record is the FP32 state's own background,
+// so Sigma - Sigma_b, v_phi - v_bg and v_theta scale by 1 - k and v_r relaxes towards the
+// record's v_r; EOS 3 relaxes p and Sigma and carries q = p / Sigma - q_b again, in FP64.
+// Thread: one cell of an interior ring.
+__global__ void k_damp_f32(DevGrid g, DevF32 f, DevParams prm, const StepScalars* __restrict__ ss, int src) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= g.jmax) return;
+  const int n = blockIdx.y;   // ring i + imax k
+  const int s = slot_of(g, n % g.imax, n / g.imax);
+  const double d_kill = 0.5 * prm.sc_h;
+  const double inner = prm.xmin + d_kill, outer = prm.xmax - d_kill;
+  const double r = g.xc[s];
+  double k;
+  if (r < inner) k = (inner - r) / d_kill;
+  else if (r > outer) k = (r - outer) / d_kill;
+  else return;
+  k *= ss->dt * pow(r, -1.5);
+  const pn_symdisk V = g.val[n];
+  const long a = (long)s * g.jmax + j;
+  float* const F0 = f.f[src][0];
+  float* const F1 = f.f[src][1];
+  float* const F2 = f.f[src][2];
+  const double dr = F0[a];
+  F0[a] = (float)((1.0 - k) * dr);
+  F1[a] = (float)((double)F1[a] + k * (V.u - (double)F1[a]));
+  F2[a] = (float)((1.0 - k) * (double)F2[a]);
+  if (g.ndim == 3 || prm.eos == 3) {   // the fourth field exists
+    float* const F3 = f.f[src][3];
+    if (g.ndim == 3) {
+      F3[a] = (float)((1.0 - k) * (double)F3[a]);
+    } else {
+      const double qb = V.p / V.r, rho = V.r + dr;
+      const double p = rho * (qb + (double)F3[a]);
+      const double p1 = p + k * (V.p - p), rho1 = V.r + (1.0 - k) * dr;
+      F3[a] = (float)(p1 / rho1 - qb);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_totals_f32(DevGrid g, DevF32 f, int src, double* partials) {
   __shared__ double sh[8];
   const long ncell = (long)g.imax * g.kmax * g.jmax;
@@ -895,6 +936,12 @@ hipError_t f32_reduce(const LaunchCtx& c, const DevF32& f, int src, int mode, do
   const int nb = red_blocks(c.g);
   k_reduce_f32<<<nb, 256, 0, c.stream>>>(c.g, f, c.p, c.ss, src, mode, exclude, c.partials);
   return launch_sum_partials(c, nb, 1, &c.ss->red[slot]);
+}
+
+hipError_t f32_damp(const LaunchCtx& c, const DevF32& f, int src) {
+  dim3 grid((c.g.jmax + 255) / 256, c.g.imax * c.g.kmax);
+  k_damp_f32<<<grid, 256, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+  return hipGetLastError();
 }
 
 hipError_t f32_totals(const LaunchCtx& c, const DevF32& f, int src, double* out2) {

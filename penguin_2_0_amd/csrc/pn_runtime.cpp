@@ -78,7 +78,9 @@ bool is_f32(const pn_ctx* c) { return c->precision == PN_FP32; }
 
 // the next step's CFL maximum can be taken in the last sweep's epilogue when FrRot cannot
 // change inside a step, i.e. unless the planet moves under the disk's torque
-bool fused_cfl(const pn_ctx* c) { return is_f32(c) && c->P.plnt_flag < 2; }
+// the next dt from the last sweep's epilogue: FP32, no work after the sweeps that changes the
+// state the CFL reads (a moving planet's update reads the sweeps' result; damping changes it)
+bool fused_cfl(const pn_ctx* c) { return is_f32(c) && c->P.plnt_flag < 2 && !c->P.kill_flag; }
 
 // the FP32 radial sweep of the context's geometry
 hipError_t f32_x(const pn_ctx* c, const LaunchCtx& L, int variant, int src) {
@@ -113,8 +115,8 @@ int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision)
   if (precision == PN_FP32 && p->eos != 0 && !(p->eos == 3 && p->ndim == 2 && !p->visc_flag))
     return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for EOS 0 and the inviscid 2-D EOS 3 disk");
   if (p->bary_flag != 0 && p->bary_flag != 1) return fail(c, PN_ERR_ARG, "bary_flag %d", p->bary_flag);
-  if (precision == PN_FP32 && (p->nlft != 3 || p->nrgh != 3 || p->kill_flag))
-    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path keeps fixed radial ghost rings and no damping zones");
+  if (precision == PN_FP32 && (p->nlft != 3 || p->nrgh != 3))
+    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path keeps fixed radial ghost rings");
   if (precision == PN_FP32 && p->bary_flag)
     return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path balances its backgrounds about a star at the origin");
   return PN_OK;
@@ -194,7 +196,7 @@ int launch_step(pn_ctx* c, int& cur) {
   const bool post = c->P.plnt_flag < 2;
   if ((rc = launch_sweeps(c, cur, fused ? 1 : 0, false, !post)) != PN_OK) return rc;
   // damping zones: ring-uniform relaxation, which commutes with the integer roll
-  if (c->P.kill_flag) PN_LAUNCH(c, "damp", launch_damp(L, cur));
+  if (c->P.kill_flag) PN_LAUNCH(c, "damp", is_f32(c) ? f32_damp(L, c->df, cur) : launch_damp(L, cur));
   if (post) PN_LAUNCH(c, "post step", launch_post_step(L, fused ? f32_fused_partials(c) : 0));
   else PN_LAUNCH(c, "planet update", launch_planet_after_sweep(L, cur, is_f32(c) ? &c->df : nullptr));
   return PN_OK;
@@ -889,14 +891,13 @@ int pn_damp_edges(pn_ctx* c, double dt) {
   if (!c) return PN_ERR_ARG;
   if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
   if (!(dt > 0.0)) return fail(c, PN_ERR_ARG, "dt %g", dt);
-  if (is_f32(c)) return fail(c, PN_ERR_UNSUPPORTED, "damping zones on the FP32 path");
   StepScalars h{};
   int rc = download_scalars(c, &h);
   if (rc != PN_OK) return rc;
   h.dt = dt;
   if ((rc = upload_scalars(c, h)) != PN_OK) return rc;
   LaunchCtx L = lctx(c);
-  PN_LAUNCH(c, "damp", launch_damp(L, c->cur));
+  PN_LAUNCH(c, "damp", is_f32(c) ? f32_damp(L, c->df, c->cur) : launch_damp(L, c->cur));
   return sync(c);
 }
 

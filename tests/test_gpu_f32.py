@@ -239,8 +239,14 @@ def test_f32_3d_load_store_roundtrip():
 
 # 3-D bars: Sigma and v_phi as in 2-D; v_r and v_theta start at zero and are judged
 # against their own largest values
-@pytest.mark.parametrize("w,n", [(disk3d(24, 72, 12), 20), (disk3d(64, 192, 32), 50)],
-                         ids=["disk3d_24x72x12", "disk3d_64x192x32"])
+def damped(w):
+    w.params.kill_flag = 1
+    w.name += "_damping"
+    return w
+
+
+@pytest.mark.parametrize("w,n", [(disk3d(24, 72, 12), 20), (disk3d(64, 192, 32), 50), (damped(disk3d(64, 192, 32)), 50)],
+                         ids=["disk3d_24x72x12", "disk3d_64x192x32", "disk3d_64x192x32_damping"])
 def test_f32_3d_vs_fp64(w, n):
     say("case: fp32 3-D vs fp64", w.name, n, "steps")
     a, sa = run(w, abi.PN_FP32, n)
@@ -281,6 +287,8 @@ FLAG_CASES = [flags_case("_fargo2", fargo_flag=2), flags_case("_inertial", frrot
               flags_case("_strong_planet", M_p=1e-3),
               # a planet that moves under the disk's forces, reduced from the FP32 state
               flags_case("_moving_planet", plnt_flag=2, M_p=1e-3),
+              # damping zones at both radial edges (the separate CFL kernel then gives dt)
+              flags_case("_damping", kill_flag=1),
               # more than 2048 slots: Index_Shift, the table rows and the roll over the whole grid
               configs.small(2048, 116, M_p=1e-3)]
 
@@ -313,7 +321,8 @@ EOS3_CASES = [flags_case("_eos3", eos=3, gam=1.4, init_p_iso=1),
               flags_case("_eos3_fargo2", eos=3, gam=1.4, init_p_iso=1, fargo_flag=2),
               flags_case("_eos3_inertial", eos=3, gam=1.4, init_p_iso=1, frrot_flag=0),
               flags_case("_eos3_noflat", eos=3, gam=1.4, init_p_iso=1, flat_flag=0),
-              flags_case("_eos3_moving_planet", eos=3, gam=1.4, init_p_iso=1, plnt_flag=2, M_p=1e-3)]
+              flags_case("_eos3_moving_planet", eos=3, gam=1.4, init_p_iso=1, plnt_flag=2, M_p=1e-3),
+              flags_case("_eos3_damping", eos=3, gam=1.4, init_p_iso=1, kill_flag=1)]
 
 
 @pytest.mark.parametrize("w", EOS3_CASES, ids=lambda w: w.name)
