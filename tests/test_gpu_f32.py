@@ -392,3 +392,35 @@ def test_3d_graph_after_an_odd_number_of_steps(prec):
         ctx.run(4, st, graph=True)
         b = ctx.store()
     assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+@pytest.mark.parametrize("over", [dict(), dict(eos=3, gam=1.4, init_p_iso=1)], ids=["isothermal", "eos3"])
+def test_f32_damp_edges_matches_fp64(over):
+    """pn_damp_edges on an FP32 context (kill_wave.cu:3-43 on the deviations) against the FP64
+    context on the same state: the rings inside the zones relax towards their background
+    record, and the FP32 result is the FP64 one to the rounding of the stored deviations."""
+    w = configs.small(96, 288, kill_flag=1, **over)
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    # the disk starts with v_r = 0 and v_phi on its background: give every field a deviation
+    rng = np.random.default_rng(5)
+    shape = d.rings["r"].shape
+    d.rings["u"] += 1e-3 * rng.uniform(-1.0, 1.0, shape)
+    d.rings["v"] *= 1.0 + 1e-4 * rng.uniform(-1.0, 1.0, shape)
+    if over:
+        d.rings["p"] *= 1.0 + 1e-4 * rng.uniform(-1.0, 1.0, shape)
+    say("case: fp32 damp_edges", w.name, over)
+    out = {}
+    for prec in (abi.PN_FP32, abi.PN_FP64):
+        with abi.Context(w.grid, w.params, prec) as ctx:
+            ctx.load(d)
+            ctx.damp_edges(0.1)
+            out[prec] = ctx.store()
+    a, b = out[abi.PN_FP32], out[abi.PN_FP64]
+    val = d.val_array()
+    fields = [("r", 0), ("u", 2), ("v", 3)] + ([("p", 1)] if over else [])
+    for f, col in fields:
+        bg = val[:, col:col + 1]
+        dev0, dev = np.abs(d.rings[f] - bg), np.abs(b[f] - bg)
+        err = np.abs(a[f] - b[f]).sum() / dev.sum()
+        say(f"  {f}: innermost ring deviation {dev0[0].sum():.3e} -> {dev[0].sum():.3e}, FP32 vs FP64 {err:.2e}")
+        assert dev[0].sum() < dev0[0].sum() and err <= 1e-6, f
