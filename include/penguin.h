@@ -286,6 +286,9 @@ int pn_batch_size(const pn_batch* b);
 pn_ctx* pn_batch_disk(pn_batch* b, int d);
 /* pn_run_begin for every disk; states: pn_batch_size records. */
 int pn_batch_run_begin(pn_batch* b, pn_run_state* states);
+/* pn_run_restart for every disk (each loaded from its picture startat with pn_load_snapshot
+ * on pn_batch_disk first); states: pn_batch_size records. */
+int pn_batch_run_restart(pn_batch* b, int startat, pn_run_state* states);
 /* Advance every disk nsteps steps (states in and out as pn_run's); graph != 0 replays a
  * captured step.  Tunes the sweep shapes on the first call unless pinned. */
 int pn_batch_run(pn_batch* b, int nsteps, int graph, pn_run_state* states);

@@ -133,6 +133,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_batch_size": (i, [vp]),
         "pn_batch_disk": (vp, [vp, i]),
         "pn_batch_run_begin": (i, [vp, P(PnRunState)]),
+        "pn_batch_run_restart": (i, [vp, i, P(PnRunState)]),
         "pn_batch_run": (i, [vp, i, i, P(PnRunState)]),
         "pn_batch_tune": (i, [vp, i, P(i), P(i)]),
         "pn_batch_set_variants": (i, [vp, i, i]),
@@ -417,6 +418,12 @@ class Batch:
     def run_begin(self):
         st = (PnRunState * len(self))()
         self._chk(self.lib.pn_batch_run_begin(self.h, st), "pn_batch_run_begin")
+        return st
+
+    def run_restart(self, startat: int):
+        """Prime every disk for a run resumed from picture startat."""
+        st = (PnRunState * len(self))()
+        self._chk(self.lib.pn_batch_run_restart(self.h, startat, st), "pn_batch_run_restart")
         return st
 
     def run(self, nsteps: int, states, graph: bool = True):

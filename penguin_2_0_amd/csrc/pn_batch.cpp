@@ -224,6 +224,15 @@ int pn_batch_run_begin(pn_batch* b, pn_run_state* states) {
   return PN_OK;
 }
 
+int pn_batch_run_restart(pn_batch* b, int startat, pn_run_state* states) {
+  if (!b || !states) return PN_ERR_ARG;
+  for (int d = 0; d < ndisk(b); ++d) {
+    const int rc = pn_run_restart(b->disks[d], startat, &states[d]);
+    if (rc != PN_OK) return bfail(b, rc, "disk %d: %s", d, b->disks[d]->err.c_str());
+  }
+  return PN_OK;
+}
+
 int pn_batch_set_variants(pn_batch* b, int var_x, int var_y) {
   if (!b) return PN_ERR_ARG;
   if (var_x < 0 || var_x >= kNumF32Variants || var_y < 0 || var_y >= kNumF32Variants)

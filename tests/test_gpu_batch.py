@@ -132,3 +132,40 @@ def test_batch_variants(variants):
         ref, _ = run_single(w, abi.PN_FP32, 10, variants=variants)
         for f in ("r", "u", "v"):
             assert rel(o[f], ref[f])[1] <= 1e-12, (w.name, f)
+
+
+def test_batch_restart_from_pictures(tmp_path):
+    """pn_batch_run_restart: every disk of a fresh batch loaded from its picture and resumed
+    reproduces the unbroken batch, with simtime counted from tmovie * startat."""
+    ws = members()
+    for w in ws:
+        w.params.tmovie = 0.25
+    disks = [abi.init_disk(w.grid, w.params, w.seed, w.noise) for w in ws]
+    paths = [str(tmp_path / f"binary_disk{d}_00002") for d in range(len(ws))]
+    say("case: batch restart", len(ws), "disks, startat 2")
+    with abi.Batch([w.grid for w in ws], [w.params for w in ws]) as b:
+        for ctx, d in zip(b.disks, disks):
+            ctx.load(d)
+        b.set_variants(0, 1)
+        st = b.run_begin()
+        b.run(6, st)
+        for ctx, path in zip(b.disks, paths):
+            ctx.save_snapshot(path)
+        t6 = [st[d].simtime for d in range(len(ws))]
+        b.run(5, st)
+        want = [ctx.store() for ctx in b.disks]
+        t11 = [st[d].simtime for d in range(len(ws))]
+    with abi.Batch([w.grid for w in ws], [w.params for w in ws]) as b:
+        for ctx, d, path in zip(b.disks, disks, paths):
+            ctx.load(d)
+            ctx.load_snapshot(path)
+        b.set_variants(0, 1)
+        st = b.run_restart(2)
+        assert all(st[d].simtime == 0.5 and st[d].timem == 0.0 for d in range(len(ws)))
+        b.run(5, st)
+        for d, ctx in enumerate(b.disks):
+            got = ctx.store()
+            say(f"  disk {d}: simtime {st[d].simtime!r} vs {0.5 + t11[d] - t6[d]!r}")
+            assert np.array_equal(got.view(np.uint8), want[d].view(np.uint8)), d
+            assert abs(st[d].simtime - (0.5 + t11[d] - t6[d])) <= 1e-12
+
