@@ -199,6 +199,17 @@ int pn_sweep_axis(pn_ctx* ctx, int axis, double dt, const pn_body* planet, doubl
  * (2-D).  exclude is in units of sc_h as in the reference. */
 int pn_reduce(pn_ctx* ctx, const pn_body* planet, double exclude, int mode, double* out);
 
+/* Azimuthal mean of one field of every interior ring, into out[i + imax*k] (n = imax*kmax):
+ * sum_j f[j] dphi[j0] / sum_j dphi[j0] over the ring's data columns j at lab columns
+ * j0 = j + rot_j -- the dphi-weighted ring average of the reader (write_output_file,
+ * read/p_output.cpp:112-127, which divides by 2 pi, the sum of dphi over a full ring).
+ * field: 0 Sigma (r), 1 P (p), 2 v_r (u), 3 v_phi (v), 4 w (v_theta in 3-D).  A wavefront-
+ * shuffle reduction on the device, FP64 sums in a fixed order; on an FP32 context the
+ * deviations from the ring's background are summed and the background added back.  A
+ * diagnostic beside the step: the FARGO background stays the initial analytic v_phi(r)
+ * (orbital_advection.cu:21-22), as in the reference. */
+int pn_ring_means(pn_ctx* ctx, int field, double* out, size_t n);
+
 /* ---- planet (host, planet.cu:28-93) ------------------------------------ */
 void pn_init_planet(const pn_params* p, pn_body* planet, double* FrRot);
 /* mass ramp + forces; for plnt_flag == 2 it runs two pn_reduce calls. */
@@ -296,6 +307,8 @@ int pn_batch_run(pn_batch* b, int nsteps, int graph, pn_run_state* states);
  * pin them instead with pn_batch_set_variants (same indices as pn_set_variants). */
 int pn_batch_tune(pn_batch* b, int nrep, int* var_x, int* var_y);
 int pn_batch_set_variants(pn_batch* b, int var_x, int var_y);
+/* The variants in use (after tuning or pinning). */
+int pn_batch_get_variants(const pn_batch* b, int* var_x, int* var_y);
 /* Mean ms over nrep launches of {pre-step, r sweep, phi sweep, post-step} of the whole
  * batch, timed with HIP events on the batch's stream.  The state advances. */
 int pn_batch_time_kernels(pn_batch* b, int nrep, double* ms4);

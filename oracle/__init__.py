@@ -38,6 +38,9 @@ def lib() -> C.CDLL:
             "orc_damp": (None, [G, Pp, vp, P(PnSymdisk), d]),
             "orc_reduce": (d, [G, Pp, vp, B, d, i]),
             "orc_totals": (None, [G, Pp, vp, dp, dp]),
+            "orc_ring_means": (None, [G, vp, i, dp]),
+            "orc_set_threads": (None, [i]),
+            "orc_get_threads": (i, []),
             "orc_init_planet": (None, [Pp, B, dp]),
             "orc_planet_forces": (None, [G, Pp, vp, B, d, d]),
             "orc_kick_drift_kick": (None, [Pp, B, dp, d, i]),
@@ -127,6 +130,13 @@ class OracleDisk:
     def reduce(self, planet: PnBody, exclude: float, mode: int) -> float:
         return lib().orc_reduce(C.byref(self.grid), C.byref(self.params), vptr(self.rings), C.byref(planet),
                                 exclude, mode)
+
+    def ring_means(self, field: int | str) -> np.ndarray:
+        if isinstance(field, str):
+            field = "rpuvw".index(field)
+        out = np.zeros(self.grid.imax * self.grid.kmax)
+        lib().orc_ring_means(C.byref(self.grid), vptr(self.rings), field, dptr(out))
+        return out
 
     def totals(self) -> tuple[float, float]:
         m, l = C.c_double(), C.c_double()

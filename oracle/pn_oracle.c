@@ -1714,6 +1714,44 @@ void orc_totals(const pn_grid* g, const pn_params* P, const void* rings, double*
   *angmom = sl + cl;
 }
 
+/* The reader's ring average (write_output_file, read/p_output.cpp:112-127): over the lab
+ * columns j of a ring, in order, sum f(lab j) dy[j] with f(lab j) at data index
+ * (j - rot_j) mod jmax (update_grid, read/p_init.cpp:108-132), divided by the sum of dy[j]
+ * (the reader divides by 2 pi, which that sum is on a full ring).  field 0..4 = r, p, u, v, w.
+ * out[n] for ring n = i + imax k. */
+void orc_ring_means(const pn_grid* g, const void* rings, int field, double* out) {
+  long nring = (long)g->imax * g->kmax;
+  for (long n = 0; n < nring; n++) {
+    ring_t R = ring_at(rings, g->jmax, n);
+    const double* F = field == 0 ? R.r : field == 1 ? R.p : field == 2 ? R.u : field == 3 ? R.v : R.w;
+    double a = 0.0, w = 0.0;
+    for (int j = 0; j < g->jmax; j++) {
+      int jd = j - *R.rot_j;
+      if (jd < 0) jd += g->jmax;
+      a += F[jd] * R.dy[j];
+      w += R.dy[j];
+    }
+    out[n] = a / w;
+  }
+}
+
+/* OpenMP threads of the sweeps (the CPU baseline is timed on all cores and on one) */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
+int orc_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 /* ------------------------------------------------------------------------ */
 /* planet.cu:11-93                                                           */
 

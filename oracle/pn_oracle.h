@@ -42,6 +42,10 @@ void orc_damp(const pn_grid* g, const pn_params* p, void* rings, const pn_symdis
 double orc_reduce(const pn_grid* g, const pn_params* p, const void* rings, const pn_body* planet,
                   double exclude, int mode);
 void orc_totals(const pn_grid* g, const pn_params* p, const void* rings, double* mass, double* angmom);
+/* the reader's dphi-weighted ring average of field 0..4 (r, p, u, v, w) per ring */
+void orc_ring_means(const pn_grid* g, const void* rings, int field, double* out);
+void orc_set_threads(int n);
+int orc_get_threads(void);
 
 /* planet.cu */
 void orc_init_planet(const pn_params* p, pn_body* b, double* FrRot);

@@ -118,6 +118,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_run_restart": (i, [vp, i, P(PnRunState)]),
         "pn_run": (i, [vp, i, i, P(PnRunState)]),
         "pn_totals": (i, [vp, P(d), P(d)]),
+        "pn_ring_means": (i, [vp, i, P(d), sz]),
         "pn_copy_bandwidth": (i, [sz, i, P(d)]),
         "pn_lab_fields": (i, [vp, sz, i, i, i, vp, vp, vp, vp, vp]),
         "pn_damp_edges": (i, [vp, d]),
@@ -137,6 +138,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_batch_run": (i, [vp, i, i, P(PnRunState)]),
         "pn_batch_tune": (i, [vp, i, P(i), P(i)]),
         "pn_batch_set_variants": (i, [vp, i, i]),
+        "pn_batch_get_variants": (i, [vp, P(i), P(i)]),
         "pn_batch_time_kernels": (i, [vp, i, P(d)]),
     }
     for name, (res, args) in sig.items():
@@ -340,6 +342,17 @@ class Context:
         self._chk(self.lib.pn_totals(self.h, C.byref(m), C.byref(l)), "pn_totals")
         return m.value, l.value
 
+    def ring_means(self, field: int | str) -> np.ndarray:
+        """Azimuthal mean of a field per interior ring (pn_ring_means); field 0..4 or one of
+        'r', 'p', 'u', 'v', 'w'.  Returns imax * kmax values in ring order i + imax k."""
+        if isinstance(field, str):
+            field = "rpuvw".index(field)
+        n = self.grid.imax * self.grid.kmax
+        out = np.zeros(n)
+        self._chk(self.lib.pn_ring_means(self.h, field, out.ctypes.data_as(C.POINTER(C.c_double)), n),
+                  "pn_ring_means")
+        return out
+
     def run_begin(self) -> PnRunState:
         st = PnRunState()
         self._chk(self.lib.pn_run_begin(self.h, C.byref(st)), "pn_run_begin")
@@ -437,6 +450,11 @@ class Batch:
 
     def set_variants(self, var_x: int, var_y: int):
         self._chk(self.lib.pn_batch_set_variants(self.h, var_x, var_y), "pn_batch_set_variants")
+
+    def variants(self) -> tuple[int, int]:
+        vx, vy = C.c_int(), C.c_int()
+        self._chk(self.lib.pn_batch_get_variants(self.h, C.byref(vx), C.byref(vy)), "pn_batch_get_variants")
+        return vx.value, vy.value
 
     def time_kernels(self, nrep: int) -> dict:
         ms = (C.c_double * 4)()

@@ -32,6 +32,7 @@ struct pn_batch {
   int* base_x = nullptr;      // device, ndisk + 1 item offsets
   int* base_y = nullptr;
   hipGraphExec_t gexec = nullptr;
+  int gexec_cur = 0;          // the buffer the captured step reads (it returns there)
 };
 
 namespace {
@@ -246,6 +247,13 @@ int pn_batch_set_variants(pn_batch* b, int var_x, int var_y) {
   return PN_OK;
 }
 
+int pn_batch_get_variants(const pn_batch* b, int* var_x, int* var_y) {
+  if (!b || !var_x || !var_y) return PN_ERR_ARG;
+  *var_x = b->var_x;
+  *var_y = b->var_y;
+  return PN_OK;
+}
+
 int pn_batch_tune(pn_batch* b, int nrep, int* var_x, int* var_y) {
   if (!b || nrep < 1) return PN_ERR_ARG;
   int cur, rc;
@@ -306,6 +314,9 @@ int pn_batch_run(pn_batch* b, int nsteps, int graph, pn_run_state* states) {
   }
   int done = 0;
   if (graph && nsteps > 0) {
+    // the captured step names its buffer; a disk moved to the other one since (for example by
+    // pn_sweep_axis on pn_batch_disk) needs a step captured from there
+    if (b->gexec && b->gexec_cur != cur) drop_graph(b);
     if (!b->gexec) {
       hipGraph_t gr;
       PB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -316,6 +327,7 @@ int pn_batch_run(pn_batch* b, int nsteps, int graph, pn_run_state* states) {
       e = hipGraphInstantiate(&b->gexec, gr, nullptr, nullptr, 0);
       hipGraphDestroy(gr);
       if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      b->gexec_cur = cur;
     }
     for (; done < nsteps; ++done) PB_HIP(b, hipGraphLaunch(b->gexec, b->stream));
   }

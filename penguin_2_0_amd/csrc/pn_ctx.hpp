@@ -26,6 +26,7 @@ struct pn_ctx {
   double* cflp = nullptr;           // device, CFL partial maxima
   int ncflp = 0;
   double* out2 = nullptr;           // device, totals
+  double* ringbuf = nullptr;        // device, one double per interior ring (ring means)
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   int cur = 0;                      // buffer holding the state

@@ -116,6 +116,8 @@ hipError_t f32_sweep_y_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs
 hipError_t f32_cfl(const LaunchCtx& c, const DevF32& f, int src, int* n);
 hipError_t f32_reduce(const LaunchCtx& c, const DevF32& f, int src, int mode, double exclude, int slot);
 hipError_t f32_totals(const LaunchCtx& c, const DevF32& f, int src, double* out2);
+// azimuthal mean of field (0 Sigma, 1 P, 2 v_r, 3 v_phi, 4 v_theta) per interior ring into out
+hipError_t f32_ring_means(const LaunchCtx& c, const DevF32& f, int src, int field, double* out);
 // damping zones at the radial edges with ss->dt (the FP64 path's launch_damp on the FP32 state)
 hipError_t f32_damp(const LaunchCtx& c, const DevF32& f, int src);
 

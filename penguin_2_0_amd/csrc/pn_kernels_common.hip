@@ -13,26 +13,6 @@ constexpr double kTwoPi = 6.283185307179586476925286766559;
 constexpr double kEarthMass = 0.000003;
 constexpr double kSigma0 = 1.0 * 0.00019126835;   // global.h:37 (MMSN_1AU)
 
-// Index_Shift (orbital_advection.cu:12-43): one thread per interior ring
-__global__ void k_index_shift(DevGrid g, DevParams prm, const StepScalars* __restrict__ ss) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= g.imax * g.kmax) return;
-  const int i = n % g.imax, k = n / g.imax;
-  const int s = slot_of(g, i, k);
-  const double dt = ss->dt, FrRot = ss->FrRot;
-  const double xc = g.xc[s];
-  const double widthy = xc * g.dy[0] / dt;
-  double rot_v;
-  if (prm.fargo == 1) rot_v = g.val[i + g.imax * k].v - FrRot * xc;
-  else if (prm.fargo == 2) rot_v = (1.0 - FrRot) * xc;
-  else rot_v = 0.0;
-  const double jp = g.rot_v[s] / widthy;
-  if (jp >= 0.0) g.inc_j[s] = (int)(jp + 0.5);
-  else g.inc_j[s] = (int)(jp - 0.5);
-  g.res_v[s] = floor(jp + 0.5) * widthy;
-  g.rot_v[s] = rot_v;
-}
-
 // rotate_rings (ppm.cu:309-322)
 __global__ void k_rotate(DevGrid g) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -144,6 +124,36 @@ __global__ __launch_bounds__(256) void k_cfl(DevGrid g, DevParams prm, const Ste
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) cflp[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// Azimuthal mean of field `field` of every interior ring: sum_j F[j] dy[j0] / sum_j dy[j0]
+// over the ring's data columns j, lab column j0 = j + rot_j (device_func.cu:71) -- the dphi
+// weighting of the reader's ring average (write_output_file, read/p_output.cpp:112-127, which
+// divides by 2 pi, the sum of dphi over a full ring).  One workgroup of 256 per ring n =
+// i + imax k: each thread sums its columns j = t, t + 256, ... in FP64, the wavefronts fold
+// with lane shuffles, and the four wave sums are added in a fixed order.
+__global__ __launch_bounds__(256) void k_ring_means(DevGrid g, int src, int field, double* __restrict__ out) {
+  __shared__ double sh[8];
+  const int n = blockIdx.x;
+  const int s = slot_of(g, n % g.imax, n / g.imax);
+  const double* __restrict__ F = g.f[src][field] + (long)s * g.jmax;
+  const int rot = g.rot_j[s];
+  double a = 0.0, w = 0.0;
+  for (int j = threadIdx.x; j < g.jmax; j += 256) {
+    int j0 = j + rot;
+    if (j0 >= g.jmax) j0 -= g.jmax;
+    const double d = g.dy[j0];
+    a += F[j] * d;
+    w += d;
+  }
+  a = wave_sum(a);
+  w = wave_sum(w);
+  if ((threadIdx.x & 63) == 0) {
+    sh[threadIdx.x >> 6] = a;
+    sh[4 + (threadIdx.x >> 6)] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) out[n] = ((sh[0] + sh[1]) + (sh[2] + sh[3])) / ((sh[4] + sh[5]) + (sh[6] + sh[7]));
 }
 
 // dt = min(1.1 dt_prev, courant / max)  (timestep.cu:194, 224-230), max over n partials
@@ -395,6 +405,14 @@ __device__ void index_shift_ring(DevGrid& g, const DevParams& prm, int n, double
   g.rot_v[s] = rot_v;
 }
 
+// Index_Shift (orbital_advection.cu:12-43) as its own launch: one thread per interior ring,
+// running the body the pre-step runs for every ring
+__global__ void k_index_shift(DevGrid g, DevParams prm, const StepScalars* __restrict__ ss) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.imax * g.kmax) return;
+  index_shift_ring(g, prm, n, ss->dt, ss->FrRot);
+}
+
 // the per-step rotation rows of the FP32 table for slot s (ghost slots keep res_v = 0):
 // sin / cos of dt (res_v / r_cyl - v_y) / 4 with the planet velocity the sweeps will read
 // (3-D: also into the theta-major copy tabz, where slot (ir, kz) sits at kz + sk ir)
@@ -631,6 +649,12 @@ hipError_t launch_reduce(const LaunchCtx& c, int src, int mode, double exclude, 
   int nb = red_blocks(c.g);
   k_reduce<<<nb, 256, 0, c.stream>>>(c.g, c.p, c.ss, src, mode, exclude, c.partials);
   k_sum_partials<<<1, kSumThreads, 0, c.stream>>>(c.partials, nb, 1, &c.ss->red[slot]);
+  return hipGetLastError();
+}
+
+hipError_t launch_ring_means(const LaunchCtx& c, int src, int field, double* out) {
+  if (field < 0 || field >= kNField) return hipErrorInvalidValue;
+  k_ring_means<<<c.g.imax * c.g.kmax, 256, 0, c.stream>>>(c.g, src, field, out);
   return hipGetLastError();
 }
 

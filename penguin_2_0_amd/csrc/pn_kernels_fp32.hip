@@ -722,6 +722,58 @@ __global__ __launch_bounds__(256) void k_totals_f32(DevGrid g, DevF32 f, int src
   }
 }
 
+// Azimuthal mean of a field of every interior ring on the FP32 state (the FP64 path's
+// k_ring_means, same dphi weighting and order): the deviations from the ring's background are
+// summed in FP64 and the background added back once, so the mean of a ring that departs little
+// from its background keeps every digit the deviations hold.  field: 0 Sigma, 1 P (EOS 0:
+// c_s^2 Sigma; EOS 3: Sigma q), 2 v_r, 3 v_phi, 4 v_theta (3-D; 0 in 2-D).
+__global__ __launch_bounds__(256) void k_ring_means_f32(DevGrid g, DevF32 f, int eos, int src, int field,
+                                                        double* __restrict__ out) {
+  __shared__ double sh[8];
+  const int n = blockIdx.x;
+  const int s = slot_of(g, n % g.imax, n / g.imax);
+  const long base = (long)s * g.jmax;
+  const int rot = g.rot_j[s];
+  const double rb = f.rbg[s], cs2 = g.cs2[s];
+  const double qb = (eos == 3) ? g.val[n].p / g.val[n].r : 0.0;
+  const float* __restrict__ F0 = f.f[src][0] + base;
+  const float* __restrict__ F1 = f.f[src][1] + base;
+  const float* __restrict__ F2 = f.f[src][2] + base;
+  const float* __restrict__ F3 = (g.ndim == 3 || eos == 3) ? f.f[src][3] + base : nullptr;
+  double a = 0.0, w = 0.0;
+  for (int j = threadIdx.x; j < g.jmax; j += 256) {
+    int j0 = j + rot;
+    if (j0 >= g.jmax) j0 -= g.jmax;
+    const double d = g.dy[j0];
+    double x = 0.0;
+    if (field == 0) {
+      x = F0[j];
+    } else if (field == 1) {
+      const double dr = F0[j];
+      x = (eos == 3) ? rb * (double)F3[j] + qb * dr + dr * (double)F3[j] : cs2 * dr;
+    } else if (field == 2) {
+      x = F1[j];
+    } else if (field == 3) {
+      x = F2[j];
+    } else if (g.ndim == 3) {
+      x = F3[j];
+    }
+    a += x * d;
+    w += d;
+  }
+  a = wave_sum(a);
+  w = wave_sum(w);
+  if ((threadIdx.x & 63) == 0) {
+    sh[threadIdx.x >> 6] = a;
+    sh[4 + (threadIdx.x >> 6)] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double bg = (field == 0) ? rb : (field == 1) ? ((eos == 3) ? rb * qb : cs2 * rb) : (field == 3) ? f.vbg[s] : 0.0;
+    out[n] = bg + ((sh[0] + sh[1]) + (sh[2] + sh[3])) / ((sh[4] + sh[5]) + (sh[6] + sh[7]));
+  }
+}
+
 int red_blocks(const DevGrid& g) {
   long ncell = (long)g.imax * g.kmax * g.jmax;
   long b = (ncell + 255) / 256;
@@ -949,6 +1001,12 @@ hipError_t f32_totals(const LaunchCtx& c, const DevF32& f, int src, double* out2
   if (nb > kMaxPartials / 2) nb = kMaxPartials / 2;
   k_totals_f32<<<nb, 256, 0, c.stream>>>(c.g, f, src, c.partials);
   return launch_sum_partials(c, nb, 2, out2);
+}
+
+hipError_t f32_ring_means(const LaunchCtx& c, const DevF32& f, int src, int field, double* out) {
+  if (field < 0 || field > 4) return hipErrorInvalidValue;
+  k_ring_means_f32<<<c.g.imax * c.g.kmax, 256, 0, c.stream>>>(c.g, f, c.p.eos, src, field, out);
+  return hipGetLastError();
 }
 
 }  // namespace pn

@@ -42,6 +42,8 @@ hipError_t launch_rotate(const LaunchCtx& c);
 hipError_t launch_cfl(const LaunchCtx& c, int src);              // partials into cflp, then ss->dt
 hipError_t launch_reduce(const LaunchCtx& c, int src, int mode, double exclude, int slot);  // -> ss->red[slot]
 hipError_t launch_totals(const LaunchCtx& c, int src, double* out2);  // out2: device, mass and angmom
+// azimuthal mean of FP64 field `field` of every interior ring (out: device, imax kmax doubles)
+hipError_t launch_ring_means(const LaunchCtx& c, int src, int field, double* out);
 // simtime, drift, the disk's forces on the planet (modes 8 / 9; from the FP32 state when f32 is
 // given), closing half kick
 struct DevF32;

@@ -269,6 +269,7 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
   if (r2 == PN_OK) r2 = dalloc(c, &c->ss, 1);
   if (r2 == PN_OK) r2 = dalloc(c, &c->partials, kMaxPartials);
   if (r2 == PN_OK) r2 = dalloc(c, &c->out2, 2);
+  if (r2 == PN_OK) r2 = dalloc(c, &c->ringbuf, static_cast<size_t>(g->imax) * g->kmax);
   // CFL partials: one per block of the grid-stride CFL kernels (at most 2048) or one per
   // workgroup of the FP32 sweep that takes the CFL in its epilogue, in any variant
   c->ncflp = 2048;
@@ -1029,6 +1030,19 @@ int pn_totals(pn_ctx* c, double* mass, double* angmom) {
   *mass = h[0];
   *angmom = h[1];
   return PN_OK;
+}
+
+int pn_ring_means(pn_ctx* c, int field, double* out, size_t n) {
+  if (!c || !out) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  if (field < 0 || field > 4) return fail(c, PN_ERR_ARG, "field %d outside 0..4", field);
+  const size_t nring = static_cast<size_t>(c->g.imax) * c->g.kmax;
+  if (n != nring) return fail(c, PN_ERR_ARG, "out holds %zu values, the disk has %zu rings", n, nring);
+  // the state's field order (r, p, u, v, w) is the hydr_ring's
+  if (is_f32(c)) PN_LAUNCH(c, "ring means", f32_ring_means(lctx(c), c->df, c->cur, field, c->ringbuf));
+  else PN_LAUNCH(c, "ring means", launch_ring_means(lctx(c), c->cur, field, c->ringbuf));
+  PN_HIP(c, hipMemcpyAsync(out, c->ringbuf, nring * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return sync(c);
 }
 
 int pn_planet_forces(pn_ctx* c, pn_body* b, double FrRot, double simtime) {

@@ -1,0 +1,232 @@
+"""GPU tests at the workloads' own sizes and step counts, through the C ABI:
+
+- the 256 x 768 disk (100 steps): the FP64 path and the FP32-bulk path against the oracle;
+- the 1024 x 3072 disk (1000 steps): the FP32-bulk path against the FP64 path, and the fold of
+  more than 4096 fused-CFL partials that only disks of this size reach;
+- Index_Shift on rings whose whole-cell shift sits exactly on a half integer of either sign
+  (orbital_advection.cu:35-39), against the oracle and the closed form;
+- the azimuthal ring means (wavefront-shuffle reduction) against the oracle.
+
+Bars: integers bit-exact; FP64 against the oracle within the rounding of the device
+transcendentals grown over the steps; FP32 against FP64 within the FP32 path's accuracy
+(relative L1 of Sigma far below the 1e-5 target).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from penguin_2_0_amd import abi, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def lab(state: np.ndarray, f: str) -> np.ndarray:
+    a = state[f]
+    return np.stack([np.roll(a[i], int(state["rot_j"][i])) for i in range(a.shape[0])])
+
+
+def rel(a, b):
+    d = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64))
+    return d.sum() / np.abs(b).sum(), d.max() / np.abs(b).max()
+
+
+def run(w, disk, prec, n, variants=None):
+    with abi.Context(w.grid, w.params, prec) as ctx:
+        ctx.load(disk)
+        if variants is not None:
+            ctx.set_variants(*variants)
+        st = ctx.run_begin()
+        ctx.run(n, st, graph=True)
+        return ctx.store(), st
+
+
+def test_c1_fp64_against_the_oracle_for_100_steps():
+    w = configs.c1()
+    say("case:", w.name, "FP64 vs oracle,", w.steps, "steps")
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    o = oracle.OracleDisk(w.grid, w.params, w.seed, w.noise)
+    assert np.array_equal(d.rings.view(np.uint8), o.rings.view(np.uint8))
+    sto = o.run_begin()
+    o.run(w.steps, sto)
+    g, st = run(w, d, abi.PN_FP64, w.steps)
+    say(f"  dt oracle {sto.dt!r} gpu {st.dt!r}")
+    assert abs(st.dt - sto.dt) <= 1e-12 * sto.dt
+    assert abs(st.simtime - sto.simtime) <= 1e-12 * sto.simtime
+    assert np.array_equal(g["rot_j"], o.rings["rot_j"])
+    assert np.array_equal(g["inc_j"], o.rings["inc_j"])
+    for f, (b1, binf) in {"r": (1e-11, 1e-8), "v": (1e-11, 1e-8), "u": (1e-9, 1e-7)}.items():
+        l1, li = rel(g[f], o.rings[f])
+        say(f"  {f}: L1 {l1:.3e} Linf {li:.3e}")
+        assert l1 <= b1 and li <= binf, (f, l1, li)
+
+
+def test_c1_fp32_against_the_oracle_for_100_steps():
+    w = configs.c1()
+    say("case:", w.name, "FP32 vs oracle,", w.steps, "steps")
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    o = oracle.OracleDisk(w.grid, w.params, w.seed, w.noise)
+    sto = o.run_begin()
+    o.run(w.steps, sto)
+    g, st = run(w, d, abi.PN_FP32, w.steps)
+    say(f"  dt oracle {sto.dt!r} gpu {st.dt!r}")
+    assert abs(st.dt - sto.dt) <= 1e-6 * sto.dt
+    assert np.array_equal(g["rot_j"], o.rings["rot_j"])
+    for f, bar in {"r": 1e-7, "v": 1e-8, "u": 1e-4}.items():
+        l1, li = rel(lab(g, f), lab(o.rings, f))
+        say(f"  {f}: L1 {l1:.3e} Linf {li:.3e}")
+        assert l1 <= bar, (f, l1)
+
+
+def test_c2_fp32_against_fp64_for_1000_steps():
+    """The metric's accuracy target on the disk it names: relative L1(Sigma) <= 1e-5 after
+    1000 steps.  The FP32 path sits at the flow's own sensitivity (about 6e-8 here); the bar
+    of 1e-6 leaves room for box-to-box differences only."""
+    w = configs.c2()
+    n = 1000
+    say("case:", w.name, "FP32 vs FP64,", n, "steps")
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    a, sa = run(w, d, abi.PN_FP32, n)
+    b, sb = run(w, d, abi.PN_FP64, n)
+    say(f"  dt fp32 {sa.dt!r} fp64 {sb.dt!r}, simtime {sa.simtime!r} {sb.simtime!r}")
+    assert abs(sa.dt - sb.dt) <= 1e-6 * sb.dt
+    assert abs(sa.simtime - sb.simtime) <= 1e-6 * sb.simtime
+    errs = {f: rel(lab(a, f), lab(b, f)) for f in ("r", "u", "v")}
+    for f, (l1, li) in errs.items():
+        say(f"  {f}: L1 {l1:.3e} Linf {li:.3e}")
+    assert errs["r"][0] <= 1e-6
+    assert errs["v"][0] <= 1e-7
+    assert errs["u"][0] <= 1e-4   # v_r is measured against its own small scale
+    # a ring whose FARGO split sits within the dt difference of a half integer may roll the
+    # other way; the lab-frame comparison above accounts for it
+    assert (a["rot_j"] != b["rot_j"]).sum() <= 2
+
+
+def test_fused_cfl_folds_more_than_4096_partials():
+    """On the 1024 x 3072 disk the 128-cell azimuthal windows make 6912 workgroups, each
+    writing one CFL partial, which the step folds in 256 chunks before the post-step (the
+    longer windows make 3328 and take the direct path).  A super-Keplerian ring whose
+    workgroups sit past partial 4096 sets the CFL: the second step's dt -- taken from those
+    partials -- must see it, as the FP64 path's separate CFL pass does."""
+    w = configs.c2()
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    n_ring = 103
+    # workgroup of ring n, window l for the (K 2, 4 waves) azimuthal variant: wave
+    # gw = n nwin + l, item gw // 4, launched as block b with swizzle(b) = item over 8 XCDs
+    nwin = -(-w.grid.jmax // 116)
+    nblk = -(-(nwin * w.grid.imax) // 4)
+    q, rem = divmod(nblk, 8)
+    assert rem == 0 and nblk > 4096
+    blocks = sorted({(it % q) * 8 + it // q for it in {(n_ring * nwin + l) // 4 for l in range(nwin)}})
+    say("case: fused CFL fold, partials", nblk, "ring", n_ring, "blocks", blocks[0], "..", blocks[-1])
+    assert blocks[0] > 4096
+    d.rings["v"][n_ring, :] += 0.25
+    dts = {}
+    for name, prec, var in (("fp32_fold", abi.PN_FP32, (0, 1)), ("fp32_direct", abi.PN_FP32, (0, 3)),
+                            ("fp64", abi.PN_FP64, None)):
+        _, st = run(w, d, prec, 2, var)
+        dts[name] = st.dt
+        say(f"  {name}: dt of step 2 {st.dt!r}")
+    # the planted ring's (|dv| + c_s) / (r dphi) ~ 327 sets dt ~ 0.5 / 327 (the unperturbed disk's
+    # maximum, ~110 at the inner edge, would allow 1.1 times the first step's dt)
+    assert dts["fp64"] < 0.5 / 250.0
+    assert abs(dts["fp32_fold"] - dts["fp64"]) <= 1e-6 * dts["fp64"]
+    assert abs(dts["fp32_direct"] - dts["fp64"]) <= 1e-6 * dts["fp64"]
+
+
+def half_integer_rot_v(xc: np.ndarray, dy0: float, dt: float, targets: np.ndarray,
+                       exact: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """rot_v per ring with rot_v / widthy == target exactly in FP64 where `exact` (widthy =
+    xc dy0 / dt, as the device evaluates it): start at target * widthy and step by ulps until the
+    quotient lands; a target no neighbouring rot_v reaches is dropped from `exact`."""
+    out = np.empty_like(xc)
+    hit = exact.copy()
+    for k, (x, t) in enumerate(zip(xc, targets)):
+        wy = x * dy0 / dt
+        v = t * wy
+        if exact[k]:
+            for _ in range(8):
+                qq = v / wy
+                if qq == t:
+                    break
+                v = np.nextafter(v, np.inf if qq < t else -np.inf)
+            hit[k] = v / wy == t
+        out[k] = v
+    return out, hit
+
+
+def test_index_shift_on_exact_half_integers():
+    w = configs.c1()
+    say("case: Index_Shift on half-integer shifts,", w.name)
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    n = w.grid.imax
+    rng = np.random.default_rng(17)
+    k = rng.integers(0, 40, n)
+    sign = np.where(np.arange(n) % 3 == 0, -1.0, np.where(np.arange(n) % 3 == 1, 1.0, 0.0))
+    targets = np.where(sign != 0, sign * (k + 0.5), rng.uniform(-30, 30, n))
+    dt = 0.0137
+    dy0 = float(d.rings["dy"][0][0])
+    rv, hit = half_integer_rot_v(d.rings["xc"], dy0, dt, targets, sign != 0)
+    d.rings["rot_v"][:] = rv
+    sign = np.where(hit, sign, 0.0)
+    assert (sign < 0).sum() >= 50 and (sign > 0).sum() >= 50
+    o = oracle.OracleDisk(w.grid, w.params, w.seed, w.noise)
+    o.rings[:] = d.rings
+    FrRot = 0.9
+    o.index_shift(dt, FrRot)
+    with abi.Context(w.grid, w.params) as ctx:
+        ctx.load(d)
+        ctx.index_shift(dt, FrRot)
+        g = ctx.store()
+    assert np.array_equal(g["inc_j"], o.rings["inc_j"])
+    assert np.array_equal(g["res_v"].view(np.uint64), o.rings["res_v"].view(np.uint64))
+    assert np.array_equal(g["rot_v"].view(np.uint64), o.rings["rot_v"].view(np.uint64))
+    # closed form (orbital_advection.cu:35-39): inc_j rounds half away from zero, res_v rounds
+    # half up, so at -(k + 1/2) they disagree by one cell
+    wy = d.rings["xc"] * dy0 / dt
+    neg, pos = sign < 0, sign > 0
+    assert np.array_equal(g["inc_j"][neg], -(k[neg] + 1))
+    assert np.array_equal(g["res_v"][neg], -k[neg] * wy[neg])
+    assert np.array_equal(g["inc_j"][pos], k[pos] + 1)
+    assert np.array_equal(g["res_v"][pos], (k[pos] + 1) * wy[pos])
+    say(f"  {neg.sum()} negative and {pos.sum()} positive half-integer rings bit-exact")
+
+
+@pytest.mark.parametrize("prec", [abi.PN_FP64, abi.PN_FP32], ids=["fp64", "fp32"])
+def test_ring_means_against_the_oracle(prec):
+    """pn_ring_means of every field on a stirred state (5 FP64 steps), given a stretched
+    azimuthal grid and a random roll per ring (so that data and lab columns weigh differently),
+    against the oracle's lab-frame average."""
+    w = configs.small(130, 250)
+    say("case: ring means", w.name, "fp64" if prec == abi.PN_FP64 else "fp32")
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    stirred, _ = run(w, d, abi.PN_FP64, 5)
+    d.rings[:] = stirred
+    J = w.grid.jmax
+    rng = np.random.default_rng(3)
+    wt = 1.0 + 0.4 * np.sin(np.arange(J) * 6.0 * np.pi / J) + 0.2 * rng.random(J)
+    dy = wt / wt.sum() * (2.0 * np.pi)
+    y = np.concatenate([[0.0], np.cumsum(dy)[:-1]])
+    for R in (d.rings, d.lft, d.rgh):
+        R["dy"][:] = dy
+        R["y"][:] = y
+        R["yc"][:] = y + 0.5 * dy
+        R["yvol"][:] = dy
+    d.rings["rot_j"][:] = rng.integers(0, J, len(d.rings))
+    o = oracle.OracleDisk(w.grid, w.params, w.seed, w.noise)
+    o.rings[:] = d.rings
+    with abi.Context(w.grid, w.params, prec) as ctx:
+        ctx.load(d)
+        got = {f: ctx.ring_means(f) for f in "rpuvw"}
+    for f in "rpuvw":
+        want = o.ring_means(f)
+        scale = max(np.abs(want).max(), 1e-300)
+        err = np.abs(got[f] - want).max() / scale
+        say(f"  {f}: max |diff| / scale {err:.3e}")
+        # FP64: summation order only; FP32: one rounding of each stored deviation
+        assert err <= (1e-14 if prec == abi.PN_FP64 else 1e-9), (f, err)
