@@ -225,8 +225,12 @@ def test_ring_means_against_the_oracle(prec):
         got = {f: ctx.ring_means(f) for f in "rpuvw"}
     for f in "rpuvw":
         want = o.ring_means(f)
-        scale = max(np.abs(want).max(), 1e-300)
-        err = np.abs(got[f] - want).max() / scale
-        say(f"  {f}: max |diff| / scale {err:.3e}")
-        # FP64: summation order only; FP32: one rounding of each stored deviation
-        assert err <= (1e-14 if prec == abi.PN_FP64 else 1e-9), (f, err)
+        # each ring's dphi-weighted mean of |f|: the size the mean's rounding is relative to (a
+        # field such as v_r averages to far less than its values)
+        absmean = np.stack([(np.abs(lab(d.rings, f)[n]) * dy).sum() / dy.sum() for n in range(len(want))])
+        scale = np.maximum(absmean, 1e-300)
+        err = (np.abs(got[f] - want) / scale).max()
+        say(f"  {f}: max |diff| / mean |{f}| {err:.3e}")
+        # FP64: summation order only; FP32: one rounding of each stored value or deviation,
+        # at most 2^-24 of |f| per cell
+        assert err <= (1e-14 if prec == abi.PN_FP64 else 1.2e-7), (f, err)
