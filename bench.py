@@ -57,6 +57,11 @@ VALU_PEAK_INSTS = 256 * 2.4e9
 ACCURACY_STEPS = 1000
 
 
+def progress(*a) -> None:
+    """One line per finished leg on stderr (the JSON result alone goes to stdout)."""
+    print("bench:", *a, file=sys.stderr, flush=True)
+
+
 def lab(state: np.ndarray, f: str) -> np.ndarray:
     """Field f with each ring's data rolled to lab columns (data j sits at j + rot_j)."""
     a = state[f]
@@ -268,6 +273,7 @@ def main_batch(a) -> dict:
     bl = batch_leg(ws, disks, a.steps, a.warmup, graph)
     value = cells * a.steps / bl["wall"]
     km = bl["km"]
+    progress(f"FP32 batch of {len(ws)} disks: {bl['wall'] / a.steps * 1e3:.4f} ms per step")
     if a.batch_only:
         return {"value": value, "ms_per_step": bl["wall"] / a.steps * 1e3, "km": km}
     # the plain FP64 path on the same disks, one after the other, same steps
@@ -275,6 +281,7 @@ def main_batch(a) -> dict:
     for w, d in zip(ws, disks):
         wall64 += run_ctx(w, d, abi.PN_FP64, a.steps, a.warmup, graph=graph)["wall"]
     v64 = cells * a.steps / wall64
+    progress(f"FP64 disks one by one: {wall64 / a.steps * 1e3:.4f} ms per step")
     dom = max(("sweep_x", "sweep_y"), key=lambda k: km[k])
     var = bl["variants"][0] if dom == "sweep_x" else bl["variants"][1]
     prof = counters(f"c4_batch_48_disks:fp32:{dom}:v{var}")
@@ -293,14 +300,18 @@ def main_batch(a) -> dict:
     }
     if not a.no_ref:
         out["accuracy_vs_fp64"] = batch_accuracy(ws, disks, ACCURACY_STEPS, graph, a.disk_report)
+        progress(f"batch accuracy after {ACCURACY_STEPS} steps done")
         c2 = configs.c2()
         sd = single_disk_rates(c2, a.steps, a.warmup, graph)
+        progress("1024 x 3072 disk rates done")
         sd.pop("_km32")
         sd.pop("_var32")
         sd["accuracy_vs_fp64"] = single_disk_accuracy(c2, ACCURACY_STEPS, graph, a.floor)
         out["single_disk"] = sd
+        progress("1024 x 3072 disk accuracy done")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ws, a.cpu_budget)
+        progress("CPU baseline done")
     return out
 
 

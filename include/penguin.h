@@ -13,7 +13,7 @@
  *   ops     get_dt                          include/timestep.h:4        (src/timestep/timestep.cu:112-149)
  *           Index_Shift                     include/orbital_advection.h:4 (src/orbital_advection/orbital_advection.cu:12-69)
  *           bundle_sweep2D                  include/ppm.h:4             (src/ppm/ppm.cu:358-386)
- *           GPU_output_reduction            include/output.h:32         (src/output/device_func.cu:155-304)
+ *           GPU_output_reduction            include/output.h:32         (src/output/device_func.cu:49-198)
  *           save_cells / copy_bfile         include/output.h:26         (src/output/output.cu:48-55, main.cu:37-42)
  *           init_planet / planet_forces /
  *           kick_drift_kick                 include/planet.h:7-9        (src/planet/planet.cu:36-93)
@@ -194,7 +194,7 @@ int pn_sweep2d(pn_ctx* ctx, double dt, const pn_body* planet, double FrRot);
  * drive a single direction (ppm.cu:26-160 or 162-307). */
 int pn_sweep_axis(pn_ctx* ctx, int axis, double dt, const pn_body* planet, double FrRot);
 
-/* GPU_output_reduction (device_func.cu:155-304): mode 0 = total mass,
+/* GPU_output_reduction (device_func.cu:49-198): mode 0 = total mass,
  * 8 / 9 = background-subtracted radial / azimuthal force on the planet
  * (2-D).  exclude is in units of sc_h as in the reference. */
 int pn_reduce(pn_ctx* ctx, const pn_body* planet, double exclude, int mode, double* out);
@@ -261,6 +261,12 @@ int pn_tune(pn_ctx* ctx, int nrep, int* var_x, int* var_y);
 int pn_set_variants(pn_ctx* ctx, int var_x, int var_y);
 /* The variants in use (after tuning or pinning; 0 for FP64 contexts). */
 int pn_get_variants(const pn_ctx* ctx, int* var_x, int* var_y);
+/* FP32 radial and theta sweeps: on (1, the default) the last window of a pencil, when it
+ * holds at most 32 K cells (K cells per lane), runs two lab columns per wavefront, one in
+ * each half; off (0) it runs one, like every other window.  The results are the same either
+ * way; the switch exists to show that.  A batch's disks follow their own contexts' setting
+ * (pn_batch_disk).  No counterpart in the reference. */
+int pn_set_packing(pn_ctx* ctx, int on);
 
 /* The achievable HBM bandwidth: a plain device copy of nbytes (a multiple of 16, at least
  * 1 MiB) launched reps times back to back after one warm-up, timed with HIP events on a
@@ -287,8 +293,9 @@ int pn_totals(pn_ctx* ctx, double* mass, double* angmom);
 typedef struct pn_batch pn_batch;
 
 /* n disks of grids[d] / params[d].  precision must be PN_FP32; every disk 2-D isothermal
- * with a planet that does not move (plnt_flag < 2).  On failure *out still holds a batch
- * whose pn_batch_last_error explains it; pn_batch_destroy it. */
+ * without damping zones; its planet fixed, growing or moving under the disk's forces
+ * (plnt_flag 0..2).  On failure *out still holds a batch whose pn_batch_last_error explains
+ * it; pn_batch_destroy it. */
 int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int precision, pn_batch** out);
 int pn_batch_destroy(pn_batch* b);
 const char* pn_batch_last_error(const pn_batch* b);
@@ -301,7 +308,12 @@ int pn_batch_run_begin(pn_batch* b, pn_run_state* states);
  * on pn_batch_disk first); states: pn_batch_size records. */
 int pn_batch_run_restart(pn_batch* b, int startat, pn_run_state* states);
 /* Advance every disk nsteps steps (states in and out as pn_run's); graph != 0 replays a
- * captured step.  Tunes the sweep shapes on the first call unless pinned. */
+ * captured step.  Tunes the sweep shapes on the first call unless pinned.  With land set in
+ * a disk's state, that disk's steps follow the endtime / tmovie landing of main.cu:352-366
+ * exactly as pn_run applies it, and the call returns after the first step in which any disk
+ * landed (its state's event says which; every disk has then taken the same number of steps).
+ * A disk that reached endtime must leave the batch: a further call with its land set fails
+ * for lack of time left. */
 int pn_batch_run(pn_batch* b, int nsteps, int graph, pn_run_state* states);
 /* Time every sweep variant over the whole batch (state unchanged) and keep the fastest;
  * pin them instead with pn_batch_set_variants (same indices as pn_set_variants). */

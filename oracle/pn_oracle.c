@@ -9,17 +9,17 @@
  *   window structure of the sweeps ......... src/ppm/ppm.cu:26-307
  *   ppmlr stage order ...................... src/ppm/ppmlr.cu:2-56
  *   flatten ................................ src/ppm/flatten.cu:7-40
- *   paraset / parabola ..................... src/ppm/parabola.cu:168-274
+ *   paraset / parabola ..................... src/ppm/parabola.cu:70-176
  *   states ................................. src/ppm/states.cu:3-73
  *   riemann (EOS 0, 1, 2) .................. src/ppm/riemann.cu:3-115
  *   evolve ................................. src/ppm/evolve.cu:3-166
  *   remap / get_flux ....................... src/ppm/remap.cu:1-226
  *   forces / potential ..................... src/ppm/forces.cu:4-372
- *   viscosity (derivs, r, phi) ............. src/ppm/viscosity.cu:2-19, 370-472
+ *   viscosity (derivs, r, phi, theta) ...... src/ppm/viscosity.cu:2-19, 370-518
  *   Index_Shift ............................ src/orbital_advection/orbital_advection.cu:12-43
  *   rotate_rings ........................... src/ppm/ppm.cu:309-322
  *   CFL dt ................................. src/timestep/timestep.cu:12-149
- *   mass / planet-force reductions ......... src/output/device_func.cu:155-238
+ *   mass / planet-force reductions ......... src/output/device_func.cu:49-198
  *   grid / get_vol ......................... src/shared/grid.cu:67-248
  *   disk profile ........................... src/disk_profile/disk_profile.cu:9-171
  *   initial state and ghost rings .......... init.cpp:3-543
@@ -738,7 +738,7 @@ static double max3(double A, double B, double C) {
   return (C > B) ? C : B;
 }
 
-/* paraset, parabola.cu:236-274 */
+/* paraset, parabola.cu:138-176 */
 static void paraset(win_t* W, int nmin, int nmax, const double* dx, double* a, double* ai, double* b) {
   for (int n = nmin; n < nmax - 1; n++) {
     double dx_n = dx[n];
@@ -772,7 +772,7 @@ static void paraset(win_t* W, int nmin, int nmax, const double* dx, double* a, d
   }
 }
 
-/* parabola, parabola.cu:168-231 */
+/* parabola, parabola.cu:70-136 */
 static void parabola(win_t* W, int nmin, int nmax, const double* flat, const double* val, double* al,
                      double* da, double* a6) {
   for (int n = nmin + 1; n < nmax - 1; n++) {
@@ -976,7 +976,7 @@ static void ppmlr_w(win_t* W, const wargs* A) {
     W->Sur[n] = tmp + hdt * W->ftot_s[n];
   }
 
-  /* ---- riemann (riemann.cu:89-127) */
+  /* ---- riemann (riemann.cu:3-115) */
   for (n = 3; n < nmax - 2; n++)
     orc_riemann(P->eos, gam, W->Spl[n], W->Srl[n], W->Sul[n], W->Spr[n], W->Srr[n], W->Sur[n], &W->umid[n],
                 &W->pmid[n]);
@@ -1085,7 +1085,7 @@ static void ppmlr_w(win_t* W, const wargs* A) {
     }
   }
 
-  /* ---- remap (remap.cu:182-394); al=tmp1, da=umid, a6=pmid */
+  /* ---- remap (remap.cu:14-226); al=tmp1, da=umid, a6=pmid */
   al = W->tmp1;
   da = W->umid;
   a6 = W->pmid;
@@ -1229,6 +1229,39 @@ static void visc_p(win_t* W, const pn_params* P, int lim, double dt, double R, c
     v[j] = W->ddv[j];
     if (P->ndim == 3) w[j] = W->ddw[j];
     u[j] = W->ddu[j];
+  }
+}
+
+/* device_viscosity_z, viscosity.cu:474-518: nu = vis_nu (no radial law) times rho; the
+   component along the pencil (w of the reference) takes 2 nu rho d2 + 2 dn d1, the two
+   transverse ones nu rho d2 + dn d1; no curvature terms.  The reference has no caller; the
+   theta sweep calls it with (u, v, w) = (v_r, v_phi, v_theta) and x = r times the theta left
+   edges (arc length, as the phi sweep passes r_cyl times its edges, ppm.cu:277-278).  The
+   reference updates 0 < k < arrsize - 1; only the stored cells [NPAD, lim - NPAD) matter. */
+static void visc_z(win_t* W, const pn_params* P, int lim, double dt, const double* x, const double* cell_r,
+                   double* u, double* v, double* w, double* nn) {
+  for (int k = 0; k < lim; k++) {
+    W->rho[k] = cell_r[k];
+    W->nr[k] = P->vis_nu * W->rho[k];
+    nn[k] = W->nr[k];
+  }
+  for (int k = NPAD; k < lim - NPAD; k++) {
+    double dn, d1, d2, nr = W->nr[k], rho = W->rho[k];
+    orc_derivs(x, nn, k, &dn, &d2);
+    orc_derivs(x, w, k, &d1, &d2);
+    double ddw = 2.0 * nr * d2 + 2.0 * dn * d1;
+    orc_derivs(x, u, k, &d1, &d2);
+    double ddu = nr * d2 + dn * d1;
+    orc_derivs(x, v, k, &d1, &d2);
+    double ddv = nr * d2 + dn * d1;
+    W->ddw[k] = w[k] + dt * ddw / rho;
+    W->ddu[k] = u[k] + dt * ddu / rho;
+    W->ddv[k] = v[k] + dt * ddv / rho;
+  }
+  for (int k = NPAD; k < lim - NPAD; k++) {
+    w[k] = W->ddw[k];
+    u[k] = W->ddu[k];
+    v[k] = W->ddv[k];
   }
 }
 
@@ -1506,7 +1539,8 @@ void orc_sweepy(const pn_grid* g, const pn_params* P, void* rings, double dt, co
    (init.cpp:415-541).  Pencil: one per (i, lab column j0) over theta rings
    k, ghost rings below / above, component swap u<-w (v_theta),
    v<-u (v_r), w<-v*rad_cyl (angular momentum about the axis, the
-   form get_fz's centrifugal term expects). */
+   form get_fz's centrifugal term expects).  With viscosity, the window then
+   takes device_viscosity_z (viscosity.cu:474-518) on (v_r, v_phi, v_theta). */
 void orc_sweepz(const pn_grid* g, const pn_params* P, void* rings, const void* udr, const void* top,
                 double dt, const pn_body* planet, double FrRot) {
   const int imax = g->imax, jmax = g->jmax, kmax = g->kmax;
@@ -1564,6 +1598,13 @@ void orc_sweepz(const pn_grid* g, const pn_params* P, void* rings, const void* u
             vrot[kk] = *R.res_v / radc[kk];
           }
           window_update(W, P, 2, lim, dt, FrRot, planet, rad, azi, pol, vrot, dvol0, xa0, dx0, r, p, u, v, w, e);
+          /* v_phi from the angular momentum the sweep carried */
+          double vphi[ARR];
+          for (int kk = 0; kk < lim; kk++) vphi[kk] = w[kk] / radc[kk];
+          if (P->visc_flag == 1) {
+            for (int kk = 0; kk < lim; kk++) xa0[kk] *= rad[kk];
+            visc_z(W, P, lim, dt, xa0, r, v, vphi, u, dx0);
+          }
           for (int kk = NPAD; kk < lim - NPAD; kk++) {
             int c = REAL * l + kk;
             ring_t R = cur[c];
@@ -1572,7 +1613,7 @@ void orc_sweepz(const pn_grid* g, const pn_params* P, void* rings, const void* u
             R.p[j] = eos_store_p(P, r[kk], p[kk], radc[kk]);
             R.w[j] = u[kk];
             R.u[j] = v[kk];
-            R.v[j] = w[kk] / radc[kk];
+            R.v[j] = vphi[kk];
           }
         }
       }
@@ -1651,7 +1692,7 @@ double orc_get_dt(const pn_grid* g, const pn_params* P, const void* rings, doubl
   return fmin(dt, dtc);
 }
 
-/* output_lv1 (device_func.cu:155-238), summed with Neumaier compensation */
+/* output_lv1 (device_func.cu:49-198), summed with Neumaier compensation */
 double orc_reduce(const pn_grid* g, const pn_params* P, const void* rings, const pn_body* planet,
                   double exclude, int mode) {
   long nring = (long)g->imax * g->kmax;

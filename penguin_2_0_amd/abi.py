@@ -110,6 +110,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_time_kernels": (i, [vp, i, P(d)]),
         "pn_tune": (i, [vp, i, P(i), P(i)]),
         "pn_set_variants": (i, [vp, i, i]),
+        "pn_set_packing": (i, [vp, i]),
         "pn_get_variants": (i, [vp, P(i), P(i)]),
         "pn_init_planet": (None, [P(PnParams), P(PnBody), P(d)]),
         "pn_planet_forces": (i, [vp, P(PnBody), d, d]),
@@ -341,6 +342,10 @@ class Context:
         m, l = C.c_double(), C.c_double()
         self._chk(self.lib.pn_totals(self.h, C.byref(m), C.byref(l)), "pn_totals")
         return m.value, l.value
+
+    def set_packing(self, on: bool):
+        """Two short last radial / theta windows per wavefront (pn_set_packing)."""
+        self._chk(self.lib.pn_set_packing(self.h, 1 if on else 0), "pn_set_packing")
 
     def ring_means(self, field: int | str) -> np.ndarray:
         """Azimuthal mean of a field per interior ring (pn_ring_means); field 0..4 or one of

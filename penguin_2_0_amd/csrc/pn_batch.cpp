@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <string>
@@ -78,7 +79,12 @@ int upload_desc(pn_batch* b) {
     D.p = c->dp;
     D.ss = c->ss;
     D.cflp = c->cflp;
-    D.ncfl_post = ny;
+    // a planet that moves changes FrRot inside the step: its CFL pass runs before the step
+    // (the next step's dt cannot come from the last sweep)
+    const bool moving = c->P.plnt_flag == 2;
+    D.ncfl_post = moving ? 0 : ny;
+    D.ncfl_pre = moving ? f32_cfl_blocks(c->dg) : 0;
+    if (D.ncfl_pre > c->ncflp) return bfail(b, PN_ERR_ARG, "disk %d: %d CFL partials exceed %d", d, D.ncfl_pre, c->ncflp);
     D.trig = c->df.trig;
     D.tab = c->df.tab;
     D.tstride = c->df.tstride;
@@ -103,13 +109,60 @@ int upload_desc(pn_batch* b) {
   return PN_OK;
 }
 
+// a disk's launch context on the batch's stream
+LaunchCtx on_batch(pn_batch* b, pn_ctx* c) {
+  LaunchCtx L = lctx(c);
+  L.stream = b->stream;
+  return L;
+}
+
+// one step of every disk: the CFL pass of each disk whose planet moves, the batched pre-step,
+// sweeps and post-step, then each moving planet's time, drift, disk forces (modes 8 / 9 from
+// its FP32 state) and closing kick -- the order pn_run gives a single disk
 int launch_batch_step(pn_batch* b, int cur) {
   const int n = ndisk(b);
-  hipError_t e = launch_pre_step_batch(b->stream, b->dd, n);
+  hipError_t e = hipSuccess;
+  for (pn_ctx* c : b->disks) {
+    if (c->P.plnt_flag != 2 || e != hipSuccess) continue;
+    int ncfl = 0;
+    e = f32_cfl(on_batch(b, c), c->df, cur, &ncfl);
+  }
+  if (e == hipSuccess) e = launch_pre_step_batch(b->stream, b->dd, n);
   if (e == hipSuccess) e = f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x);
   if (e == hipSuccess) e = f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur ^ 1, b->grid_y);
   if (e == hipSuccess) e = launch_post_step_batch(b->stream, b->dd, n);
+  for (pn_ctx* c : b->disks) {
+    if (c->P.plnt_flag != 2 || e != hipSuccess) continue;
+    e = launch_planet_after_sweep(on_batch(b, c), cur, &c->df);
+  }
   if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "batched step: %s", hipGetErrorString(e));
+  return PN_OK;
+}
+
+// n steps of every disk from buffer cur (a 2-D step returns there): replays of the step
+// captured as a graph from that buffer, or direct launches
+int batch_steps(pn_batch* b, int cur, int nsteps, int graph) {
+  int done = 0, rc;
+  if (graph && nsteps > 0) {
+    // the captured step names its buffer; a disk moved to the other one since (for example by
+    // pn_sweep_axis on pn_batch_disk) needs a step captured from there
+    if (b->gexec && b->gexec_cur != cur) drop_graph(b);
+    if (!b->gexec) {
+      hipGraph_t gr;
+      PB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      rc = launch_batch_step(b, cur);
+      hipError_t e = hipStreamEndCapture(b->stream, &gr);
+      if (rc != PN_OK) return rc;
+      if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&b->gexec, gr, nullptr, nullptr, 0);
+      hipGraphDestroy(gr);
+      if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      b->gexec_cur = cur;
+    }
+    for (; done < nsteps; ++done) PB_HIP(b, hipGraphLaunch(b->gexec, b->stream));
+  }
+  for (; done < nsteps; ++done)
+    if ((rc = launch_batch_step(b, cur)) != PN_OK) return rc;
   return PN_OK;
 }
 
@@ -173,8 +226,6 @@ int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int pr
   for (int d = 0; d < n; ++d) {
     if (params[d].ndim != 2)
       return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches are built for 2-D disks", d);
-    if (params[d].plnt_flag == 2)
-      return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: a moving planet needs its force reductions between sweeps", d);
     if (params[d].eos != 0)
       return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches are built for the isothermal closure (EOS 0)", d);
     if (params[d].kill_flag)
@@ -195,8 +246,8 @@ int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int pr
 
 int pn_batch_destroy(pn_batch* b) {
   if (!b) return PN_OK;
-  drop_graph(b);
   if (b->stream) hipStreamSynchronize(b->stream);
+  drop_graph(b);
   for (pn_ctx* c : b->disks) pn_destroy(c);
   if (b->dd) hipFree(b->dd);
   if (b->dfs) hipFree(b->dfs);
@@ -294,57 +345,78 @@ int pn_batch_run(pn_batch* b, int nsteps, int graph, pn_run_state* states) {
   if (!b || !states || nsteps < 0) return PN_ERR_ARG;
   int cur, rc;
   if ((rc = common_cur(b, &cur)) != PN_OK) return rc;
-  for (int d = 0; d < ndisk(b); ++d)
-    if (states[d].land)
-      return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: the endtime / tmovie landing is built for single contexts", d);
+  const int n = ndisk(b);
+  bool land = false;
+  for (int d = 0; d < n; ++d) {
+    if (!states[d].land) continue;
+    land = true;
+    // as pn_run: a landing needs time left before both marks
+    const pn_params& P = b->disks[d]->P;
+    if (!(states[d].timem < P.tmovie && states[d].simtime < P.endtime))
+      return bfail(b, PN_ERR_ARG, "disk %d: landing with no time left: timem %g of tmovie %g, simtime %g of endtime %g",
+                   d, states[d].timem, P.tmovie, states[d].simtime, P.endtime);
+  }
   if (!b->tuned && (rc = pn_batch_tune(b, 3, nullptr, nullptr)) != PN_OK) return rc;
   if ((rc = upload_desc(b)) != PN_OK) return rc;
-  // every disk's scalars, and its first dt from its state (the fused CFL supplies the rest)
-  for (int d = 0; d < ndisk(b); ++d) {
+  // every disk's scalars; a disk whose CFL is fused into its last sweep takes the first dt
+  // from its state here (the sweeps supply the rest), one whose planet moves in its pre-step
+  std::vector<StepScalars> h(n);
+  for (int d = 0; d < n; ++d) {
     pn_ctx* c = b->disks[d];
-    StepScalars h{};
-    h.dt = states[d].dt;
-    h.FrRot = states[d].FrRot;
-    h.simtime = states[d].simtime;
-    h.planet = states[d].planet;
-    h.ncycle = states[d].ncycle;
-    if ((rc = upload_scalars(c, h)) != PN_OK || (nsteps > 0 && (rc = launch_cfl_any(c, cur)) != PN_OK) ||
-        (rc = sync(c)) != PN_OK)
+    h[d].dt = states[d].dt;
+    h[d].FrRot = states[d].FrRot;
+    h[d].simtime = states[d].simtime;
+    h[d].planet = states[d].planet;
+    h[d].ncycle = states[d].ncycle;
+    h[d].timem = states[d].timem;
+    h[d].land = states[d].land ? 1 : 0;
+    h[d].event = 0;
+    if ((rc = upload_scalars(c, h[d])) != PN_OK ||
+        (nsteps > 0 && fused_cfl(c) && (rc = launch_cfl_any(c, cur)) != PN_OK) || (rc = sync(c)) != PN_OK)
       return bfail(b, rc, "disk %d: %s", d, c->err.c_str());
   }
-  int done = 0;
-  if (graph && nsteps > 0) {
-    // the captured step names its buffer; a disk moved to the other one since (for example by
-    // pn_sweep_axis on pn_batch_disk) needs a step captured from there
-    if (b->gexec && b->gexec_cur != cur) drop_graph(b);
-    if (!b->gexec) {
-      hipGraph_t gr;
-      PB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      rc = launch_batch_step(b, cur);
-      hipError_t e = hipStreamEndCapture(b->stream, &gr);
-      if (rc != PN_OK) return rc;
-      if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-      e = hipGraphInstantiate(&b->gexec, gr, nullptr, nullptr, 0);
-      hipGraphDestroy(gr);
-      if (e != hipSuccess) return bfail(b, PN_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-      b->gexec_cur = cur;
+  auto download_all = [&]() -> int {
+    for (int d = 0; d < n; ++d) {
+      pn_ctx* c = b->disks[d];
+      const int r = download_scalars(c, &h[d]);
+      if (r != PN_OK) return bfail(b, r, "disk %d: %s", d, c->err.c_str());
     }
-    for (; done < nsteps; ++done) PB_HIP(b, hipGraphLaunch(b->gexec, b->stream));
+    return PN_OK;
+  };
+  if (!land) {
+    if ((rc = batch_steps(b, cur, nsteps, graph)) != PN_OK) return rc;
+  } else {
+    // the landing is applied on the device, per disk, every step; the host stops after a step
+    // in which any disk landed, enqueuing at once only the steps that cannot reach one
+    int done = 0;
+    while (done < nsteps) {
+      int m = nsteps - done;
+      for (int d = 0; d < n; ++d) {
+        if (!h[d].land) continue;
+        const pn_params& P = b->disks[d]->P;
+        const double room = std::fmin(P.tmovie - h[d].timem, P.endtime - h[d].simtime);
+        m = std::min(m, std::max(1, steps_before_landing(h[d].dt, room, nsteps - done)));
+      }
+      if ((rc = batch_steps(b, cur, m, graph)) != PN_OK) return rc;
+      done += m;
+      if ((rc = download_all()) != PN_OK) return rc;
+      bool stop = false;
+      for (int d = 0; d < n; ++d) stop = stop || h[d].event != 0 || h[d].bad;
+      if (stop) break;
+    }
   }
-  for (; done < nsteps; ++done)
-    if ((rc = launch_batch_step(b, cur)) != PN_OK) return rc;
   PB_HIP(b, hipStreamSynchronize(b->stream));
-  for (int d = 0; d < ndisk(b); ++d) {
-    pn_ctx* c = b->disks[d];
-    StepScalars h{};
-    if ((rc = download_scalars(c, &h)) != PN_OK) return bfail(b, rc, "disk %d: %s", d, c->err.c_str());
-    // the record already holds the next step's dt; report the last one used
-    states[d].dt = nsteps > 0 ? h.dt_prev : h.dt;
-    states[d].FrRot = h.FrRot;
-    states[d].simtime = h.simtime;
-    states[d].planet = h.planet;
-    states[d].ncycle = h.ncycle;
-    if (h.bad) return bfail(b, PN_ERR_NUMERIC, "disk %d: non-finite or non-positive dt during the run", d);
+  if ((rc = download_all()) != PN_OK) return rc;
+  for (int d = 0; d < n; ++d) {
+    // a fused CFL leaves the next step's dt in the record: report the last one used
+    states[d].dt = (fused_cfl(b->disks[d]) && nsteps > 0) ? h[d].dt_prev : h[d].dt;
+    states[d].FrRot = h[d].FrRot;
+    states[d].simtime = h[d].simtime;
+    states[d].planet = h[d].planet;
+    states[d].ncycle = h[d].ncycle;
+    states[d].timem = h[d].timem;
+    states[d].event = h[d].event;
+    if (h[d].bad) return bfail(b, PN_ERR_NUMERIC, "disk %d: non-finite or non-positive dt during the run", d);
   }
   return PN_OK;
 }

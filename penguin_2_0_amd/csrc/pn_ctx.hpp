@@ -20,6 +20,7 @@ struct pn_ctx {
   pn::DevParams dp{};
   pn::DevF32 df{};                    // FP32 state and tables (FP32 contexts)
   int var_x = 0, var_y = 3;         // FP32 sweep variants (indices into kF32VariantsX / Y)
+  int pack_tails = 1;               // FP32 radial / theta: two short last windows per wavefront
   bool tuned = false;
   pn::StepScalars* ss = nullptr;      // device
   double* partials = nullptr;       // device
@@ -53,6 +54,8 @@ int upload_scalars(pn_ctx* c, const StepScalars& h);
 int download_scalars(pn_ctx* c, StepScalars* h);
 int sync(pn_ctx* c);
 int launch_cfl_any(pn_ctx* c, int src);
+// steps from dt0 that cannot reach a landing room away (dt grows by at most 1.1 per step)
+int steps_before_landing(double dt0, double room, int cap);
 
 // mean time of nrep back-to-back launches, measured with HIP events on the context's stream
 template <class F>

@@ -13,7 +13,7 @@ namespace pn {
 //   c_s^2, c_s, L_b, L_b(s+1) - L_b(s),
 //   res_bg, a_bg, dp_bm = p_b(s) - p_b(s-1), f_face,
 //   L_f = (L_b(s) + L_b(s-1))/2, 1/xa0^3, vrot = res_v / r (rewritten each step), dvol0,
-//   nu, (nu rho_b)', Omega_b', Omega_b'', and c0..c4 of paraset (parabola.cu:236-274) on the
+//   nu, (nu rho_b)', Omega_b', Omega_b'', and c0..c4 of paraset (parabola.cu:138-176) on the
 //   slot widths
 // with, in FP64: a_bg = dp_bm V0 / (rho_b dvol0), V0 = (xa0(s) + xa0(s+1))/2, the pressure
 // acceleration of the background; res_bg = -1/rad^2 + L_b^2/rad^3 - a_bg, the background's
@@ -71,6 +71,8 @@ struct DevF32 {
   int nwin_x, nwin_y, nwin_z;  // windows of the variant in use
   int uniform_phi;             // every dphi equal: the azimuthal paraset is (1/2, -1/6, 1/6, 1/2, 1/2)
   int uniform_theta;           // every dtheta equal (3-D)
+  int pack_tails;              // radial / theta: a last window of at most 32 K cells runs two
+                               // columns per wavefront (same results, half the passes)
 };
 
 constexpr int kNumF32Variants = 4;
@@ -114,6 +116,8 @@ hipError_t f32_sweep_y_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs
                              int variant, int src, int grid);
 // CFL maxima per block into c.cflp; returns the partial count through *n
 hipError_t f32_cfl(const LaunchCtx& c, const DevF32& f, int src, int* n);
+// CFL partials f32_cfl writes for this grid
+int f32_cfl_blocks(const DevGrid& g);
 hipError_t f32_reduce(const LaunchCtx& c, const DevF32& f, int src, int mode, double exclude, int slot);
 hipError_t f32_totals(const LaunchCtx& c, const DevF32& f, int src, double* out2);
 // azimuthal mean of field (0 Sigma, 1 P, 2 v_r, 3 v_phi, 4 v_theta) per interior ring into out

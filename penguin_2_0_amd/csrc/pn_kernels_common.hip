@@ -6,6 +6,7 @@
 
 namespace pn {
 
+
 namespace {
 
 constexpr double kPi = 3.1415926535897932384626433832795;
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(256) void k_ring_means(DevGrid g, int src, int fiel
   if (threadIdx.x == 0) out[n] = ((sh[0] + sh[1]) + (sh[2] + sh[3])) / ((sh[4] + sh[5]) + (sh[6] + sh[7]));
 }
 
-// dt = min(1.1 dt_prev, courant / max)  (timestep.cu:194, 224-230), max over n partials
+// dt = min(1.1 dt_prev, courant / max)  (timestep.cu:135-141), max over n partials
 __global__ __launch_bounds__(256) void k_dt_finalize(DevParams prm, StepScalars* ss, const double* __restrict__ cflp,
                                                      int n) {
   __shared__ double sh[4];
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(256) void k_dt_finalize(DevParams prm, StepScalars*
   ss->dt = dt;
 }
 
-// output_lv1 (device_func.cu:155-238): per-block sums in a fixed order
+// output_lv1 (device_func.cu:49-198): per-block sums in a fixed order
 __global__ __launch_bounds__(256) void k_reduce(DevGrid g, DevParams prm, const StepScalars* __restrict__ ss,
                                                 int src, int mode, double exclude, double* partials) {
   __shared__ double sh[4];
@@ -379,7 +380,7 @@ __device__ double block_max_partials(const double* __restrict__ cflp, int n, dou
   return m;
 }
 
-// thread 0: dt = min(1.1 dt_prev, courant / max)  (timestep.cu:194, 224-230)
+// thread 0: dt = min(1.1 dt_prev, courant / max)  (timestep.cu:135-141)
 __device__ void finalize_dt(const DevParams& prm, StepScalars* ss, double mx) {
   ss->cfl_max = mx;
   double dt = fmin(1.1 * ss->dt, prm.courant / mx);
@@ -527,29 +528,33 @@ __global__ __launch_bounds__(256) void k_pre_rings(DevGrid g, DevParams prm, con
   ring_slot(g, prm, ss->dt, ss->fr_shift, ss->planet.vy, tab, tabz, tstride, s);
 }
 
-// one workgroup per disk of a batch (the fused CFL leaves dt already set: ncfl 0)
+// one workgroup per disk of a batch (the fused CFL leaves dt already set: ncfl_pre 0; a
+// planet that moves has its CFL partials from the pass before the step)
 __global__ __launch_bounds__(kStepThreads) void k_pre_step_batch(const DiskDev* __restrict__ dd) {
   const DiskDev& D = dd[blockIdx.x];
-  pre_step_body(D.g, D.p, D.ss, D.cflp, 0, D.trig, D.tab, nullptr, D.tstride, true);
+  pre_step_body(D.g, D.p, D.ss, D.cflp, D.ncfl_pre, D.trig, D.tab, nullptr, D.tstride, true);
 }
 
 // Everything of a step after the sweeps for a planet that does not feel the disk
 // (plnt_flag < 2), in one workgroup: the integer roll of every ring (ppm.cu:309-322), time
 // advance, drift, forces and closing half kick (main.cu:378-387, planet.cu:54-93), then,
 // with ncfl > 0, the next step's dt from the CFL partials the last sweep wrote.
+// the integer roll of every ring (ppm.cu:309-322) by one workgroup
+__device__ __forceinline__ void roll_rings(DevGrid g) {
+  const int nring = g.imax * g.kmax;
+  for (int n = threadIdx.x; n < nring; n += blockDim.x) {
+    const int s = slot_of(g, n % g.imax, n / g.imax);
+    int j = g.rot_j[s] + g.inc_j[s];
+    if (j < 0) j += g.jmax;
+    if (j >= g.jmax) j -= g.jmax;
+    g.rot_j[s] = j;
+  }
+}
+
 __device__ __forceinline__ void post_step_body(DevGrid g, const DevParams& prm, StepScalars* ss,
                                                const double* __restrict__ cflp, int ncfl, bool roll) {
   __shared__ double sh[kStepThreads / 64];
-  if (roll && prm.fargo > 0) {
-    const int nring = g.imax * g.kmax;
-    for (int n = threadIdx.x; n < nring; n += blockDim.x) {
-      const int s = slot_of(g, n % g.imax, n / g.imax);
-      int j = g.rot_j[s] + g.inc_j[s];
-      if (j < 0) j += g.jmax;
-      if (j >= g.jmax) j -= g.jmax;
-      g.rot_j[s] = j;
-    }
-  }
+  if (roll && prm.fargo > 0) roll_rings(g);
   double mx = 0.0;
   if (ncfl > 0) mx = block_max_partials(cflp, ncfl, sh);
   if (threadIdx.x != 0) return;
@@ -597,8 +602,14 @@ __global__ __launch_bounds__(256) void k_max_chunks(const double* __restrict__ i
   if (threadIdx.x == 0) out[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
+// one workgroup per disk of a batch; a planet that moves takes only the roll here, its time,
+// drift, disk forces and closing kick following from launch_planet_after_sweep
 __global__ __launch_bounds__(kStepThreads) void k_post_step_batch(const DiskDev* __restrict__ dd) {
   const DiskDev& D = dd[blockIdx.x];
+  if (D.p.plnt == 2) {
+    if (D.p.fargo > 0) roll_rings(D.g);
+    return;
+  }
   post_step_body(D.g, D.p, D.ss, D.cflp, D.ncfl_post, true);
 }
 

@@ -10,6 +10,8 @@
 // Work decomposition as in the FP64 sweeps, with K cells per lane (window 64 K, halo 6) and
 // the workgroup shape picked by the autotuner; the CFL maximum of the next step is taken in
 // the azimuthal sweep's epilogue from the values it stores, one partial per workgroup.
+#include <type_traits>
+
 #include "pn_f32.hpp"
 #include "ppm_wave_f32.hpp"
 
@@ -202,13 +204,24 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
   // theta: the theta-major table, whose slots run along the pencil (reads past the window
   // land on the next ring's slots or the overhang: finite values no kept cell depends on)
   const unsigned tbytes = 4u * (unsigned)kTabEntries * (unsigned)f.tstride;
-  const TabLane<K> tx{Buf(TH ? f.tabz : f.tab, tbytes),
-                      4u * (unsigned)((TH ? (REAL * l - kNpad + g.k0) + g.sk * (row + kNpad) : s0) + K * lane),
-                      f.tstride};
-  for (int qq = 0; qq < Q; ++qq) {
-    const int q = wv * Q + qq;
-    const int j0 = jb + q;
-    if (j0 >= jmax) break;   // wave-uniform
+  // A window of at most half a wavefront's cells (the short last window of a pencil) runs
+  // 64 / SL = 2 columns per wavefront pass, one in each half of SL lanes; every stage indexes
+  // cells within its own window (cell_in<SL>), so each column's result is the one-column
+  // pass's.  Column group gi = the NSEG columns gi NSEG + seg; wave wv takes groups
+  // wv QP .. wv QP + QP - 1.
+  auto pencils = [&](auto sl_tag) {
+    constexpr int SL = decltype(sl_tag)::value, NSEG = 64 / SL, QP = (Q + NSEG - 1) / NSEG;
+    const int seg = lane / SL, ll = lane & (SL - 1);
+    const TabLane<K> tx{Buf(TH ? f.tabz : f.tab, tbytes),
+                        4u * (unsigned)((TH ? (REAL * l - kNpad + g.k0) + g.sk * (row + kNpad) : s0) + K * ll),
+                        f.tstride};
+  for (int qq = 0; qq < QP; ++qq) {
+    const int gi = wv * QP + qq;
+    if (gi * NSEG >= NC || jb + gi * NSEG >= jmax) break;   // wave-uniform
+    const int q = gi * NSEG + seg;
+    // a second column past jmax (the last group) computes on column jmax - 1's copies and is
+    // never stored
+    const int j0 = jb + q < jmax ? jb + q : jmax - 1;
     WinP<K> Wp;
     Wp.rad = tx.get(kTabRad);
     if constexpr (TH) {
@@ -246,10 +259,10 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
     Wp.ts = splat<float, K>(tr.x);
     Wp.tc = splat<float, K>(tr.y);
     const C<float, K> rc = tx.get(kTabRc);
-    const C<float, K> f0 = lds_k<K>(&tile[0][q][K * lane]), f1 = lds_k<K>(&tile[1][q][K * lane]),
-                      f2 = lds_k<K>(&tile[2][q][K * lane]);
+    const C<float, K> f0 = lds_k<K>(&tile[0][q][K * ll]), f1 = lds_k<K>(&tile[1][q][K * ll]),
+                      f2 = lds_k<K>(&tile[2][q][K * ll]);
     C<float, K> f3;
-    if constexpr (NF == 4) f3 = lds_k<K>(&tile[3][q][K * lane]);
+    if constexpr (NF == 4) f3 = lds_k<K>(&tile[3][q][K * ll]);
     Wp.r = f0;
 #pragma unroll
     for (int h = 0; h < K; ++h) {
@@ -267,7 +280,7 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
     }
     Para<float, K> P0;
     if constexpr (TH) {
-      P0 = f.uniform_theta ? paraset_uniform<K>() : paraset(0, lim, Wp.dx0);
+      P0 = f.uniform_theta ? paraset_uniform<K>() : paraset<float, K, SL>(0, lim, Wp.dx0);
     } else {
       P0.c0 = tx.get(kTabP0);
       P0.c1 = tx.get(kTabP1);
@@ -275,17 +288,18 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
       P0.c3 = tx.get(kTabP3);
       P0.c4 = tx.get(kTabP4);
     }
-    ppmlr_pert<K, G, (G == kGeoRcyl ? 2 : 3), TabLane<K>, ADIA>(Wp, tx, P0, lim, dt, 0.0f, pl, prm.plnt > 0,
-                                                                 prm.flat != 0, sqrt_gam, 0.0f, false, (float)prm.gam);
+    ppmlr_pert<K, G, (G == kGeoRcyl ? 2 : 3), TabLane<K>, ADIA, SL>(Wp, tx, P0, lim, dt, 0.0f, pl, prm.plnt > 0,
+                                                                     prm.flat != 0, sqrt_gam, 0.0f, false,
+                                                                     (float)prm.gam);
     if constexpr (G == kGeoRcyl)
-      if (VISC == 1 || (VISC == 2 && prm.visc)) visc_r_pert<K>(Wp, tx, lim, dt);
+      if (VISC == 1 || (VISC == 2 && prm.visc)) visc_r_pert<K, SL>(Wp, tx, lim, dt);
     const C<float, K> irc = tx.get(kTabIrc);
     C<float, K> ixc;
     if constexpr (G == kGeoRsph) ixc = tx.get(kTabIxc);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
-      const int c = K * lane + h;
-      if (c >= kNpad && c < lim - kNpad) {   // only this wave touches column q
+      const int c = K * ll + h;
+      if (c >= kNpad && c < lim - kNpad) {   // only this lane's half-wave touches column q
         tile[0][q][c] = Wp.r[h];
         if constexpr (TH) {
           tile[3][q][c] = Wp.u[h];
@@ -300,6 +314,9 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
       }
     }
   }
+  };
+  if (f.pack_tails && lim <= WIN / 2) pencils(std::integral_constant<int, 32>{});
+  else pencils(std::integral_constant<int, 64>{});
   __syncthreads();
   float m = 0.0f;
   const double FrRot = ss->FrRot;
@@ -611,7 +628,7 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// mass (mode 0) and the planet-force integrands (modes 8, 9) of device_func.cu:155-238 on
+// mass (mode 0) and the planet-force integrands (modes 8, 9) of device_func.cu:49-198 on
 // the FP32 state, in FP64 with the FP64 background added back
 __global__ __launch_bounds__(256) void k_reduce_f32(DevGrid g, DevF32 f, DevParams prm,
                                                     const StepScalars* __restrict__ ss, int src, int mode,
@@ -975,6 +992,8 @@ hipError_t f32_sweep_y_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs
   }
   return hipErrorInvalidValue;
 }
+
+int f32_cfl_blocks(const DevGrid& g) { return red_blocks(g); }
 
 hipError_t f32_cfl(const LaunchCtx& c, const DevF32& f, int src, int* n) {
   const int nb = red_blocks(c.g);

@@ -241,7 +241,9 @@ __global__ __launch_bounds__(256) void k_sweep_y(DevGrid g, DevParams prm, const
 // no caller for it (main.cu:376); it is assembled from the theta pieces it holds
 // (forces.cu:298-372, states.cu:29-39, evolve.cu:26-29, 63-67, 110-124, remap.cu:50-54,
 // init.cpp:415-541).  Sweep components: u <- v_theta (w), v <- v_r (u),
-// w <- v_phi * r sin(theta) (the angular momentum get_fz's centrifugal term expects).
+// w <- v_phi * r sin(theta) (the angular momentum get_fz's centrifugal term expects).  With
+// viscosity the window then takes device_viscosity_z (viscosity.cu:474-518) on
+// (v_r, v_phi, v_theta).
 template <class CF>
 __global__ __launch_bounds__(256) void k_sweep_z(DevGrid g, DevParams prm, const StepScalars* __restrict__ ss,
                                                  int src) {
@@ -289,6 +291,16 @@ __global__ __launch_bounds__(256) void k_sweep_z(DevGrid g, DevParams prm, const
   }
 
   ppmlr_wave<double, kK, CF>(W, lim, dt, FrRot, pl, rf, prm.gam);
+  // v_phi from the angular momentum the sweep carried
+  C<double, kK> vphi;
+#pragma unroll
+  for (int h = 0; h < kK; ++h) vphi[h] = W.w[h] / radc[h];
+  if constexpr (CF::VISC) {
+    C<double, kK> x;
+#pragma unroll
+    for (int h = 0; h < kK; ++h) x[h] = W.xa0[h] * W.rad[h];
+    visc_z<double, kK>(W.r, lim, dt, prm.vis_nu, x, W.v, vphi, W.u);
+  }
 
 #pragma unroll
   for (int h = 0; h < kK; ++h) {
@@ -306,7 +318,7 @@ __global__ __launch_bounds__(256) void k_sweep_z(DevGrid g, DevParams prm, const
       g.f[src ^ 1][FP][a] = p;
       g.f[src ^ 1][FW][a] = W.u[h];
       g.f[src ^ 1][FU][a] = W.v[h];
-      g.f[src ^ 1][FV][a] = W.w[h] / radc[h];
+      g.f[src ^ 1][FV][a] = vphi[h];
     }
   }
 }
@@ -337,10 +349,11 @@ hipError_t dispatch(const LaunchCtx& c, int src, long waves) {
       default: return hipErrorInvalidValue;
     }
   } else if (p.ngeomx == 2 && p.ngeomy == 4 && p.ngeomz == 5) {
-    if (p.visc) return hipErrorInvalidValue;
-    switch (p.eos) {
+    switch (p.eos * 2 + (p.visc ? 1 : 0)) {
       case 0: PN_LAUNCH(2, 4, 5, 0, 0); break;
-      case 2: PN_LAUNCH(2, 4, 5, 2, 0); break;
+      case 1: PN_LAUNCH(2, 4, 5, 0, 1); break;
+      case 4: PN_LAUNCH(2, 4, 5, 2, 0); break;
+      case 5: PN_LAUNCH(2, 4, 5, 2, 1); break;
       default: return hipErrorInvalidValue;
     }
   } else {

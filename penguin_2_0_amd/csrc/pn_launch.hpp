@@ -26,6 +26,8 @@ struct DiskDev {
   StepScalars* ss;
   double* cflp;      // this disk's CFL partials
   int ncfl_post;     // partials the post-step reduces into the next dt (fused CFL), or 0
+  int ncfl_pre;      // partials the pre-step reduces into this step's dt (a planet that moves:
+                     // the CFL pass runs before the step, as FrRot changes inside it), or 0
   float2* trig;      // FP32 per-step tables, or null
   float* tab;        // the FP32 per-slot table (per-step rows rewritten by the pre-step), or null
   int tstride;

@@ -101,7 +101,8 @@ int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision)
   const bool sph = (p->ngeomx == 2 && p->ngeomy == 4 && p->ngeomz == 5);
   if (p->ndim == 2 && !cyl) return fail(c, PN_ERR_UNSUPPORTED, "2-D needs cylindrical geometry (1,3,0)");
   if (p->ndim == 3 && !sph) return fail(c, PN_ERR_UNSUPPORTED, "3-D needs spherical geometry (2,4,5)");
-  if (p->ndim == 3 && p->visc_flag) return fail(c, PN_ERR_UNSUPPORTED, "3-D viscosity is not built");
+  if (precision == PN_FP32 && p->ndim == 3 && p->visc_flag)
+    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for the inviscid 3-D disk");
   if (p->ndim == 3 && (p->eos == 1 || p->eos == 3)) return fail(c, PN_ERR_UNSUPPORTED, "3-D EOS %d is not built", p->eos);
   if (p->fargo_flag < 0 || p->fargo_flag > 2) return fail(c, PN_ERR_ARG, "fargo_flag %d", p->fargo_flag);
   if (p->plnt_flag < 0 || p->plnt_flag > 2) return fail(c, PN_ERR_ARG, "plnt_flag %d", p->plnt_flag);
@@ -340,8 +341,9 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
 
 int pn_destroy(pn_ctx* c) {
   if (!c) return PN_OK;
-  if (c->gexec) hipGraphExecDestroy(c->gexec);
+  // drain first: no replay of the graph, and no kernel reading these buffers, may still run
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->gexec) hipGraphExecDestroy(c->gexec);
   for (void* q : c->allocs) hipFree(q);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -570,7 +572,7 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
         q[kTabD1Ob] = d1;
         q[kTabD2Ob] = d2;
       }
-      // paraset (parabola.cu:236-274) on the row's widths; neighbours past the row repeat
+      // paraset (parabola.cu:138-176) on the row's widths; neighbours past the row repeat
       // its end widths (those coefficients never reach a kept parabola)
       {
         auto w = [&](long t) { return sdx[kz * sr + std::min(std::max(t - kz * sr, 0L), sr - 1)]; };
@@ -653,6 +655,7 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
     c->df.uniform_theta = d3 ? 1 : 0;
     for (long s = 1; d3 && s < nslot; ++s)
       if (sdz[s] != sdz[0]) c->df.uniform_theta = 0;
+    c->df.pack_tails = c->pack_tails;
     // the host vectors above must outlive the copies
     PN_HIP(c, hipStreamSynchronize(st));
   }
@@ -982,6 +985,17 @@ int pn_get_variants(const pn_ctx* c, int* var_x, int* var_y) {
   return PN_OK;
 }
 
+int pn_set_packing(pn_ctx* c, int on) {
+  if (!c || (on != 0 && on != 1)) return PN_ERR_ARG;
+  c->pack_tails = on;
+  c->df.pack_tails = on;
+  if (c->gexec) {   // the captured launches carry the old flag
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  return PN_OK;
+}
+
 int pn_set_variants(pn_ctx* c, int var_x, int var_y) {
   if (!c) return PN_ERR_ARG;
   if (!is_f32(c)) return (var_x == 0 && var_y == 0) ? PN_OK : fail(c, PN_ERR_ARG, "the FP64 path has one variant");
@@ -1143,6 +1157,8 @@ int run_steps(pn_ctx* c, int nsteps, int graph) {
   return PN_OK;
 }
 
+}  // namespace
+
 // steps that cannot reach a landing from dt0: each dt is at most 1.1 times the one before,
 // so n steps span at most dt0 (1.1 + ... + 1.1^n); room is the time to the nearer of the
 // next picture and endtime (a small margin covers the rounding of the device's sums)
@@ -1158,7 +1174,6 @@ int steps_before_landing(double dt0, double room, int cap) {
   return n;
 }
 
-}  // namespace
 }  // namespace rt
 }  // namespace pn
 

@@ -102,6 +102,12 @@ __device__ __forceinline__ C<T, K> next2(const C<T, K>& x) {
 }
 
 __device__ __forceinline__ int cell_of(int K, int h) { return K * (int)(threadIdx.x & 63) + h; }
+// the cell index within a window when a wavefront holds 64 / SL windows side by side, SL
+// lanes each (SL = 64: one window per wavefront, cell_of).  Stencils reach across the seam
+// between two windows exactly as they reach past a window's last cell: every stage's index
+// range keeps such values out of the cells a window stores.
+template <int SL>
+__device__ __forceinline__ int cell_in(int K, int h) { return K * (int)(threadIdx.x & (SL - 1)) + h; }
 
 template <typename T>
 __device__ __forceinline__ T min3(T A, T B, T Cc) {   // simple_func.cu:2-14
@@ -291,8 +297,8 @@ struct Para {
   C<T, K> c0, c1, c2, c3, c4;
 };
 
-// paraset (parabola.cu:236-274) on the index range [nmin, nmaxp)
-template <typename T, int K>
+// paraset (parabola.cu:138-176) on the index range [nmin, nmaxp)
+template <typename T, int K, int SL = 64>
 __device__ __forceinline__ Para<T, K> paraset(int nmin, int nmaxp, const C<T, K>& dx) {
   Para<T, K> P;
   C<T, K> dxp = next1(dx);
@@ -309,7 +315,7 @@ __device__ __forceinline__ Para<T, K> paraset(int nmin, int nmaxp, const C<T, K>
   C<T, K> anm = prev1(an), anp = next1(an), bnm = prev1(bn), ainm = prev1(ain);
 #pragma unroll
   for (int h = 0; h < K; ++h) {
-    const int c = cell_of(K, h);
+    const int c = cell_in<SL>(K, h);
     T c0 = T(0), c1 = T(0), c2 = T(0), c3 = T(0), c4 = T(0);
     if (c >= nmin + 1 && c < nmaxp - 2) {
       T d = T(1.0) / (anm[h] + anp[h]);
@@ -332,8 +338,8 @@ struct Parab {
   C<T, K> al, da, a6;   // left value, jump, curvature; valid on [nmin+2, nmaxp-2)
 };
 
-// parabola (parabola.cu:168-231) on [nmin, nmaxp)
-template <typename T, int K>
+// parabola (parabola.cu:70-136) on [nmin, nmaxp)
+template <typename T, int K, int SL = 64>
 __device__ __forceinline__ Parab<T, K> parabola(int nmin, int nmaxp, const C<T, K>& flat, const C<T, K>& val,
                                                 const Para<T, K>& P) {
   C<T, K> vm = prev1(val), vp = next1(val);
@@ -358,7 +364,7 @@ __device__ __forceinline__ Parab<T, K> parabola(int nmin, int nmaxp, const C<T, 
   Parab<T, K> o;
 #pragma unroll
   for (int h = 0; h < K; ++h) {
-    const int c = cell_of(K, h);
+    const int c = cell_in<SL>(K, h);
     T a = val[h];
     T t1 = flat[h] * a;
     T t2 = T(1.0) - flat[h];
@@ -754,7 +760,7 @@ __device__ __forceinline__ void ppmlr_wave(Win<T, K>& W, int nmax, T dt, T FrRot
     }
   }
 
-  // ---- remap (remap.cu:182-394)
+  // ---- remap (remap.cu:14-226)
   Para<T, K> P1 = paraset(3, nmax - 3, dxl);
   C<T, K> dxf, dxf2, fr, delC;
   C<T, K> dxm = prev1(dxl);
@@ -786,7 +792,7 @@ __device__ __forceinline__ void ppmlr_wave(Win<T, K>& W, int nmax, T dt, T FrRot
     }
     dxf[h] = d;
   }
-  // face average of a parabola swept through the face (remap.cu:242-258, get_flux :169-178)
+  // face average of a parabola swept through the face (remap.cu:59-90, get_flux :1-10)
   auto face = [&](const Parab<T, K>& Q, C<T, K>& out) {
     C<T, K> alm = prev1(Q.al), dam = prev1(Q.da), a6m = prev1(Q.a6);
 #pragma unroll
@@ -927,6 +933,44 @@ __device__ __forceinline__ void visc_p(Win<T, K>& W, int lim, T dt, T R, T nu, c
       W.u[h] = ddv;
       if (do_w) W.v[h] = ddw;
       W.w[h] = ddu;
+    }
+  }
+}
+
+// device_viscosity_z (viscosity.cu:474-518): nu rho with the constant vis_nu, the component
+// along the pencil (the reference's w) taking 2 nu rho d2 + 2 dn d1 and the transverse ones
+// nu rho d2 + dn d1, no curvature terms.  The theta sweep passes (u, v, w) = (v_r, v_phi,
+// v_theta) and x = r times the theta left edges.
+template <typename T, int K>
+__device__ __forceinline__ void visc_z(const C<T, K>& rho, int lim, T dt, T nu, const C<T, K>& x, C<T, K>& u,
+                                       C<T, K>& v, C<T, K>& w) {
+  C<T, K> nr;
+#pragma unroll
+  for (int h = 0; h < K; ++h) nr[h] = nu * rho[h];
+  C<T, K> xm = prev1(x), xp = next1(x);
+  C<T, K> nm = prev1(nr), np = next1(nr);
+  C<T, K> um = prev1(u), up = next1(u);
+  C<T, K> vm = prev1(v), vp = next1(v);
+  C<T, K> wm = prev1(w), wp = next1(w);
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    const int c = cell_of(K, h);
+    T n = nr[h], rh = rho[h];
+    T dn, d1, d2;
+    derivs<T>(xm[h], x[h], xp[h], nm[h], n, np[h], dn, d2);
+    derivs<T>(xm[h], x[h], xp[h], wm[h], w[h], wp[h], d1, d2);
+    T ddw = T(2.0) * n * d2 + T(2.0) * dn * d1;
+    derivs<T>(xm[h], x[h], xp[h], um[h], u[h], up[h], d1, d2);
+    T ddu = n * d2 + dn * d1;
+    derivs<T>(xm[h], x[h], xp[h], vm[h], v[h], vp[h], d1, d2);
+    T ddv = n * d2 + dn * d1;
+    ddw = w[h] + dt * ddw / rh;
+    ddu = u[h] + dt * ddu / rh;
+    ddv = v[h] + dt * ddv / rh;
+    if (c >= kNpad && c < lim - kNpad) {
+      w[h] = ddw;
+      u[h] = ddu;
+      v[h] = ddv;
     }
   }
 }

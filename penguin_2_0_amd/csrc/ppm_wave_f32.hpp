@@ -119,18 +119,24 @@ __device__ __forceinline__ float planet_fth_sph(const PlanetF& p, float r, float
   return p.m * p.x * cosd * ct * (ri * ri * ri) - p.m * cosd * ct / st * p.inv_x2;
 }
 
-// log(1 + x): 2 atanh(x / (2 + x)) as an odd series for |x| < 1/4, else the library log
-// (both evaluated, one selected: no divergent branch)
+// true in every lane when the condition holds in any lane of the wavefront (wave-uniform)
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
+// log(1 + x): 2 atanh(x / (2 + x)) as an odd series for |x| < 1/4, else the library log.
+// Deviations that large are rare (next to a planet), so the library branch is taken only by
+// wavefronts that hold one; elsewhere it costs a compare and a scalar branch.
 __device__ __forceinline__ float log1p_acc(float x) {
   const float z = x / (2.0f + x), z2 = z * z;
-  const float s = 2.0f * z * (1.0f + z2 * (1.0f / 3.0f + z2 * (1.0f / 5.0f + z2 * (1.0f / 7.0f + z2 * (1.0f / 9.0f)))));
-  return fabsf(x) < 0.25f ? s : __logf(1.0f + x);
+  float s = 2.0f * z * (1.0f + z2 * (1.0f / 3.0f + z2 * (1.0f / 5.0f + z2 * (1.0f / 7.0f + z2 * (1.0f / 9.0f)))));
+  if (wave_any(!(fabsf(x) < 0.25f))) s = fabsf(x) < 0.25f ? s : __logf(1.0f + x);
+  return s;
 }
-// exp(y) - 1: Taylor series to y^7 for |y| < 1/4, else the library exp
+// exp(y) - 1: Taylor series to y^7 for |y| < 1/4, else the library exp (branch as above)
 __device__ __forceinline__ float expm1_acc(float y) {
-  const float s =
+  float s =
       y * (1.0f + y * (0.5f + y * (1.0f / 6.0f + y * (1.0f / 24.0f + y * (1.0f / 120.0f + y * (1.0f / 720.0f + y * (1.0f / 5040.0f)))))));
-  return fabsf(y) < 0.25f ? s : __expf(y) - 1.0f;
+  if (wave_any(!(fabsf(y) < 0.25f))) s = fabsf(y) < 0.25f ? s : __expf(y) - 1.0f;
+  return s;
 }
 
 // paraset of a uniform grid (CW84 eq. 1.6 with equal widths): constants
@@ -145,10 +151,10 @@ __device__ __forceinline__ Para<float, K> paraset_uniform() {
   return P;
 }
 
-// parabola of X = X_b + x in each cell's frame (parabola.cu:168-231): val = x (the cell's
+// parabola of X = X_b + x in each cell's frame (parabola.cu:70-136): val = x (the cell's
 // deviation), dbg = X_b(c+1) - X_b(c).  Outputs are relative to the cell's own X_b.  With
 // dbg == 0 this is the plain parabola of ppm_wave.hpp.
-template <int K>
+template <int K, int SL = 64>
 __device__ __forceinline__ Parab<float, K> parabola_off(int nmin, int nmaxp, const C<float, K>& flat,
                                                         const C<float, K>& val, const C<float, K>& dbg,
                                                         const Para<float, K>& P) {
@@ -174,7 +180,7 @@ __device__ __forceinline__ Parab<float, K> parabola_off(int nmin, int nmaxp, con
   Parab<float, K> o;
 #pragma unroll
   for (int h = 0; h < K; ++h) {
-    const int c = cell_of(K, h);
+    const int c = cell_in<SL>(K, h);
     const float a = val[h];
     float t1 = flat[h] * a;
     float t2 = 1.0f - flat[h];
@@ -305,7 +311,7 @@ __device__ __forceinline__ void rot_theta(float st, float ct, float oms, float e
 // Eulerian grid (its values outside the window's paraset ranges never reach a parabola the
 // sweep keeps).  wres: the azimuthal grid's uniform angular rate.  rsph: the spherical
 // radius of an azimuthal 3-D ring.  On return cells [6, nmax-6) hold the update.
-template <int K, int G, int NDIM, class Tab, bool ADIA = false>
+template <int K, int G, int NDIM, class Tab, bool ADIA = false, int SL = 64>
 __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para<float, K>& P0, int nmax,
                                            float dt, float wres, const PlanetF& pl, bool planet, bool flat_on,
                                            float sqrt_gam, float rsph = 0.0f, bool phi_rows = false,
@@ -338,7 +344,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
 #pragma unroll
     for (int h = 0; h < K; ++h) {
       // evaluated in every lane and selected (no divergent branch around it)
-      const int c = cell_of(K, h);
+      const int c = cell_in<SL>(K, h);
       const float dp1 = pp1[h] - pm1[h];
       float dp2 = pp2[h] - pm2[h];
       dp2 = (fabsf(dp2) < 1.0e-15f) ? 1.0e-15f : dp2;
@@ -350,7 +356,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
     C<float, K> sm = prev1(steep), sp = next1(steep);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
-      const int c = cell_of(K, h);
+      const int c = cell_in<SL>(K, h);
       const float fl = fminf(0.5f, fmaxf(sm[h], fmaxf(steep[h], sp[h])));
       flat[h] = (c >= 2 && c < nmax - 2) ? fl : 0.0f;
     }
@@ -453,8 +459,8 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
       else pp[h] = W.r[h] * W.cs2[h];
     }
     // along a ring p_b is one constant
-    if constexpr (G == kGeoPhi) edges(parabola(0, nmax, flat, pp, P0), re, le);
-    else edges(parabola_off(0, nmax, flat, pp, W.dpb, P0), re, le);
+    if constexpr (G == kGeoPhi) edges(parabola<float, K, SL>(0, nmax, flat, pp, P0), re, le);
+    else edges(parabola_off<K, SL>(0, nmax, flat, pp, W.dpb, P0), re, le);
     C<float, K> q;
 #pragma unroll
     for (int h = 0; h < K; ++h) q[h] = re[h] / W.pb[h];
@@ -467,7 +473,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
   }
   {
     C<float, K> re, le;
-    edges(parabola_off(0, nmax, flat, W.r, W.drb, P0), re, le);
+    edges(parabola_off<K, SL>(0, nmax, flat, W.r, W.drb, P0), re, le);
     C<float, K> q;
 #pragma unroll
     for (int h = 0; h < K; ++h) q[h] = re[h] / W.rb[h];
@@ -480,7 +486,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
   }
   {
     C<float, K> re, le;
-    edges(parabola(0, nmax, flat, W.u, P0), re, le);
+    edges(parabola<float, K, SL>(0, nmax, flat, W.u, P0), re, le);
     C<float, K> rem = prev1(re);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
@@ -489,7 +495,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
     }
   }
 
-  // ---- riemann, isothermal two-rarefaction closed form (riemann.cu:119-126):
+  // ---- riemann, isothermal two-rarefaction closed form (riemann.cu:46-53):
   //   t1 = (sqrt(pl/rl) + sqrt(pr/rr))/2, t2 = -(ur-ul)/(2 t1), p* = sqrt(pl pr) e^t2,
   //   u* = ul - t1 log(p*/pl); here with pl = p_b(1+epl), ..., p* = p_b(1+em)
   C<float, K> umid, em;
@@ -526,8 +532,9 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
 #pragma unroll
     for (int h = 0; h < K; ++h) {
       const float lpl = log1p_acc(epl[h]), lpr = log1p_acc(epr[h]);
-      const float lrl = log1p_acc(erl[h]), lrr = log1p_acc(err[h]);
-      const float t1 = 0.5f * csm[h] * (__expf(0.5f * (lpl - lrl)) + __expf(0.5f * (lpr - lrr)));
+      // t1 only scales small differences (of velocity and of log pressure), so its plain
+      // FP32 relative accuracy suffices: no logs of the density states
+      const float t1 = 0.5f * csm[h] * (sqrtf((1.0f + epl[h]) / (1.0f + erl[h])) + sqrtf((1.0f + epr[h]) / (1.0f + err[h])));
       const float t2 = -0.5f * (ur[h] - ul[h]) / t1;
       em[h] = expm1_acc(0.5f * (lpl + lpr) + t2);
       umid[h] = ul[h] - t1 * (0.5f * (lpr - lpl) + t2);
@@ -552,7 +559,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
   const float Wd = (G == kGeoPhi) ? dt * wres : 0.0f;
 #pragma unroll
   for (int h = 0; h < K; ++h) {
-    const int c = cell_of(K, h);
+    const int c = cell_in<SL>(K, h);
     const bool mv = (c >= 3 && c < nmax - 2);
     float s;
     if constexpr (RAD) s = dt * umid[h];
@@ -565,7 +572,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
     C<float, K> sp = next1(smv);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
-      const int c = cell_of(K, h);
+      const int c = cell_in<SL>(K, h);
       const bool in = (c >= 3 && c < nmax - 3);
       Dl[h] = in ? sp[h] - smv[h] : 0.0f;   // dxl - dx0, free of the uniform motion
       dltx[h] = in ? dlt[h] : 0.0f;          // the left edge of a moved cell
@@ -595,7 +602,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
       }
 #pragma unroll
       for (int h = 0; h < K; ++h) {
-        const int c = cell_of(K, h);
+        const int c = cell_in<SL>(K, h);
         const bool in = (c >= 3 && c < nmax - 3);
         const float ixc = tixc[h];
         const float ixc3 = ixc * ixc * ixc;
@@ -680,7 +687,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
       }
 #pragma unroll
       for (int h = 0; h < K; ++h) {
-        const int c = cell_of(K, h);
+        const int c = cell_in<SL>(K, h);
         const bool in = (c >= 3 && c < nmax - 3);
         float f = 0.0f, f21 = 0.0f;
         if (planet) {
@@ -727,7 +734,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
       C<float, K> vfdp = next1(vfd);
 #pragma unroll
       for (int h = 0; h < K; ++h) {
-        const int c = cell_of(K, h);
+        const int c = cell_in<SL>(K, h);
         const bool in = (c >= 3 && c < nmax - 3);
         const float rho = W.rb[h] + W.r[h];
         const float dl = W.w[h];
@@ -758,11 +765,11 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
     }
   }
 
-  // ---- remap (remap.cu:182-394)
+  // ---- remap (remap.cu:14-226)
   C<float, K> dxl;
 #pragma unroll
   for (int h = 0; h < K; ++h) dxl[h] = W.dx0[h] + Dl[h];
-  Para<float, K> P1 = paraset(3, nmax - 3, dxl);
+  Para<float, K> P1 = paraset<float, K, SL>(3, nmax - 3, dxl);
   C<float, K> dxf, dxf2, lvol, delC;
   C<float, K> dxm = prev1(dxl);
 #pragma unroll
@@ -783,7 +790,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
     }
     dxf[h] = e;
   }
-  // average over the swept region of the left edge (remap.cu:242-258), relative to the
+  // average over the swept region of the left edge (remap.cu:59-90), relative to the
   // cell's own background (dbg: background difference to the next cell)
   auto face = [&](const Parab<float, K>& Q, const C<float, K>& dbg, C<float, K>& out) {
     C<float, K> alm = prev1(Q.al), dam = prev1(Q.da), a6m = prev1(Q.a6), dbm = prev1(dbg);
@@ -796,7 +803,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
   const C<float, K> zero = splat<float, K>(0.0f);
   C<float, K> mass, invm, fr;
   {
-    Parab<float, K> Q = parabola_off(3, nmax - 3, flat, rL, W.drb, P1);
+    Parab<float, K> Q = parabola_off<K, SL>(3, nmax - 3, flat, rL, W.drb, P1);
     C<float, K> fv, frd;
     face(Q, W.drb, fv);
 #pragma unroll
@@ -814,7 +821,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
     C<float, K> frp = next1(frd);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
-      const int c = cell_of(K, h);
+      const int c = cell_in<SL>(K, h);
       const float rn = W.r[h] + (frd[h] - frp[h]) / W.dvol0[h];
       mass[h] = (W.rb[h] + W.r[h]) * W.dvol0[h];
       invm[h] = 1.0f / (fmaxf(W.rb[h] + rn, smallr) * W.dvol0[h]);
@@ -823,7 +830,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
   }
   // X <- (X m + F_left - F_right) / m_new
   auto remap_plain = [&](C<float, K>& X) {
-    Parab<float, K> Q = parabola(3, nmax - 3, flat, X, P1);
+    Parab<float, K> Q = parabola<float, K, SL>(3, nmax - 3, flat, X, P1);
     C<float, K> fl;
     face(Q, zero, fl);
 #pragma unroll
@@ -831,7 +838,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
     C<float, K> flp = next1(fl);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
-      const int c = cell_of(K, h);
+      const int c = cell_in<SL>(K, h);
       if (c >= 6 && c < nmax - 6) X[h] = (X[h] * mass[h] + (fl[h] - flp[h])) * invm[h];
     }
   };
@@ -839,7 +846,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
   // fluxes relative to the cell's own background; the right edge's flux comes from the next
   // cell's frame, b(c+1) = b(c) + dbg
   auto remap_bg = [&](C<float, K>& X, const C<float, K>& dbg) {
-    Parab<float, K> Q = parabola_off(3, nmax - 3, flat, X, dbg, P1);
+    Parab<float, K> Q = parabola_off<K, SL>(3, nmax - 3, flat, X, dbg, P1);
     C<float, K> fl;
     face(Q, dbg, fl);
 #pragma unroll
@@ -847,7 +854,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
     C<float, K> flp = next1(fl), frp = next1(fr);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
-      const int c = cell_of(K, h);
+      const int c = cell_in<SL>(K, h);
       if (c >= 6 && c < nmax - 6) X[h] = (X[h] * mass[h] + (fl[h] - (flp[h] + dbg[h] * frp[h]))) * invm[h];
     }
   };
@@ -869,7 +876,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
 // background's (from the table, FP64 rounded once) plus the deviation's (FP32, from the
 // cell widths: the left edges of c-1, c, c+1 sit at 0, dx(c-1), dx(c-1) + dx(c)).
 // l_new = l + dt R ddv / rho leaves L_b in place: dl += dt R ddv / rho.
-template <int K>
+template <int K, int SL = 64>
 __device__ __forceinline__ void visc_r_pert(WinP<K>& W, const TabLane<K>& tx, int lim, float dt) {
   const C<float, K> nu = tx.get(kTabNu), dnb = tx.get(kTabDnb), d1ob = tx.get(kTabD1Ob), d2ob = tx.get(kTabD2Ob),
                     ixc = tx.get(kTabIxc);
@@ -883,7 +890,7 @@ __device__ __forceinline__ void visc_r_pert(WinP<K>& W, const TabLane<K>& tx, in
   C<float, K> nm = prev1(np_), npp = next1(np_), um = prev1(W.u), up = next1(W.u), omm = prev1(om), omp = next1(om);
 #pragma unroll
   for (int h = 0; h < K; ++h) {
-    const int c = cell_of(K, h);
+    const int c = cell_in<SL>(K, h);
     const float x2 = wm[h], x3 = wm[h] + W.dx0[h], R = W.rad[h];
     const float rho = W.rb[h] + W.r[h], n = nu[h] * rho;
     float dn, d1, d2, e1, e2;

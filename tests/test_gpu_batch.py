@@ -169,3 +169,86 @@ def test_batch_restart_from_pictures(tmp_path):
             assert np.array_equal(got.view(np.uint8), want[d].view(np.uint8)), d
             assert abs(st[d].simtime - (0.5 + t11[d] - t6[d])) <= 1e-12
 
+
+
+def moving_members():
+    """Disks whose planets move under the disks' forces (plnt_flag 2) beside fixed ones."""
+    out = []
+    for d, (nr, q, alpha, flag) in enumerate([(64, 1e-3, 3e-3, 2), (96, 3e-4, 0.0, 1), (130, 1e-3, 1e-3, 2),
+                                              (48, 1e-4, 0.0, 2)]):
+        over = dict(visc_flag=1, ss_alpha=alpha, vis_nu=alpha * 0.05 * 0.05) if alpha > 0 else {}
+        w = configs.small(nr, 3 * nr, M_p=q, plnt_flag=flag, **over)
+        w.seed = 300 + d
+        out.append(w)
+    return out
+
+
+def test_batch_with_moving_planets_equals_single_disks():
+    """A planet that moves takes its CFL pass before the step and its disk forces and closing
+    kick after the sweeps, inside the batch's captured step: every disk equals its own
+    single-context FP32 run, planet included."""
+    ws = moving_members()
+    n = 30
+    say("case: batch with moving planets,", len(ws), "disks,", n, "steps")
+    outs, sts = run_batch(ws, n)
+    for w, o, s in zip(ws, outs, sts):
+        ref, rs = run_single(w, abi.PN_FP32, n)
+        say(f"  {w.name} plnt_flag {w.params.plnt_flag}: planet ({s.planet.x!r}, {s.planet.y!r}) vs "
+            f"({rs.planet.x!r}, {rs.planet.y!r}); dt {s.dt!r} vs {rs.dt!r}")
+        assert s.dt == rs.dt and s.simtime == rs.simtime and s.FrRot == rs.FrRot
+        assert s.planet.x == rs.planet.x and s.planet.y == rs.planet.y and s.planet.vx == rs.planet.vx
+        assert np.array_equal(o["rot_j"], ref["rot_j"])
+        for f in ("r", "u", "v"):
+            l1, li = rel(o[f], ref[f])
+            assert li <= 1e-12, (w.name, f, l1, li)
+    # the planets did move
+    assert any(abs(s.planet.x - 1.0) > 0 for w, s in zip(ws, sts) if w.params.plnt_flag == 2)
+
+
+def test_batch_landing_matches_single_disks():
+    """Picture times and end times differ per disk: the batch stops after the first step in
+    which any disk lands, and every disk -- landed or not -- equals its own single-context
+    run with the landing over the same steps; a second call goes on to the next landing."""
+    ws = members()[:3] + moving_members()[:1]
+    for w, (tm, te) in zip(ws, [(0.11, 5.0), (0.07, 5.0), (0.5, 0.09), (0.2, 5.0)]):
+        w.params.tmovie, w.params.endtime = tm, te
+    disks = [abi.init_disk(w.grid, w.params, w.seed, w.noise) for w in ws]
+    say("case: batch landing,", len(ws), "disks")
+    with abi.Batch([w.grid for w in ws], [w.params for w in ws]) as b:
+        for ctx, d in zip(b.disks, disks):
+            ctx.load(d)
+        b.set_variants(0, 1)
+        st = b.run_begin()
+        for k in range(len(ws)):
+            st[k].land = 1
+        singles = []
+        for w, d in zip(ws, disks):
+            c = abi.Context(w.grid, w.params, abi.PN_FP32)
+            c.load(d)
+            c.set_variants(0, 1)
+            s1 = c.run_begin()
+            s1.land = 1
+            singles.append((c, s1))
+        try:
+            for call in range(2):
+                n0 = st[0].ncycle
+                b.run(1000, st)
+                n = st[0].ncycle - n0
+                events = [st[k].event for k in range(len(ws))]
+                say(f"  call {call}: {n} steps, events {events}")
+                assert 0 < n < 1000 and any(events)
+                assert all(st[k].ncycle - n0 == n for k in range(len(ws)))
+                for k, (w, (c, s1)) in enumerate(zip(ws, singles)):
+                    c.run(n, s1)
+                    assert s1.ncycle == st[k].ncycle and s1.event == st[k].event, (w.name, s1.ncycle, s1.event)
+                    assert s1.simtime == st[k].simtime and s1.dt == st[k].dt and s1.timem == st[k].timem
+                    o, ref = b.disks[k].store(), c.store()
+                    assert np.array_equal(o["rot_j"], ref["rot_j"])
+                    for f in ("r", "u", "v"):
+                        l1, li = rel(o[f], ref[f])
+                        assert li <= 1e-12, (w.name, f, l1, li)
+                if any(st[k].event == 2 for k in range(len(ws))):
+                    break
+        finally:
+            for c, _ in singles:
+                c.close()

@@ -287,4 +287,20 @@ def test_restart_from_a_picture_resumes_the_run(tmp_path, prec):
             want.append(body.m)
         say(f"  planet mass at restart {st.planet.m!r}, ramp at 0 / at 0.5: {want}")
         assert st.planet.m == want[1] and want[1] > want[0]
+    # a planet that moves: the resumed run starts from a fresh planet on its initial circular
+    # orbit (main.cu:244-330 calls init_planet again), not from where the picture's run left it
+    w.params.t_growth = 0.0
+    w.params.plnt_flag = 2
+    with abi.Context(w.grid, w.params, prec) as c:
+        c.load(d)
+        s0 = c.run_begin()
+        c.run(5, s0, graph=True)
+        moved = (s0.planet.x, s0.planet.y)
+        st = c.run_restart(2)
+        body, frot = abi.PnBody(), C.c_double()
+        c.lib.pn_init_planet(C.byref(w.params), C.byref(body), C.byref(frot))
+        say(f"  moving planet: after 5 steps at {moved}, resumed at ({st.planet.x!r}, {st.planet.y!r})")
+        assert moved != (body.x, body.y)
+        assert (st.planet.x, st.planet.y, st.planet.vx, st.planet.vy) == (body.x, body.y, body.vx, body.vy)
+        assert st.planet.x == w.params.R_p and st.FrRot == frot.value
 

@@ -2,7 +2,7 @@
 isothermal disk: viscosity, the isentropic (EOS 1) and adiabatic (EOS 2) closures, the
 energy-consistent adiabatic step (EOS 3), a planet that moves under the disk's torque,
 FARGO frame mode 2, a non-rotating frame, no flattening, a growing planet, and the 3-D
-r-phi-theta disk with its theta sweep.
+r-phi-theta disk with its theta sweep (inviscid, and viscous with the theta viscosity).
 
 Bars as in test_gpu_parity.py: integers (rot_j) bit-exact, reals within the written
 relative L1 / Linf tolerances after each single sweep and after 10 full steps.  The
@@ -116,6 +116,12 @@ CASES = [
      ("r", "u", "v", "w"), (1e-12, 1e-9), (1e-10, 1e-7)),
     (named(configs.small(2048, 96, M_p=1e-3), "_grid_wide_rings"), ("r", "u", "v"), (1e-13, 1e-11),
      (1e-11, 1e-8)),
+    # 3-D with viscosity: device_viscosity_r / _p with their theta-velocity terms and
+    # device_viscosity_z (viscosity.cu:474-518) in the theta sweep
+    (named(reparam(disk3d(24, 72, 12), visc_flag=1, ss_alpha=3e-3, vis_nu=3e-3 * 0.05 * 0.05), "_visc"),
+     ("r", "u", "v", "w"), (1e-12, 1e-9), (1e-10, 1e-7)),
+    (named(reparam(disk3d(24, 72, 12), visc_flag=1, ss_alpha=3e-3, vis_nu=3e-3 * 0.05 * 0.05, eos=2, gam=1.4,
+                   init_p_iso=1), "_visc_eos2"), ("r", "p", "u", "v", "w"), (1e-12, 1e-9), (1e-10, 1e-7)),
 ]
 
 

@@ -151,10 +151,6 @@ def test_batch_argument_errors_before_any_device_call():
     with pytest.raises(abi.PnError) as e:
         abi.Batch(g, p, precision=abi.PN_FP64)
     assert e.value.code == abi.PN_ERR_UNSUPPORTED
-    # a planet that moves needs its force reductions between the sweeps
-    with pytest.raises(abi.PnError) as e:
-        abi.Batch(g, [p[0].replace(plnt_flag=2), p[1]])
-    assert e.value.code == abi.PN_ERR_UNSUPPORTED
     with pytest.raises(abi.PnError) as e:
         abi.Batch([], [])
     assert e.value.code == abi.PN_ERR_ARG
@@ -173,6 +169,11 @@ def test_fp32_parameter_errors_before_any_device_call():
         assert e.value.code == abi.PN_ERR_UNSUPPORTED
     with pytest.raises(abi.PnError) as e:
         abi.Batch([PnGrid(32, 48, 1)], [configs.disk2d(32, 48, eos=3, gam=1.4)])
+    assert e.value.code == abi.PN_ERR_UNSUPPORTED
+    # the viscous 3-D disk is FP64-only
+    p3 = configs.c3().params.replace(visc_flag=1, ss_alpha=1e-3, vis_nu=1e-3 * 0.05 * 0.05)
+    with pytest.raises(abi.PnError) as e:
+        abi.Context(PnGrid(24, 72, 12), p3, abi.PN_FP32)
     assert e.value.code == abi.PN_ERR_UNSUPPORTED
 
 

@@ -91,7 +91,7 @@ def test_initial_disk_profiles():
 
 
 # ---------------------------------------------------------------------------------------
-# paraset / parabola (parabola.cu:168-274)
+# paraset / parabola (parabola.cu:70-176)
 
 def test_paraset_uniform_grid_coefficients():
     n = 20

@@ -32,7 +32,7 @@ def theta_disk(nth, half=0.3, nr=2, nphi=12, c=0.1, r0=2.0, th0=HPI, **over):
     """Thin shell r in [r0, 1.001 r0], theta in th0 -+ half, no planet, no frame rotation."""
     P = configs.disk2d(nr, nphi).replace(ndim=3, ngeomx=2, ngeomy=4, ngeomz=5, plnt_flag=0, fargo_flag=0,
                                          visc_flag=0, frrot_flag=0, p_beta=0.0, sc_h=c, xmin=r0,
-                                         xmax=1.001 * r0, zmin=th0 - half, zmax=th0 + half, **over)
+                                         xmax=1.001 * r0, zmin=th0 - half, zmax=th0 + half).replace(**over)
     o = oracle.OracleDisk(PnGrid(nr, nphi, nth), P, 1, 0.0)
     for A in (o.rings, o.lft, o.rgh, o.udr, o.top):
         A["rot_j"] = 0

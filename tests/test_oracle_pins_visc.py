@@ -212,3 +212,62 @@ def test_phi_viscosity_diffuses_the_azimuthal_velocity_at_the_analytic_rate():
     e1, e2, e3 = (_phi_vphi_error(n) for n in (48, 96, 192))
     assert e1 < 2e-2
     assert e1 / e2 > 3.5 and e2 / e3 > 3.5
+
+
+# ---------------------------------------------------------------------------------------
+# theta viscosity (device_viscosity_z, viscosity.cu:474-518), in the theta sweep
+
+def theta_twin(nth, nu, r0=2.0, half=0.3, c=0.1):
+    from test_oracle_pins_theta import theta_disk
+    P, a = theta_disk(nth, half=half, c=c, r0=r0, vis_nu=nu, visc_flag=1)
+    _, b = theta_disk(nth, half=half, c=c, r0=r0, vis_nu=nu, visc_flag=0)
+    return a, b
+
+
+def _theta_rings(o):
+    return (o.rings, o.udr, o.top)
+
+
+def test_theta_viscosity_leaves_a_theta_uniform_state_alone():
+    a, b = theta_twin(40, 1e-2)
+    for o in (a, b):
+        for A in _theta_rings(o):
+            A["r"] = 1.3
+            A["p"] = 1.3 * 0.01
+            A["u"] = 0.02
+            A["v"] = 0.0   # (a uniform v_phi is not a uniform l = r sin(theta) v_phi)
+            A["w"] = 0.0
+    a.sweepz(0.01, PnBody(), 0.0)
+    b.sweepz(0.01, PnBody(), 0.0)
+    for f in ("r", "u", "v", "w"):
+        assert np.array_equal(a.rings[f], b.rings[f])
+
+
+def _theta_shear_error(nth, field, factor, nu=1e-3, k=4.0, amp=1e-6, r0=2.0):
+    """field = amp sin(k (theta - pi/2)) on a shell at rest: the viscous increment of a short
+    step against factor * nu * d2/ds2 of it along the arc s = r theta."""
+    a, b = theta_twin(nth, nu, r0=r0)
+    for o in (a, b):
+        for A in _theta_rings(o):
+            A["r"] = 1.0
+            A["p"] = 0.01
+            A["u"] = 0.0
+            A["v"] = 0.0
+            A["w"] = 0.0
+            A[field] = (amp * np.sin(k * (A["zc"] - 0.5 * math.pi)))[:, None]
+    f0 = a.rings[field].copy()
+    dt = 1e-5
+    a.sweepz(dt, PnBody(), 0.0)
+    b.sweepz(dt, PnBody(), 0.0)
+    rate = (a.rings[field] - b.rings[field]) / dt
+    rad = a.rings["xc"][:, None]
+    want = -factor * nu * (k / rad) ** 2 * f0
+    return np.abs(rate - want).max() / np.abs(want).max()
+
+
+def test_theta_viscosity_diffuses_each_velocity_at_the_analytic_rate():
+    """v_r and v_phi across theta at nu k^2 / r^2, v_theta along it at twice that."""
+    for field, factor in (("u", 1.0), ("v", 1.0), ("w", 2.0)):
+        e1, e2, e3 = (_theta_shear_error(n, field, factor) for n in (32, 64, 128))
+        assert e1 < 3e-2, field
+        assert e1 / e2 > 3.5 and e2 / e3 > 3.5, field
