@@ -261,6 +261,12 @@ int pn_tune(pn_ctx* ctx, int nrep, int* var_x, int* var_y);
 int pn_set_variants(pn_ctx* ctx, int var_x, int var_y);
 /* The variants in use (after tuning or pinning; 0 for FP64 contexts). */
 int pn_get_variants(const pn_ctx* ctx, int* var_x, int* var_y);
+/* The 3-D FP32 theta sweep's own shape: var_z 0..3 as var_x (what pn_set_variants sets it
+ * to), or 4 = (3, 4, 2), 192-cell windows, which run two theta pencils of up to 84 cells with
+ * their halos per wavefront.  pn_tune picks it apart from the radial shape.  0 for FP64 and
+ * 2-D contexts. */
+int pn_set_variant_z(pn_ctx* ctx, int var_z);
+int pn_get_variant_z(const pn_ctx* ctx, int* var_z);
 /* FP32 radial and theta sweeps: on (1, the default) the last window of a pencil, when it
  * holds at most 32 K cells (K cells per lane), runs two lab columns per wavefront, one in
  * each half; off (0) it runs one, like every other window.  The results are the same either

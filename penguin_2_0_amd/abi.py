@@ -111,6 +111,8 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_tune": (i, [vp, i, P(i), P(i)]),
         "pn_set_variants": (i, [vp, i, i]),
         "pn_set_packing": (i, [vp, i]),
+        "pn_set_variant_z": (i, [vp, i]),
+        "pn_get_variant_z": (i, [vp, P(i)]),
         "pn_get_variants": (i, [vp, P(i), P(i)]),
         "pn_init_planet": (None, [P(PnParams), P(PnBody), P(d)]),
         "pn_planet_forces": (i, [vp, P(PnBody), d, d]),
@@ -342,6 +344,15 @@ class Context:
         m, l = C.c_double(), C.c_double()
         self._chk(self.lib.pn_totals(self.h, C.byref(m), C.byref(l)), "pn_totals")
         return m.value, l.value
+
+    def set_variant_z(self, var_z: int):
+        """The 3-D FP32 theta sweep's own shape (pn_set_variant_z)."""
+        self._chk(self.lib.pn_set_variant_z(self.h, var_z), "pn_set_variant_z")
+
+    def variant_z(self) -> int:
+        v = C.c_int()
+        self._chk(self.lib.pn_get_variant_z(self.h, C.byref(v)), "pn_get_variant_z")
+        return v.value
 
     def set_packing(self, on: bool):
         """Two short last radial / theta windows per wavefront (pn_set_packing)."""

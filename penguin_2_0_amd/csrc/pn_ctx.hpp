@@ -20,6 +20,7 @@ struct pn_ctx {
   pn::DevParams dp{};
   pn::DevF32 df{};                    // FP32 state and tables (FP32 contexts)
   int var_x = 0, var_y = 3;         // FP32 sweep variants (indices into kF32VariantsX / Y)
+  int var_z = 0;                    // FP32 theta sweep variant (index into kF32VariantsZ)
   int pack_tails = 1;               // FP32 radial / theta: two short last windows per wavefront
   bool tuned = false;
   pn::StepScalars* ss = nullptr;      // device

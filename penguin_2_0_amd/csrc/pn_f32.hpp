@@ -79,6 +79,10 @@ constexpr int kNumF32Variants = 4;
 // radial sweep: (cells per lane K, waves per workgroup W, columns per wave Q); a workgroup
 // stages a window of W*Q columns through LDS
 extern const int kF32VariantsX[kNumF32Variants][3];
+// theta sweep: kF32VariantsX, plus (3, 4, 2): 192-cell windows, which hold two theta pencils of
+// 64 cells with their halos per wavefront (76 cells each)
+constexpr int kNumF32VariantsZ = 5;
+extern const int kF32VariantsZ[kNumF32VariantsZ][3];
 // azimuthal sweep: (cells per lane K, waves per workgroup W)
 extern const int kF32VariantsY[kNumF32Variants][2];
 

@@ -204,23 +204,27 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
   // theta: the theta-major table, whose slots run along the pencil (reads past the window
   // land on the next ring's slots or the overhang: finite values no kept cell depends on)
   const unsigned tbytes = 4u * (unsigned)kTabEntries * (unsigned)f.tstride;
-  // A window of at most half a wavefront's cells (the short last window of a pencil) runs
-  // 64 / SL = 2 columns per wavefront pass, one in each half of SL lanes; every stage indexes
-  // cells within its own window (cell_in<SL>), so each column's result is the one-column
-  // pass's.  Column group gi = the NSEG columns gi NSEG + seg; wave wv takes groups
-  // wv QP .. wv QP + QP - 1.
+  // A window of at most SL K cells (a short last window, or a whole short pencil) runs
+  // NSEG = 64 / SL columns per wavefront pass, one in each run of SL lanes (lanes past the
+  // last run idle); every stage indexes cells within its own window (cell_in<SL>), so each
+  // column's result is the one-column pass's.  Column group gi = the NSEG columns
+  // gi NSEG + seg; wave wv takes groups wv QP .. wv QP + QP - 1; a column past the
+  // workgroup's NC reads column NC - 1 and stores nothing.
   auto pencils = [&](auto sl_tag) {
-    constexpr int SL = decltype(sl_tag)::value, NSEG = 64 / SL, QP = (Q + NSEG - 1) / NSEG;
-    const int seg = lane / SL, ll = lane & (SL - 1);
+    constexpr int SL = decltype(sl_tag)::value, NSEG = 64 / SL;
+    constexpr int NG = (NC + NSEG - 1) / NSEG, QP = (NG + W - 1) / W;
+    const int seg = lane / SL, ll = lane % SL;
     const TabLane<K> tx{Buf(TH ? f.tabz : f.tab, tbytes),
                         4u * (unsigned)((TH ? (REAL * l - kNpad + g.k0) + g.sk * (row + kNpad) : s0) + K * ll),
                         f.tstride};
   for (int qq = 0; qq < QP; ++qq) {
     const int gi = wv * QP + qq;
-    if (gi * NSEG >= NC || jb + gi * NSEG >= jmax) break;   // wave-uniform
-    const int q = gi * NSEG + seg;
-    // a second column past jmax (the last group) computes on column jmax - 1's copies and is
-    // never stored
+    if (gi >= NG || jb + gi * NSEG >= jmax) break;   // wave-uniform
+    const int qv = gi * NSEG + seg;
+    const bool mine = seg < NSEG && qv < NC;   // this lane's column is one of the workgroup's
+    const int q = mine ? qv : NC - 1;
+    // a column past jmax (the last group) computes on column jmax - 1's copies and is never
+    // stored
     const int j0 = jb + q < jmax ? jb + q : jmax - 1;
     WinP<K> Wp;
     Wp.rad = tx.get(kTabRad);
@@ -299,7 +303,7 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
 #pragma unroll
     for (int h = 0; h < K; ++h) {
       const int c = K * ll + h;
-      if (c >= kNpad && c < lim - kNpad) {   // only this lane's half-wave touches column q
+      if (mine && c >= kNpad && c < lim - kNpad) {   // only this lane's run touches column q
         tile[0][q][c] = Wp.r[h];
         if constexpr (TH) {
           tile[3][q][c] = Wp.u[h];
@@ -315,8 +319,17 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
     }
   }
   };
-  if (f.pack_tails && lim <= WIN / 2) pencils(std::integral_constant<int, 32>{});
-  else pencils(std::integral_constant<int, 64>{});
+  // workgroup-uniform choice: three windows of 21 lanes (K = 4: a 64-cell theta pencil with
+  // its halos, 76 cells), two of 32, or one
+  bool three = false;
+  if constexpr (K == 4) three = f.pack_tails && lim <= 21 * K;
+  if (three) {
+    if constexpr (K == 4) pencils(std::integral_constant<int, 21>{});
+  } else if (f.pack_tails && lim <= 32 * K) {
+    pencils(std::integral_constant<int, 32>{});
+  } else {
+    pencils(std::integral_constant<int, 64>{});
+  }
   __syncthreads();
   float m = 0.0f;
   const double FrRot = ss->FrRot;
@@ -856,11 +869,12 @@ int f32_windows(int len, int K) {
 }
 
 const int kF32VariantsX[kNumF32Variants][3] = {{2, 4, 2}, {2, 8, 1}, {2, 16, 1}, {2, 4, 4}};
+const int kF32VariantsZ[kNumF32VariantsZ][3] = {{2, 4, 2}, {2, 8, 1}, {2, 16, 1}, {2, 4, 4}, {3, 4, 2}};
 const int kF32VariantsY[kNumF32Variants][2] = {{2, 2}, {2, 4}, {4, 2}, {4, 4}};
 
 int f32_cfl_partials_z(const DevGrid& g, int variant) {
-  const int nc = kF32VariantsX[variant][1] * kF32VariantsX[variant][2];
-  return ((g.jmax + nc - 1) / nc) * f32_windows(g.kmax, kF32VariantsX[variant][0]) * g.imax;
+  const int nc = kF32VariantsZ[variant][1] * kF32VariantsZ[variant][2];
+  return ((g.jmax + nc - 1) / nc) * f32_windows(g.kmax, kF32VariantsZ[variant][0]) * g.imax;
 }
 
 int f32_cfl_partials_y(const DevGrid& g, int variant) {
@@ -902,6 +916,7 @@ hipError_t f32_sweep_z(const LaunchCtx& c, const DevF32& f, int variant, int src
     case 1: return sweep_pencil3_k<2, 8, 1, kGeoTheta>(c, f, src, do_cfl);
     case 2: return sweep_pencil3_k<2, 16, 1, kGeoTheta>(c, f, src, do_cfl);
     case 3: return sweep_pencil3_k<2, 4, 4, kGeoTheta>(c, f, src, do_cfl);
+    case 4: return sweep_pencil3_k<3, 4, 2, kGeoTheta>(c, f, src, do_cfl);
   }
   return hipErrorInvalidValue;
 }

@@ -89,7 +89,7 @@ hipError_t f32_x(const pn_ctx* c, const LaunchCtx& L, int variant, int src) {
 
 // CFL partials the step's last FP32 sweep writes
 int f32_fused_partials(const pn_ctx* c) {
-  return (c->P.ndim == 3) ? f32_cfl_partials_z(c->dg, c->var_x) : f32_cfl_partials_y(c->dg, c->var_y);
+  return (c->P.ndim == 3) ? f32_cfl_partials_z(c->dg, c->var_z) : f32_cfl_partials_y(c->dg, c->var_y);
 }
 
 int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision) {
@@ -161,7 +161,7 @@ int launch_sweeps(pn_ctx* c, int& cur, int do_cfl, bool prep, bool roll) {
     PN_LAUNCH(c, "sweep_x", f32_x(c, L, c->var_x, cur));
     if (c->P.ndim == 3) {
       PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, cur ^ 1, 0));
-      PN_LAUNCH(c, "sweep_z", f32_sweep_z(L, c->df, c->var_x, cur, do_cfl));
+      PN_LAUNCH(c, "sweep_z", f32_sweep_z(L, c->df, c->var_z, cur, do_cfl));
       cur ^= 1;
     } else {
       PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, cur ^ 1, do_cfl));
@@ -275,8 +275,8 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
   // workgroup of the FP32 sweep that takes the CFL in its epilogue, in any variant
   c->ncflp = 2048;
   if (precision == PN_FP32)
-    for (int v = 0; v < kNumF32Variants; ++v) {
-      c->ncflp = std::max(c->ncflp, f32_cfl_partials_y(d, v));
+    for (int v = 0; v < kNumF32VariantsZ; ++v) {
+      if (v < kNumF32Variants) c->ncflp = std::max(c->ncflp, f32_cfl_partials_y(d, v));
       if (p->ndim == 3) c->ncflp = std::max(c->ncflp, f32_cfl_partials_z(d, v));
     }
   if (r2 == PN_OK) r2 = dalloc(c, &c->cflp, c->ncflp);
@@ -648,7 +648,7 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
     PN_HIP(c, hipMemcpyAsync(const_cast<double*>(c->df.vbg), svbg.data(), nslot * sizeof(double), hipMemcpyHostToDevice, st));
     c->df.nwin_x = f32_windows(I, kF32VariantsX[c->var_x][0]);
     c->df.nwin_y = f32_windows(J, kF32VariantsY[c->var_y][0]);
-    c->df.nwin_z = f32_windows(K, kF32VariantsX[c->var_x][0]);
+    c->df.nwin_z = f32_windows(K, kF32VariantsZ[c->var_z][0]);
     c->df.uniform_phi = 1;
     for (int j = 1; j < J; ++j)
       if (c->h_dy[j] != c->h_dy[0]) c->df.uniform_phi = 0;
@@ -838,7 +838,7 @@ int pn_sweep_axis(pn_ctx* c, int axis, double dt, const pn_body* planet, double 
     PN_LAUNCH(c, "prep", f32_prep(L, c->df));
     if (axis == 0) PN_LAUNCH(c, "sweep_x", f32_x(c, L, c->var_x, c->cur));
     else if (axis == 1) PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, c->cur, 0));
-    else PN_LAUNCH(c, "sweep_z", f32_sweep_z(L, c->df, c->var_x, c->cur, 0));
+    else PN_LAUNCH(c, "sweep_z", f32_sweep_z(L, c->df, c->var_z, c->cur, 0));
   } else {
     if (axis == 0) PN_LAUNCH(c, "ghosts", launch_fill_ghosts(L, c->cur));
     hipError_t e = (axis == 0) ? fp64_sweep_x(L, c->cur) : (axis == 1) ? fp64_sweep_y(L, c->cur) : fp64_sweep_z(L, c->cur);
@@ -880,7 +880,7 @@ int pn_time_kernels(pn_ctx* c, int nrep, double* ms6) {
           c->cur ^= 1;
           break;
         case 4:
-          e = is_f32(c) ? f32_sweep_z(L, c->df, c->var_x, c->cur, 1) : fp64_sweep_z(L, c->cur);
+          e = is_f32(c) ? f32_sweep_z(L, c->df, c->var_z, c->cur, 1) : fp64_sweep_z(L, c->cur);
           c->cur ^= 1;
           break;
         case 5: e = launch_rotate(L); break;
@@ -945,29 +945,33 @@ int pn_tune(pn_ctx* c, int nrep, int* var_x, int* var_y) {
   PN_LAUNCH(c, "prep", f32_prep(L, c->df));
   // every candidate reads the current buffer and writes the other one, so the state is
   // left as it was
-  double best_x = 1e30, best_y = 1e30;
-  int bx = c->var_x, by = c->var_y;
-  for (int v = 0; v < kNumF32Variants; ++v) {
-    double tx = 0.0, ty = 0.0;
-    // in 3-D the radial variant also shapes the theta sweep: both are timed
-    int rc = event_time(c, c->stream, nrep, &tx, [&]() { return f32_x(c, L, v, c->cur); });
+  double best_x = 1e30, best_y = 1e30, best_z = 1e30;
+  int bx = c->var_x, by = c->var_y, bz = c->var_z;
+  for (int v = 0; v < kNumF32VariantsZ; ++v) {
+    double tx = 0.0, ty = 0.0, tz = 0.0;
+    int rc = PN_OK;
+    const bool xy = v < kNumF32Variants;
+    if (xy) rc = event_time(c, c->stream, nrep, &tx, [&]() { return f32_x(c, L, v, c->cur); });
     if (rc == PN_OK && c->P.ndim == 3) {
-      double tz = 0.0;
-      rc = event_time(c, c->stream, nrep, &tz, [&]() { return f32_sweep_z(L, c->df, v, c->cur, 0); });
-      tx += tz;
+      // the theta sweep's windows follow its own variant
+      DevF32 fz = c->df;
+      fz.nwin_z = f32_windows(c->g.kmax, kF32VariantsZ[v][0]);
+      rc = event_time(c, c->stream, nrep, &tz, [&]() { return f32_sweep_z(L, fz, v, c->cur, 0); });
     }
-    const bool y_ok = f32_y_variant_allowed(v, c->P.visc_flag != 0);
+    const bool y_ok = xy && f32_y_variant_allowed(v, c->P.visc_flag != 0);
     if (rc == PN_OK && y_ok)
       rc = event_time(c, c->stream, nrep, &ty, [&]() { return f32_sweep_y(L, c->df, v, c->cur, 0); });
     if (rc != PN_OK) return rc;
-    if (tx < best_x) { best_x = tx; bx = v; }
+    if (xy && tx < best_x) { best_x = tx; bx = v; }
     if (y_ok && ty < best_y) { best_y = ty; by = v; }
+    if (c->P.ndim == 3 && tz < best_z) { best_z = tz; bz = v; }
   }
   c->var_x = bx;
   c->var_y = by;
+  c->var_z = bz;
   c->df.nwin_x = f32_windows(c->g.imax, kF32VariantsX[bx][0]);
   c->df.nwin_y = f32_windows(c->g.jmax, kF32VariantsY[by][0]);
-  c->df.nwin_z = f32_windows(c->g.kmax, kF32VariantsX[bx][0]);
+  c->df.nwin_z = f32_windows(c->g.kmax, kF32VariantsZ[bz][0]);
   c->tuned = true;
   if (c->gexec) {
     hipGraphExecDestroy(c->gexec);
@@ -982,6 +986,27 @@ int pn_get_variants(const pn_ctx* c, int* var_x, int* var_y) {
   if (!c || !var_x || !var_y) return PN_ERR_ARG;
   *var_x = is_f32(c) ? c->var_x : 0;
   *var_y = is_f32(c) ? c->var_y : 0;
+  return PN_OK;
+}
+
+int pn_set_variant_z(pn_ctx* c, int var_z) {
+  if (!c) return PN_ERR_ARG;
+  if (!is_f32(c) || c->P.ndim != 3) return var_z == 0 ? PN_OK : fail(c, PN_ERR_ARG, "no FP32 theta sweep here");
+  if (var_z < 0 || var_z >= kNumF32VariantsZ)
+    return fail(c, PN_ERR_ARG, "theta variant %d outside [0, %d)", var_z, kNumF32VariantsZ);
+  c->var_z = var_z;
+  c->df.nwin_z = f32_windows(c->g.kmax, kF32VariantsZ[var_z][0]);
+  c->tuned = true;
+  if (c->gexec) {
+    hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  return PN_OK;
+}
+
+int pn_get_variant_z(const pn_ctx* c, int* var_z) {
+  if (!c || !var_z) return PN_ERR_ARG;
+  *var_z = (is_f32(c) && c->P.ndim == 3) ? c->var_z : 0;
   return PN_OK;
 }
 
@@ -1005,9 +1030,10 @@ int pn_set_variants(pn_ctx* c, int var_x, int var_y) {
     return fail(c, PN_ERR_ARG, "azimuthal variant %d: a viscous disk keeps 128-cell azimuthal windows", var_y);
   c->var_x = var_x;
   c->var_y = var_y;
+  c->var_z = var_x;   // the theta sweep takes the radial shape (pn_set_variant_z sets its own)
   c->df.nwin_x = f32_windows(c->g.imax, kF32VariantsX[var_x][0]);
   c->df.nwin_y = f32_windows(c->g.jmax, kF32VariantsY[var_y][0]);
-  c->df.nwin_z = f32_windows(c->g.kmax, kF32VariantsX[var_x][0]);
+  c->df.nwin_z = f32_windows(c->g.kmax, kF32VariantsZ[var_x][0]);
   c->tuned = true;
   if (c->gexec) {
     hipGraphExecDestroy(c->gexec);

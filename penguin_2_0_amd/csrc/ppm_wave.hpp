@@ -103,16 +103,33 @@ __device__ __forceinline__ C<T, K> next2(const C<T, K>& x) {
 
 __device__ __forceinline__ int cell_of(int K, int h) { return K * (int)(threadIdx.x & 63) + h; }
 // the cell index within a window when a wavefront holds 64 / SL windows side by side, SL
-// lanes each (SL = 64: one window per wavefront, cell_of).  Stencils reach across the seam
-// between two windows exactly as they reach past a window's last cell: every stage's index
-// range keeps such values out of the cells a window stores.
+// lanes each (SL = 64: one window per wavefront, cell_of; lanes past the last whole window
+// count from 0 again and store nothing).  Stencils reach across the seam between two windows
+// exactly as they reach past a window's last cell: every stage's index range keeps such
+// values out of the cells a window stores.
 template <int SL>
-__device__ __forceinline__ int cell_in(int K, int h) { return K * (int)(threadIdx.x & (SL - 1)) + h; }
+__device__ __forceinline__ int cell_in(int K, int h) { return K * (int)((threadIdx.x & 63) % SL) + h; }
 
 template <typename T>
 __device__ __forceinline__ T min3(T A, T B, T Cc) {   // simple_func.cu:2-14
   if (A < B) return (Cc < A) ? Cc : A;
   return (Cc < B) ? Cc : B;
+}
+
+// the limited slope of parabola.cu:87-96: sign(t1) min(|t1|, 2 |da_l|, 2 |da_r|), zero at an
+// extremum.  FP64 keeps the reference's compare chain (min3) operation for operation; FP32 takes
+// the same minimum with one v_min3 and the sign by copysign (the values are the same: a
+// minimum and a sign change round nothing).
+template <typename T>
+__device__ __forceinline__ T limited_slope(T t1, T da_l, T da_r) {
+  if constexpr (sizeof(T) == 4) {
+    t1 = copysignf(fminf(fminf(fabsf(t1), 2.0f * fabsf(da_l)), 2.0f * fabsf(da_r)), t1);
+  } else {
+    if (t1 < T(0.0)) t1 = -min3(-t1, T(2.0) * fabs(da_l), T(2.0) * fabs(da_r));
+    else t1 = min3(t1, T(2.0) * fabs(da_l), T(2.0) * fabs(da_r));
+  }
+  if (da_l * da_r < T(0.0)) t1 = T(0.0);
+  return t1;
 }
 template <typename T>
 __device__ __forceinline__ T max3(T A, T B, T Cc) {   // simple_func.cu:16-28
@@ -349,11 +366,7 @@ __device__ __forceinline__ Parab<T, K> parabola(int nmin, int nmaxp, const C<T, 
     T a = val[h];
     T da_r = vp[h] - a;
     T da_l = a - vm[h];
-    T t1 = P.c3[h] * da_r + P.c4[h] * da_l;
-    if (t1 < T(0.0)) t1 = -min3(-t1, T(2.0) * fabs(da_l), T(2.0) * fabs(da_r));
-    else t1 = min3(t1, T(2.0) * fabs(da_l), T(2.0) * fabs(da_r));
-    if (da_l * da_r < T(0.0)) t1 = T(0.0);
-    slope[h] = t1;
+    slope[h] = limited_slope<T>(P.c3[h] * da_r + P.c4[h] * da_l, da_l, da_r);
     dar[h] = da_r;
   }
   C<T, K> slp = next1(slope);

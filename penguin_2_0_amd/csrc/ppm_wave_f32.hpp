@@ -36,6 +36,7 @@ namespace pn {
 // stores (the hardware range check).  Word 3 of the descriptor: raw 32-bit data (gfx9).
 // the wide buffer loads return integer vectors: take them whole and bit-cast the vector
 typedef int i32x2_t __attribute__((ext_vector_type(2)));
+typedef int i32x3_t __attribute__((ext_vector_type(3)));
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
@@ -165,11 +166,7 @@ __device__ __forceinline__ Parab<float, K> parabola_off(int nmin, int nmaxp, con
     const float a = val[h];
     const float da_r = (vp[h] - a) + dbg[h];
     const float da_l = (a - vm[h]) + dbm[h];
-    float t1 = P.c3[h] * da_r + P.c4[h] * da_l;
-    if (t1 < 0.0f) t1 = -min3(-t1, 2.0f * fabsf(da_l), 2.0f * fabsf(da_r));
-    else t1 = min3(t1, 2.0f * fabsf(da_l), 2.0f * fabsf(da_r));
-    if (da_l * da_r < 0.0f) t1 = 0.0f;
-    slope[h] = t1;
+    slope[h] = limited_slope<float>(P.c3[h] * da_r + P.c4[h] * da_l, da_l, da_r);
     dar[h] = da_r;
   }
   C<float, K> slp = next1(slope);
@@ -241,6 +238,11 @@ struct TabLane {
           __builtin_bit_cast(f32x2_t, (i32x2_t)__builtin_amdgcn_raw_buffer_load_b64(b.r, (int)voff, soff, 0));
       o[0] = v.x;
       o[1] = v.y;
+    } else if constexpr (K == 3) {
+      const i32x3_t v = __builtin_amdgcn_raw_buffer_load_b96(b.r, (int)voff, soff, 0);
+      o[0] = __builtin_bit_cast(float, v.x);
+      o[1] = __builtin_bit_cast(float, v.y);
+      o[2] = __builtin_bit_cast(float, v.z);
     } else if constexpr (K == 4) {
       const f32x4_t v =
           __builtin_bit_cast(f32x4_t, (i32x4_t)__builtin_amdgcn_raw_buffer_load_b128(b.r, (int)voff, soff, 0));
