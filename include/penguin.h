@@ -289,6 +289,16 @@ int pn_damp_edges(pn_ctx* ctx, double dt);
 /* Mass and angular-momentum totals in FP64 with compensated summation. */
 int pn_totals(pn_ctx* ctx, double* mass, double* angmom);
 
+/* pn_totals without any allocation: the same kernels, with their per-block partials and the
+ * two sums in the library's file-scope device scratch (8192 + 8 doubles per device, held
+ * until the process ends) instead of the context's buffers.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Calls no hipMalloc / hipFree.  The
+ * reduction runs at most 4096 blocks of two partials, so every grid fits the scratch.  Every
+ * call on a device shares that device's scratch: calls that overlap on one device (two host
+ * threads, two streams) race. */
+int pn_totals_noalloc(pn_ctx* ctx, double* mass, double* angmom);
+
 /* ---- batches of independent disks ---------------------------------------
  * No counterpart in the reference, which advances one disk per process.  Each disk of a
  * batch is an ordinary context (reach it with pn_batch_disk for pn_load_rings,

@@ -121,6 +121,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "pn_run_restart": (i, [vp, i, P(PnRunState)]),
         "pn_run": (i, [vp, i, i, P(PnRunState)]),
         "pn_totals": (i, [vp, P(d), P(d)]),
+        "pn_totals_noalloc": (i, [vp, P(d), P(d)]),
         "pn_ring_means": (i, [vp, i, P(d), sz]),
         "pn_copy_bandwidth": (i, [sz, i, P(d)]),
         "pn_lab_fields": (i, [vp, sz, i, i, i, vp, vp, vp, vp, vp]),
@@ -357,6 +358,12 @@ class Context:
     def set_packing(self, on: bool):
         """Two short last radial / theta windows per wavefront (pn_set_packing)."""
         self._chk(self.lib.pn_set_packing(self.h, 1 if on else 0), "pn_set_packing")
+
+    def totals_noalloc(self) -> tuple[float, float]:
+        """pn_totals on the library's file-scope scratch (pn_totals_noalloc)."""
+        m, l = C.c_double(), C.c_double()
+        self._chk(self.lib.pn_totals_noalloc(self.h, C.byref(m), C.byref(l)), "pn_totals_noalloc")
+        return m.value, l.value
 
     def ring_means(self, field: int | str) -> np.ndarray:
         """Azimuthal mean of a field per interior ring (pn_ring_means); field 0..4 or one of

@@ -1,11 +1,44 @@
 // pn_kernels_common.hip -- the small per-step kernels: FARGO split and roll, CFL max,
 // reductions, device-side planet update.  FP64 throughout (-ffp-contract=off).
+#include <mutex>
+
 #include "pn_f32.hpp"
 #include "pn_launch.hpp"
 #include "ppm_wave.hpp"
 
 namespace pn {
 
+// ---------------------------------------------------------------------------------------
+// Scratch of the entry points that allocate nothing (pn_totals_noalloc): the per-block
+// partials of a reduction and their sums in one file-scope array.  Every device the library
+// runs on holds its own copy until the process ends.  On the host the array's name is only a
+// stand-in: kernels and copies take the address hipGetSymbolAddress returns for the current
+// device, looked up once per device (noalloc_scratch, first called from pn_create).
+constexpr int kNoAllocDoubles = kMaxPartials + 8;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+double* g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+}  // namespace
+
+hipError_t noalloc_scratch(double** ptr, size_t* ndoubles) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::call_once(g_scratch_once[dev], [dev]() {
+    void* q = nullptr;
+    g_scratch_err[dev] = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    g_scratch_ptr[dev] = (g_scratch_err[dev] == hipSuccess) ? static_cast<double*>(q) : nullptr;
+  });
+  if (g_scratch_err[dev] != hipSuccess) return g_scratch_err[dev];
+  *ptr = g_scratch_ptr[dev];
+  *ndoubles = kNoAllocDoubles;
+  return hipSuccess;
+}
 
 namespace {
 

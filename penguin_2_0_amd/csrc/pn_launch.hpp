@@ -57,6 +57,9 @@ hipError_t launch_sum_partials(const LaunchCtx& c, int nb, int stride, double* o
 hipError_t launch_fill_ghosts(const LaunchCtx& c, int src);
 // damping zones at the radial edges with the step's dt (FP64 state)
 hipError_t launch_damp(const LaunchCtx& c, int src);
+// the current device's copy of the file-scope scratch array of the no-allocation entry
+// points (its address looked up once per device) and its size in doubles
+hipError_t noalloc_scratch(double** ptr, size_t* ndoubles);
 // plain device copy of nbytes (a multiple of 16)
 hipError_t launch_copy(hipStream_t s, const void* a, void* b, size_t nbytes);
 // CFL maxima per block into cflp (no dt); *n receives the partial count

@@ -229,6 +229,16 @@ int pn_create(const pn_grid* g, const pn_params* p, int precision, pn_ctx** out)
     *out = c;
     return fail(c, PN_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
   }
+  {
+    // set-up of the no-allocation entry points: this device's scratch address, once
+    double* scr = nullptr;
+    size_t nscr = 0;
+    e = noalloc_scratch(&scr, &nscr);
+    if (e != hipSuccess) {
+      *out = c;
+      return fail(c, PN_ERR_HIP, "scratch address (hipGetSymbolAddress): %s", hipGetErrorString(e));
+    }
+  }
   DevGrid& d = c->dg;
   d.imax = g->imax;
   d.jmax = g->jmax;
@@ -1065,6 +1075,31 @@ int pn_totals(pn_ctx* c, double* mass, double* angmom) {
   else PN_LAUNCH(c, "totals", launch_totals(lctx(c), c->cur, c->out2));
   double h[2];
   PN_HIP(c, hipMemcpyAsync(h, c->out2, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  int rc = sync(c);
+  if (rc != PN_OK) return rc;
+  *mass = h[0];
+  *angmom = h[1];
+  return PN_OK;
+}
+
+int pn_totals_noalloc(pn_ctx* c, double* mass, double* angmom) {
+  if (!c || !mass || !angmom) return PN_ERR_ARG;
+  if (!c->loaded) return fail(c, PN_ERR_STATE, "no state loaded");
+  double* scr = nullptr;
+  size_t nscr = 0;
+  PN_HIP(c, noalloc_scratch(&scr, &nscr));
+  // the totals kernels run at most kMaxPartials / 2 grid-stride blocks of two partials each,
+  // then the two sums: every grid fits the array
+  if (nscr < static_cast<size_t>(kMaxPartials) + 2)
+    return fail(c, PN_ERR_ARG, "scratch of %zu doubles holds fewer than the %d partials and 2 sums", nscr,
+                kMaxPartials);
+  LaunchCtx L = lctx(c);
+  L.partials = scr;
+  double* sums = scr + kMaxPartials;
+  if (is_f32(c)) PN_LAUNCH(c, "totals", f32_totals(L, c->df, c->cur, sums));
+  else PN_LAUNCH(c, "totals", launch_totals(L, c->cur, sums));
+  double h[2];
+  PN_HIP(c, hipMemcpyAsync(h, sums, sizeof h, hipMemcpyDeviceToHost, c->stream));
   int rc = sync(c);
   if (rc != PN_OK) return rc;
   *mass = h[0];
