@@ -94,7 +94,7 @@ def run_ctx(w: configs.Workload, disk: abi.Disk, prec: int, steps: int, warmup: 
         wall = time.perf_counter() - t0
         m1, l1 = ctx.totals()
         out = {"wall": wall, "value": w.cells * steps / wall if steps else 0.0, "state": ctx.store(), "st": st,
-               "variants": ctx.variants(), "dmass": (m1 - m0) / m0, "dangmom": (l1 - l0) / l0}
+               "variants": (*ctx.variants(), ctx.variant_z()), "dmass": (m1 - m0) / m0, "dangmom": (l1 - l0) / l0}
         if times:
             out["km"] = ctx.time_kernels(5)   # after the snapshot: timing advances the state
     return out
@@ -321,7 +321,7 @@ def main_single(a) -> dict:
     sd = single_disk_rates(w, a.steps, a.warmup, graph)
     km, var32 = sd.pop("_km32"), sd.pop("_var32")
     dom = max(("sweep_x", "sweep_y", "sweep_z"), key=lambda k: km[k])
-    var = var32[1] if dom == "sweep_y" else var32[0]
+    var = {"sweep_x": var32[0], "sweep_y": var32[1], "sweep_z": var32[2]}[dom]
     fields = 3 + (1 if w.params.ndim == 3 else 0) + (1 if w.params.eos != 0 else 0)
     prof = counters(f"{w.name}:fp32:{dom}:v{var}")
     roof = roofline_entry(dom, var, km[dom], 2 * fields * 4 * w.cells, prof, km, abi.copy_bandwidth())

@@ -131,9 +131,12 @@ struct XTile {
 };
 
 // waves per SIMD a pencil kernel asks for: five where a fifth workgroup of four waves fits the
-// CU's 160 KiB of LDS, else no request
+// CU's 160 KiB of LDS, else no request; none for K = 3 either (three cells per lane spill
+// under the 96 registers of five waves and the 128 of four; unconstrained they take 165, three
+// waves per SIMD, without spilling)
 template <int K, int W, int Q, int NF>
 constexpr int pencil_waves() {
+  if (K == 3) return 1;
   return (W == 4 && 5 * sizeof(XTile<K, W, Q, NF>) <= 160 * 1024) ? 5 : 1;
 }
 

@@ -36,7 +36,6 @@ namespace pn {
 // stores (the hardware range check).  Word 3 of the descriptor: raw 32-bit data (gfx9).
 // the wide buffer loads return integer vectors: take them whole and bit-cast the vector
 typedef int i32x2_t __attribute__((ext_vector_type(2)));
-typedef int i32x3_t __attribute__((ext_vector_type(3)));
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
@@ -238,11 +237,6 @@ struct TabLane {
           __builtin_bit_cast(f32x2_t, (i32x2_t)__builtin_amdgcn_raw_buffer_load_b64(b.r, (int)voff, soff, 0));
       o[0] = v.x;
       o[1] = v.y;
-    } else if constexpr (K == 3) {
-      const i32x3_t v = __builtin_amdgcn_raw_buffer_load_b96(b.r, (int)voff, soff, 0);
-      o[0] = __builtin_bit_cast(float, v.x);
-      o[1] = __builtin_bit_cast(float, v.y);
-      o[2] = __builtin_bit_cast(float, v.z);
     } else if constexpr (K == 4) {
       const f32x4_t v =
           __builtin_bit_cast(f32x4_t, (i32x4_t)__builtin_amdgcn_raw_buffer_load_b128(b.r, (int)voff, soff, 0));
@@ -251,6 +245,9 @@ struct TabLane {
       o[2] = v.z;
       o[3] = v.w;
     } else {
+      // K = 3: three dword loads, which the compiler merges into one buffer_load_dwordx3
+      // (the 96-bit load builtin itself is not used: this compiler fetches only its first
+      // dword)
 #pragma unroll
       for (int h = 0; h < K; ++h)
         o[h] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)(voff + 4 * h), soff, 0));

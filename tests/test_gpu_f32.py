@@ -274,6 +274,42 @@ def test_f32_3d_every_variant_and_graph():
     assert np.array_equal(e.view(np.uint8), g.view(np.uint8))
 
 
+@pytest.mark.parametrize("nth", [64, 200])
+def test_f32_theta_variants(nth):
+    """Every theta shape (pn_set_variant_z) is a valid 3-D step: the four radial shapes and the
+    192-cell one (K = 3) that runs two theta pencils per wavefront when a window holds at most
+    96 cells with its halos (64 theta cells: 76), and one per wavefront on the full windows of
+    200 theta cells.  The packed and unpacked K = 3 runs give the same bytes."""
+    w = disk3d(16, 48, nth)
+    say("case: fp32 theta variants", w.name)
+    b, _ = run(w, abi.PN_FP64, 10)
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    outs = {}
+    for vz, pack in [(0, True), (1, True), (2, True), (3, True), (4, True), (4, False)]:
+        with abi.Context(w.grid, w.params, abi.PN_FP32) as ctx:
+            ctx.load(d)
+            ctx.set_variants(0, 3)
+            ctx.set_variant_z(vz)
+            ctx.set_packing(pack)
+            assert ctx.variant_z() == vz
+            st = ctx.run_begin()
+            ctx.run(10, st, graph=True)
+            a = ctx.store()
+        outs[(vz, pack)] = a
+        assert np.array_equal(a["rot_j"], b["rot_j"])
+        for f, t1, tinf in (("r", 1e-6, 1e-4), ("v", 1e-6, 1e-5), ("u", 1e-3, 1e-2), ("w", 1e-3, 1e-2)):
+            l1, li = rel(lab(a, f), lab(b, f))
+            say(f"  theta variant {vz} packing {pack}: {f} L1 {l1:.3e} Linf {li:.3e}")
+            assert l1 <= t1 and li <= tinf, (vz, f, l1, li)
+    assert np.array_equal(outs[(4, True)].view(np.uint8), outs[(4, False)].view(np.uint8))
+    same = np.array_equal(outs[(4, True)].view(np.uint8), outs[(0, True)].view(np.uint8))
+    say(f"  K = 3 and K = 2 theta windows give the same bytes: {same}")
+    with abi.Context(w.grid, w.params, abi.PN_FP32) as ctx:
+        with pytest.raises(abi.PnError) as e:
+            ctx.set_variant_z(5)
+        assert e.value.code == abi.PN_ERR_ARG
+
+
 def flags_case(suffix, **over):
     w = configs.small(96, 288, **over)
     w.name += suffix
