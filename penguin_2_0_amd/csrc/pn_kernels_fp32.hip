@@ -303,6 +303,9 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
     const C<float, K> irc = tx.get(kTabIrc);
     C<float, K> ixc;
     if constexpr (G == kGeoRsph) ixc = tx.get(kTabIxc);
+    // angular momenta back to velocities as increments: v_new = v + (l_new - r v) / r with
+    // r v formed as on the way in, so a cell whose l did not change keeps v exactly (the
+    // product (r v) (1/r) would scale it by the table's fixed r (1/r) - 1 every step)
 #pragma unroll
     for (int h = 0; h < K; ++h) {
       const int c = K * ll + h;
@@ -311,11 +314,16 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
         if constexpr (TH) {
           tile[3][q][c] = Wp.u[h];
           tile[1][q][c] = Wp.v[h];
-          tile[2][q][c] = Wp.w[h] * irc[h];
+          const float v0 = tile[2][q][c];
+          tile[2][q][c] = v0 + (Wp.w[h] - rc[h] * v0) * irc[h];
         } else {
           tile[1][q][c] = Wp.u[h];
-          tile[2][q][c] = Wp.v[h] * irc[h];
-          if constexpr (G == kGeoRsph) tile[3][q][c] = Wp.w[h] * ixc[h];
+          const float v0 = tile[2][q][c];
+          tile[2][q][c] = v0 + (Wp.v[h] - rc[h] * v0) * irc[h];
+          if constexpr (G == kGeoRsph) {
+            const float w0 = tile[3][q][c];
+            tile[3][q][c] = w0 + (Wp.w[h] - Wp.rad[h] * w0) * ixc[h];
+          }
           if constexpr (ADIA) tile[3][q][c] = Wp.q[h];
         }
       }

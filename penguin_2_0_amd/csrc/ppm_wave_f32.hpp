@@ -661,7 +661,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
                 w0 * D * D + D * D * D * (1.0f / 3.0f);
         }
         const float dvol = W.dvol0[h] + ddv;
-        rL[h] = in ? (W.r[h] * W.dvol0[h] - W.rb[h] * ddv) / dvol : W.r[h];
+        rL[h] = in ? W.r[h] - rho * ddv / dvol : W.r[h];   // (r dvol0 - rho_b ddv) / dvol
         if constexpr (ADIA) {
           // EOS 3 (the line evolve.cu:146 leaves commented out): e = eps + u^2/2 gains
           // -dt/dm (F_r - F_l) + dt (u f1 + u' f2)/2, eps' = e - u'^2/2.  With u' = u + du and
@@ -707,7 +707,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
         const float un = W.u[h] + (0.5f * dt * f - dt * dpm / (rho * W.dvol0[h]));
         const float ddv = Dl[h] * W.rad[h];
         const float dvol = W.dvol0[h] + ddv;
-        rL[h] = in ? (W.r[h] * W.dvol0[h] - W.rb[h] * ddv) / dvol : W.r[h];
+        rL[h] = in ? W.r[h] - rho * ddv / dvol : W.r[h];   // (r dvol0 - rho_b ddv) / dvol
         if constexpr (ADIA) {   // as in the radial sweep, with unit area factors
           const float du = un - W.u[h];
           const float de = -dt / (rho * W.dvol0[h]) * ((Flp[h] - Fl[h]) - (W.u[h] + 0.5f * du) * dpm) +
@@ -758,7 +758,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
         // dvol = r (cos(x0 + d) - cos(x1 + d + D)) with x1 the right edge
         const float ddv = W.rad[h] * (dcos(szl[h], czl[h], dltx[h]) - dcos(szr[h], czr[h], dltx[h] + Dl[h]));
         const float dvol = W.dvol0[h] + ddv;
-        rL[h] = in ? (W.r[h] * W.dvol0[h] - W.rb[h] * ddv) / dvol : W.r[h];
+        rL[h] = in ? W.r[h] - rho * ddv / dvol : W.r[h];   // (r dvol0 - rho_b ddv) / dvol
         if (in) W.u[h] = W.u[h] + dt * acc;
       }
     }
@@ -800,7 +800,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
     }
   };
   const C<float, K> zero = splat<float, K>(0.0f);
-  C<float, K> mass, invm, fr;
+  C<float, K> invm, fr;
   {
     Parab<float, K> Q = parabola_off<K, SL>(3, nmax - 3, flat, rL, W.drb, P1);
     C<float, K> fv, frd;
@@ -822,40 +822,41 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
     for (int h = 0; h < K; ++h) {
       const int c = cell_in<SL>(K, h);
       const float rn = W.r[h] + (frd[h] - frp[h]) / W.dvol0[h];
-      mass[h] = (W.rb[h] + W.r[h]) * W.dvol0[h];
       invm[h] = 1.0f / (fmaxf(W.rb[h] + rn, smallr) * W.dvol0[h]);
       W.r[h] = (c >= 6 && c < nmax - 6) ? rn : rL[h];
     }
   }
-  // X <- (X m + F_left - F_right) / m_new
-  auto remap_plain = [&](C<float, K>& X) {
-    Parab<float, K> Q = parabola<float, K, SL>(3, nmax - 3, flat, X, P1);
-    C<float, K> fl;
-    face(Q, zero, fl);
+  // X m_new = X m + F_left - F_right (remap.cu:93-131), taken in increment form: with
+  // m_new - m = fr - fr(c+1) (the density update above),
+  //   X_new = X + ((face_l - X) fr - (face_r - X) fr(c+1)) / m_new,
+  // face_r the next cell's left face in this cell's frame.  Every rounding is then relative to
+  // the increment, and a cell whose fluxes vanish keeps X exactly; the product form
+  // (X m + ...) / m_new rounds X m / m_new away from X by up to an ulp of X, the same ulp
+  // step after step in a steady flow, a drift of 1e-4 of X over 1000 steps.
+  C<float, K> frp = next1(fr);
+  auto remap_inc = [&](C<float, K>& X, const Parab<float, K>& Q, const C<float, K>& dbg, bool has_bg) {
+    C<float, K> rl;
+    face(Q, dbg, rl);
 #pragma unroll
-    for (int h = 0; h < K; ++h) fl[h] = fl[h] * fr[h];
-    C<float, K> flp = next1(fl);
+    for (int h = 0; h < K; ++h) rl[h] = rl[h] - X[h];
+    C<float, K> rlp = next1(rl), xp = next1(X);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
       const int c = cell_in<SL>(K, h);
-      if (c >= 6 && c < nmax - 6) X[h] = (X[h] * mass[h] + (fl[h] - flp[h])) * invm[h];
+      // the right face relative to this cell's X: (face(c+1) - X(c+1)) + (X(c+1) + dbg - X)
+      const float dx_r = has_bg ? (xp[h] - X[h]) + dbg[h] : xp[h] - X[h];
+      const float rr = rlp[h] + dx_r;
+      if (c >= 6 && c < nmax - 6) X[h] = X[h] + (rl[h] * fr[h] - rr * frp[h]) * invm[h];
     }
   };
+  auto remap_plain = [&](C<float, K>& X) {
+    remap_inc(X, parabola<float, K, SL>(3, nmax - 3, flat, X, P1), zero, false);
+  };
   // deviations from a background that varies along the sweep (l - L_b; EOS 3's q - q_b):
-  // fluxes relative to the cell's own background; the right edge's flux comes from the next
+  // fluxes relative to the cell's own background; the right edge's face comes from the next
   // cell's frame, b(c+1) = b(c) + dbg
   auto remap_bg = [&](C<float, K>& X, const C<float, K>& dbg) {
-    Parab<float, K> Q = parabola_off<K, SL>(3, nmax - 3, flat, X, dbg, P1);
-    C<float, K> fl;
-    face(Q, dbg, fl);
-#pragma unroll
-    for (int h = 0; h < K; ++h) fl[h] = fl[h] * fr[h];
-    C<float, K> flp = next1(fl), frp = next1(fr);
-#pragma unroll
-    for (int h = 0; h < K; ++h) {
-      const int c = cell_in<SL>(K, h);
-      if (c >= 6 && c < nmax - 6) X[h] = (X[h] * mass[h] + (fl[h] - (flp[h] + dbg[h] * frp[h]))) * invm[h];
-    }
+    remap_inc(X, parabola_off<K, SL>(3, nmax - 3, flat, X, dbg, P1), dbg, true);
   };
   remap_plain(W.u);
   if constexpr (RAD) remap_bg(W.v, W.dLb);

@@ -107,6 +107,24 @@ def test_c2_fp32_against_fp64_for_1000_steps():
     assert (a["rot_j"] != b["rot_j"]).sum() <= 2
 
 
+@pytest.mark.parametrize("idx", [45, 39, 4])
+def test_strongly_forced_c4_disks_within_1e5_after_1000_steps(idx):
+    """Three disks of the 48-disk sweep whose planets force the flow hardest relative to their
+    sensitivity: the FP32 path within the metric's 1e-5 of FP64 after 1000 steps.  Before the
+    remap and the angular-momentum conversion took increments (a cell whose fluxes vanish
+    keeps its value exactly), a fixed rounding bias per cell and per ring drifted these disks
+    to 4.2e-5, 2.1e-5 and 2.2e-5; now 1.2e-6, 2.3e-6 and 1.1e-7."""
+    w = configs.c4_members()[idx]
+    n = 1000
+    say("case:", w.name, f"q {w.params.M_p:.2e}, FP32 vs FP64,", n, "steps")
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    a, _ = run(w, d, abi.PN_FP32, n)
+    b, _ = run(w, d, abi.PN_FP64, n)
+    l1, li = rel(lab(a, "r"), lab(b, "r"))
+    say(f"  rho: L1 {l1:.3e} Linf {li:.3e}")
+    assert l1 <= 1e-5
+
+
 def test_fused_cfl_folds_more_than_4096_partials():
     """On the 1024 x 3072 disk the 128-cell azimuthal windows make 6912 workgroups, each
     writing one CFL partial, which the step folds in 256 chunks before the post-step (the
