@@ -151,50 +151,59 @@ __device__ __forceinline__ Para<float, K> paraset_uniform() {
   return P;
 }
 
-// parabola of X = X_b + x in each cell's frame (parabola.cu:70-136): val = x (the cell's
-// deviation), dbg = X_b(c+1) - X_b(c).  Outputs are relative to the cell's own X_b.  With
-// dbg == 0 this is the plain parabola of ppm_wave.hpp.
-template <int K, int SL = 64>
-__device__ __forceinline__ Parab<float, K> parabola_off(int nmin, int nmaxp, const C<float, K>& flat,
-                                                        const C<float, K>& val, const C<float, K>& dbg,
-                                                        const Para<float, K>& P) {
-  C<float, K> vm = prev1(val), vp = next1(val), dbm = prev1(dbg);
-  C<float, K> slope, dar;
+// parabola of X = X_b + x in each cell's frame (parabola.cu:70-136), in relative form: val = x
+// (the cell's deviation), dbg = X_b(c+1) - X_b(c) (BG = false: a uniform background, dbg
+// unused).  The edges are carried as offsets from the cell's own value (the right edge
+// c0 da_r + c1 slope(c+1) + c2 slope, the left edge the left neighbour's right edge in this
+// frame), so the flattening, the extremum and overshoot tests, the jump and the curvature
+// never subtract two values of the cell's size; the algebra is the reference's.  Outputs:
+// al = left edge - x, da, a6, and the jumps dl = X(c) - X(c-1), dr = X(c+1) - X(c) in this
+// cell's frame.  A cell outside [nmin+2, nmaxp-2) returns the plain parabola's zero edge
+// (al = -x, da = a6 = 0).
+template <int K>
+struct ParabR {
+  C<float, K> al, da, a6, dl, dr;
+};
+
+template <int K, int SL = 64, bool BG = true>
+__device__ __forceinline__ ParabR<K> parabola_rel(int nmin, int nmaxp, const C<float, K>& flat,
+                                                  const C<float, K>& val, const C<float, K>& dbg,
+                                                  const Para<float, K>& P) {
+  C<float, K> vm = prev1(val), vp = next1(val), dbm;
+  if constexpr (BG) dbm = prev1(dbg);
+  ParabR<K> o;
+  C<float, K> slope;
 #pragma unroll
   for (int h = 0; h < K; ++h) {
     const float a = val[h];
-    const float da_r = (vp[h] - a) + dbg[h];
-    const float da_l = (a - vm[h]) + dbm[h];
-    slope[h] = limited_slope<float>(P.c3[h] * da_r + P.c4[h] * da_l, da_l, da_r);
-    dar[h] = da_r;
+    o.dr[h] = BG ? (vp[h] - a) + dbg[h] : vp[h] - a;
+    o.dl[h] = BG ? (a - vm[h]) + dbm[h] : a - vm[h];
+    slope[h] = limited_slope<float>(P.c3[h] * o.dr[h] + P.c4[h] * o.dl[h], o.dl[h], o.dr[h]);
   }
   C<float, K> slp = next1(slope);
-  C<float, K> ar;
+  C<float, K> ar;   // right edge - x
 #pragma unroll
-  for (int h = 0; h < K; ++h) ar[h] = val[h] + P.c0[h] * dar[h] + P.c1[h] * slp[h] + P.c2[h] * slope[h];
-  C<float, K> alv = prev1(ar);
-  Parab<float, K> o;
+  for (int h = 0; h < K; ++h) ar[h] = P.c0[h] * o.dr[h] + P.c1[h] * slp[h] + P.c2[h] * slope[h];
+  C<float, K> arm = prev1(ar);
 #pragma unroll
   for (int h = 0; h < K; ++h) {
     const int c = cell_in<SL>(K, h);
-    const float a = val[h];
-    float t1 = flat[h] * a;
-    float t2 = 1.0f - flat[h];
-    float arr = t1 + t2 * ar[h];
-    float la = t1 + t2 * (alv[h] - dbm[h]);   // the left neighbour's right edge, in this frame
-    t1 = arr - la;
-    t2 = t1 * t1;
-    t1 = t1 * (6.0f * (a - 0.5f * (arr + la)));
-    if ((arr - a) * (a - la) <= 0.0f) {
-      arr = a;
-      la = a;
+    const float g = 1.0f - flat[h];
+    float r = g * ar[h];                 // flattened right edge - x
+    float l = g * (arm[h] - o.dl[h]);    // flattened left edge - x
+    const float d = r - l;
+    const float t2 = d * d;
+    const float t1 = -3.0f * d * (r + l);   // (ar - la) 6 (x - (ar + la)/2)
+    if (r * l >= 0.0f) {                 // (ar - x)(x - la) <= 0: an extremum
+      r = 0.0f;
+      l = 0.0f;
     }
-    if (t2 < t1) la = 3.0f * a - 2.0f * arr;
-    if (t2 < -t1) arr = 3.0f * a - 2.0f * la;
+    if (t2 < t1) l = -2.0f * r;          // la = 3 x - 2 ar
+    if (t2 < -t1) r = -2.0f * l;         // ar = 3 x - 2 la
     const bool in = (c >= nmin + 2 && c < nmaxp - 2);
-    o.da[h] = in ? arr - la : 0.0f;
-    o.a6[h] = in ? 6.0f * (a - 0.5f * (arr + la)) : 0.0f;
-    o.al[h] = in ? la : 0.0f;
+    o.da[h] = in ? r - l : 0.0f;
+    o.a6[h] = in ? -3.0f * (r + l) : 0.0f;
+    o.al[h] = in ? l : -val[h];
   }
   return o;
 }
@@ -439,12 +448,13 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
     csn[h] = s;
     csfac[h] = 1.0f - fourthd * s;
   }
-  // edge averages of the traced parabolae (states.cu:45-68), relative to the cell's frame
-  auto edges = [&](const Parab<float, K>& q, C<float, K>& re, C<float, K>& le) {
+  // edge averages of the traced parabolae (states.cu:45-68), in the cell's frame: the
+  // cell's value x plus the average's offset from it
+  auto edges = [&](const ParabR<K>& q, const C<float, K>& x, C<float, K>& re, C<float, K>& le) {
 #pragma unroll
     for (int h = 0; h < K; ++h) {
-      re[h] = q.al[h] + q.da[h] - csn[h] * (q.da[h] - csfac[h] * q.a6[h]);
-      le[h] = q.al[h] + csn[h] * (q.da[h] + csfac[h] * q.a6[h]);
+      re[h] = x[h] + (q.al[h] + q.da[h] - csn[h] * (q.da[h] - csfac[h] * q.a6[h]));
+      le[h] = x[h] + (q.al[h] + csn[h] * (q.da[h] + csfac[h] * q.a6[h]));
     }
   };
   // interface c: left state = right edge of cell c-1, right state = left edge of cell c, as
@@ -458,8 +468,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
       else pp[h] = W.r[h] * W.cs2[h];
     }
     // along a ring p_b is one constant
-    if constexpr (G == kGeoPhi) edges(parabola<float, K, SL>(0, nmax, flat, pp, P0), re, le);
-    else edges(parabola_off<K, SL>(0, nmax, flat, pp, W.dpb, P0), re, le);
+    edges(parabola_rel<K, SL, G != kGeoPhi>(0, nmax, flat, pp, W.dpb, P0), pp, re, le);
     C<float, K> q;
 #pragma unroll
     for (int h = 0; h < K; ++h) q[h] = re[h] / W.pb[h];
@@ -472,7 +481,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
   }
   {
     C<float, K> re, le;
-    edges(parabola_off<K, SL>(0, nmax, flat, W.r, W.drb, P0), re, le);
+    edges(parabola_rel<K, SL, G != kGeoPhi>(0, nmax, flat, W.r, W.drb, P0), W.r, re, le);
     C<float, K> q;
 #pragma unroll
     for (int h = 0; h < K; ++h) q[h] = re[h] / W.rb[h];
@@ -485,7 +494,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
   }
   {
     C<float, K> re, le;
-    edges(parabola<float, K, SL>(0, nmax, flat, W.u, P0), re, le);
+    edges(parabola_rel<K, SL, false>(0, nmax, flat, W.u, W.u, P0), W.u, re, le);
     C<float, K> rem = prev1(re);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
@@ -789,24 +798,24 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
     }
     dxf[h] = e;
   }
-  // average over the swept region of the left edge (remap.cu:59-90), relative to the
-  // cell's own background (dbg: background difference to the next cell)
-  auto face = [&](const Parab<float, K>& Q, const C<float, K>& dbg, C<float, K>& out) {
-    C<float, K> alm = prev1(Q.al), dam = prev1(Q.da), a6m = prev1(Q.a6), dbm = prev1(dbg);
+  // average over the swept region of the left edge (remap.cu:59-90) minus the cell's own
+  // value: from the left neighbour's parabola (its offset from X(c-1), then the jump
+  // X(c) - X(c-1) in this frame) or from this cell's own
+  auto face = [&](const ParabR<K>& Q, C<float, K>& out) {
+    C<float, K> alm = prev1(Q.al), dam = prev1(Q.da), a6m = prev1(Q.a6);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
-      if (delC[h] != 0.0f) out[h] = (alm[h] + dam[h] - dxf[h] * (dam[h] - dxf2[h] * a6m[h])) - dbm[h];
+      if (delC[h] != 0.0f) out[h] = (alm[h] + dam[h] - dxf[h] * (dam[h] - dxf2[h] * a6m[h])) - Q.dl[h];
       else out[h] = Q.al[h] - dxf[h] * (Q.da[h] + dxf2[h] * Q.a6[h]);
     }
   };
-  const C<float, K> zero = splat<float, K>(0.0f);
   C<float, K> invm, fr;
   {
-    Parab<float, K> Q = parabola_off<K, SL>(3, nmax - 3, flat, rL, W.drb, P1);
     C<float, K> fv, frd;
-    face(Q, W.drb, fv);
+    face(parabola_rel<K, SL, G != kGeoPhi>(3, nmax - 3, flat, rL, W.drb, P1), fv);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
+      fv[h] = rL[h] + fv[h];   // the face's deviation from rho_b of this cell
       if constexpr (G != kGeoPhi) {
         frd[h] = (W.rb[h] + fv[h]) * lvol[h];
         fr[h] = frd[h];
@@ -834,29 +843,24 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
   // (X m + ...) / m_new rounds X m / m_new away from X by up to an ulp of X, the same ulp
   // step after step in a steady flow, a drift of 1e-4 of X over 1000 steps.
   C<float, K> frp = next1(fr);
-  auto remap_inc = [&](C<float, K>& X, const Parab<float, K>& Q, const C<float, K>& dbg, bool has_bg) {
+  auto remap_inc = [&](C<float, K>& X, const ParabR<K>& Q) {
     C<float, K> rl;
-    face(Q, dbg, rl);
-#pragma unroll
-    for (int h = 0; h < K; ++h) rl[h] = rl[h] - X[h];
-    C<float, K> rlp = next1(rl), xp = next1(X);
+    face(Q, rl);
+    C<float, K> rlp = next1(rl);
 #pragma unroll
     for (int h = 0; h < K; ++h) {
       const int c = cell_in<SL>(K, h);
       // the right face relative to this cell's X: (face(c+1) - X(c+1)) + (X(c+1) + dbg - X)
-      const float dx_r = has_bg ? (xp[h] - X[h]) + dbg[h] : xp[h] - X[h];
-      const float rr = rlp[h] + dx_r;
+      const float rr = rlp[h] + Q.dr[h];
       if (c >= 6 && c < nmax - 6) X[h] = X[h] + (rl[h] * fr[h] - rr * frp[h]) * invm[h];
     }
   };
-  auto remap_plain = [&](C<float, K>& X) {
-    remap_inc(X, parabola<float, K, SL>(3, nmax - 3, flat, X, P1), zero, false);
-  };
+  auto remap_plain = [&](C<float, K>& X) { remap_inc(X, parabola_rel<K, SL, false>(3, nmax - 3, flat, X, X, P1)); };
   // deviations from a background that varies along the sweep (l - L_b; EOS 3's q - q_b):
   // fluxes relative to the cell's own background; the right edge's face comes from the next
   // cell's frame, b(c+1) = b(c) + dbg
   auto remap_bg = [&](C<float, K>& X, const C<float, K>& dbg) {
-    remap_inc(X, parabola_off<K, SL>(3, nmax - 3, flat, X, dbg, P1), dbg, true);
+    remap_inc(X, parabola_rel<K, SL, true>(3, nmax - 3, flat, X, dbg, P1));
   };
   remap_plain(W.u);
   if constexpr (RAD) remap_bg(W.v, W.dLb);

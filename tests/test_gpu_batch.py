@@ -113,6 +113,33 @@ def test_batch_vs_fp64():
         assert l1 <= 1e-6
 
 
+def test_batch_of_large_c4_members_within_bar_after_1000_steps():
+    """The 20 members of the 48-disk sweep with 384 or 512 rings, as one batch for 1000
+    steps with the tuner's shapes: every disk within max(1e-5, its floor) of its own FP64 run
+    in relative L1(Sigma), the floor being the FP64 run from the FP32-rounded initial state
+    (measured: all within 1.2e-6)."""
+    ws = [w for w in configs.c4_members() if w.grid.imax >= 384]
+    n = 1000
+    disks = [abi.init_disk(w.grid, w.params, w.seed, w.noise) for w in ws]
+    say("case: batch of the", len(ws), "C4 members with 384 or 512 rings,", n, "steps")
+    with abi.Batch([w.grid for w in ws], [w.params for w in ws]) as b:
+        for ctx, d in zip(b.disks, disks):
+            ctx.load(d)
+        st = b.run_begin()
+        b.run(n, st, graph=True)
+        outs = [ctx.store() for ctx in b.disks]
+    for w, d, o in zip(ws, disks, outs):
+        ref, _ = run_single(w, abi.PN_FP64, n, disk=d)
+        dr = d.copy()
+        for f in ("r", "p", "u", "v"):
+            dr.rings[f] = dr.rings[f].astype(np.float32).astype(np.float64)
+        flo, _ = run_single(w, abi.PN_FP64, n, disk=dr)
+        l1, li = rel(lab(o, "r"), lab(ref, "r"))
+        floor, _ = rel(lab(flo, "r"), lab(ref, "r"))
+        say(f"  {w.name}: rho L1 {l1:.3e} Linf {li:.3e}, floor {floor:.3e}")
+        assert l1 <= max(1e-5, floor), w.name
+
+
 def test_batch_graph_equals_eager_bitwise():
     ws = members()[:3]
     say("case: batch graph vs eager")
