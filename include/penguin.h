@@ -98,7 +98,8 @@ typedef struct pn_params {
   int32_t nlft, nrgh;
   /* 1: damping zones of width sc_h/2 at xmin and xmax relaxing every field towards the
    * background at rate (distance into the zone / width) r^-1.5 per unit time
-   * (kill_wave.cu:3-43; kill_flag, global.h:22).  FP64 path only. */
+   * (kill_wave.cu:3-43; kill_flag, global.h:22).  Both paths; the FP32 path scales the
+   * deviations from the ring backgrounds. */
   int32_t kill_flag;
   /* 1: barycentric frame (bary_flag, global.h:17): star and planet orbit their barycentre,
    * the star's pull is computed from its offset position and the indirect term goes
@@ -283,7 +284,7 @@ int pn_copy_bandwidth(size_t nbytes, int reps, double* gbs);
 
 /* One application of the damping zones (kill_wave, kill_wave.cu:3-43) with time step dt to
  * the current state, whatever kill_flag says; pn_run applies it after every step's sweeps
- * when kill_flag is 1.  FP64 contexts only. */
+ * when kill_flag is 1.  Both precisions. */
 int pn_damp_edges(pn_ctx* ctx, double dt);
 
 /* Mass and angular-momentum totals in FP64 with compensated summation. */
@@ -308,9 +309,10 @@ int pn_totals_noalloc(pn_ctx* ctx, double* mass, double* angmom);
  * operations of main.cu:343-387 exactly as pn_run performs them. */
 typedef struct pn_batch pn_batch;
 
-/* n disks of grids[d] / params[d].  precision must be PN_FP32; every disk 2-D isothermal
- * without damping zones; its planet fixed, growing or moving under the disk's forces
- * (plnt_flag 0..2).  On failure *out still holds a batch whose pn_batch_last_error explains
+/* n disks of grids[d] / params[d].  precision must be PN_FP32; every disk 2-D isothermal,
+ * viscous or not, with or without damping zones; its planet fixed, growing or moving under
+ * the disk's forces (plnt_flag 0..2).  A disk with a moving planet or damping zones takes
+ * its CFL pass before each step, as pn_run does.  On failure *out still holds a batch whose pn_batch_last_error explains
  * it; pn_batch_destroy it. */
 int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int precision, pn_batch** out);
 int pn_batch_destroy(pn_batch* b);

@@ -63,6 +63,9 @@ void drop_graph(pn_batch* b) {
   }
 }
 
+// a disk whose next dt cannot come from its last sweep: a moving planet or damping zones
+bool cfl_before_step(const pn_ctx* c) { return c->P.plnt_flag == 2 || c->P.kill_flag != 0; }
+
 // the per-disk records and item offsets for the current variants
 int upload_desc(pn_batch* b) {
   const int n = ndisk(b);
@@ -79,11 +82,12 @@ int upload_desc(pn_batch* b) {
     D.p = c->dp;
     D.ss = c->ss;
     D.cflp = c->cflp;
-    // a planet that moves changes FrRot inside the step: its CFL pass runs before the step
-    // (the next step's dt cannot come from the last sweep)
-    const bool moving = c->P.plnt_flag == 2;
-    D.ncfl_post = moving ? 0 : ny;
-    D.ncfl_pre = moving ? f32_cfl_blocks(c->dg) : 0;
+    // a planet that moves changes FrRot inside the step, and damping zones change the state
+    // after the sweeps: such a disk's CFL pass runs before the step (the next step's dt cannot
+    // come from the last sweep)
+    const bool pre = cfl_before_step(c);
+    D.ncfl_post = pre ? 0 : ny;
+    D.ncfl_pre = pre ? f32_cfl_blocks(c->dg) : 0;
     if (D.ncfl_pre > c->ncflp) return bfail(b, PN_ERR_ARG, "disk %d: %d CFL partials exceed %d", d, D.ncfl_pre, c->ncflp);
     D.trig = c->df.trig;
     D.tab = c->df.tab;
@@ -116,20 +120,26 @@ LaunchCtx on_batch(pn_batch* b, pn_ctx* c) {
   return L;
 }
 
-// one step of every disk: the CFL pass of each disk whose planet moves, the batched pre-step,
-// sweeps and post-step, then each moving planet's time, drift, disk forces (modes 8 / 9 from
-// its FP32 state) and closing kick -- the order pn_run gives a single disk
+// one step of every disk: the CFL pass of each disk whose planet moves or which has damping
+// zones, the batched pre-step and sweeps, each damped disk's damping, the batched post-step,
+// then each moving planet's time, drift, disk forces (modes 8 / 9 from its FP32 state) and
+// closing kick -- the order pn_run gives a single disk (damping is ring-uniform, so it commutes
+// with the roll that the post-step does for a moving planet's disk)
 int launch_batch_step(pn_batch* b, int cur) {
   const int n = ndisk(b);
   hipError_t e = hipSuccess;
   for (pn_ctx* c : b->disks) {
-    if (c->P.plnt_flag != 2 || e != hipSuccess) continue;
+    if (!cfl_before_step(c) || e != hipSuccess) continue;
     int ncfl = 0;
     e = f32_cfl(on_batch(b, c), c->df, cur, &ncfl);
   }
   if (e == hipSuccess) e = launch_pre_step_batch(b->stream, b->dd, n);
   if (e == hipSuccess) e = f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x);
   if (e == hipSuccess) e = f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur ^ 1, b->grid_y);
+  for (pn_ctx* c : b->disks) {
+    if (!c->P.kill_flag || e != hipSuccess) continue;
+    e = f32_damp(on_batch(b, c), c->df, cur);
+  }
   if (e == hipSuccess) e = launch_post_step_batch(b->stream, b->dd, n);
   for (pn_ctx* c : b->disks) {
     if (c->P.plnt_flag != 2 || e != hipSuccess) continue;
@@ -228,8 +238,6 @@ int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int pr
       return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches are built for 2-D disks", d);
     if (params[d].eos != 0)
       return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches are built for the isothermal closure (EOS 0)", d);
-    if (params[d].kill_flag)
-      return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches run no damping zones", d);
     pn_ctx* c = nullptr;
     const int rc = pn_create(&grids[d], &params[d], precision, &c);
     if (c) b->disks.push_back(c);

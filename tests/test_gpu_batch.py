@@ -232,6 +232,32 @@ def test_batch_with_moving_planets_equals_single_disks():
     assert any(abs(s.planet.x - 1.0) > 0 for w, s in zip(ws, sts) if w.params.plnt_flag == 2)
 
 
+def test_batch_with_damping_zones_equals_single_disks():
+    """Damping zones (kill_wave.cu:3-43) change the state after the sweeps, so a damped disk
+    takes its CFL pass before the step and its damping between the batched sweeps and the
+    post-step: every disk, damped or not, with a fixed or a moving planet, equals its own
+    single-context FP32 run."""
+    ws = []
+    for d, (nr, q, alpha, flag, kill) in enumerate([(48, 1e-4, 0.0, 1, 1), (64, 3e-4, 3e-3, 1, 1),
+                                                    (96, 1e-3, 1e-3, 2, 1), (130, 1e-4, 0.0, 1, 0)]):
+        over = dict(visc_flag=1, ss_alpha=alpha, vis_nu=alpha * 0.05 * 0.05) if alpha > 0 else {}
+        w = configs.small(nr, 3 * nr, M_p=q, plnt_flag=flag, kill_flag=kill, **over)
+        w.seed = 400 + d
+        ws.append(w)
+    n = 30
+    say("case: batch with damping zones,", len(ws), "disks,", n, "steps")
+    outs, sts = run_batch(ws, n)
+    for w, o, s in zip(ws, outs, sts):
+        ref, rs = run_single(w, abi.PN_FP32, n)
+        say(f"  {w.name} kill_flag {w.params.kill_flag} plnt_flag {w.params.plnt_flag}: dt {s.dt!r} vs {rs.dt!r}")
+        assert s.dt == rs.dt and s.simtime == rs.simtime and s.FrRot == rs.FrRot
+        assert s.planet.x == rs.planet.x and s.planet.y == rs.planet.y
+        assert np.array_equal(o["rot_j"], ref["rot_j"])
+        for f in ("r", "u", "v"):
+            l1, li = rel(o[f], ref[f])
+            assert li <= 1e-12, (w.name, f, l1, li)
+
+
 def test_batch_landing_matches_single_disks():
     """Picture times and end times differ per disk: the batch stops after the first step in
     which any disk lands, and every disk -- landed or not -- equals its own single-context
