@@ -258,14 +258,17 @@ int pn_tune(pn_ctx* ctx, int nrep, int* var_x, int* var_y);
 /* Pin the FP32 sweep variants instead of tuning.  var_x 0..3, radial / theta sweeps as
  * (cells per lane, waves per workgroup, lab columns per wave): (2,4,2), (2,8,1), (2,16,1),
  * (2,4,4).  var_y 0..3, azimuthal sweep as (cells per lane, waves per workgroup): (2,2),
- * (2,4), (4,2), (4,4). */
+ * (2,4), (4,2), (4,4); a viscous disk takes var_y 0 or 1 only (PN_ERR_ARG otherwise), since
+ * the viscosity after the remap reads halo cells at window edges and so depends on the
+ * reference's 128-cell windows. */
 int pn_set_variants(pn_ctx* ctx, int var_x, int var_y);
 /* The variants in use (after tuning or pinning; 0 for FP64 contexts). */
 int pn_get_variants(const pn_ctx* ctx, int* var_x, int* var_y);
 /* The 3-D FP32 theta sweep's own shape: var_z 0..3 as var_x (what pn_set_variants sets it
  * to), or 4 = (3, 4, 2), 192-cell windows, which run two theta pencils of up to 84 cells with
- * their halos per wavefront.  pn_tune picks it apart from the radial shape.  0 for FP64 and
- * 2-D contexts. */
+ * their halos per wavefront (refused with PN_ERR_ARG for a viscous disk, which keeps 128-cell
+ * theta windows).  pn_tune picks it apart from the radial shape.  0 for FP64 and 2-D
+ * contexts. */
 int pn_set_variant_z(pn_ctx* ctx, int var_z);
 int pn_get_variant_z(const pn_ctx* ctx, int* var_z);
 /* FP32 radial and theta sweeps: on (1, the default) the last window of a pencil, when it

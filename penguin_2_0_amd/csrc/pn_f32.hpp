@@ -43,6 +43,9 @@ enum TabEntry {
   kTabZa0, kTabDz0, kTabDvolZ, kTabSzl, kTabCzl, kTabSzr, kTabCzr,
   kTabDrbZ, kTabDpbZ, kTabDLbZ, kTabDpbmZ,
   kTabResBgZ, kTabAbgZ, kTabFfaceZ, kTabLfZ, kTabGfZ, kTabGcZ, kTabGc1Z, kTabGc2Z,
+  // theta viscosity (viscosity.cu:474-518): the quadratic fit's first derivative of
+  // vis_nu rho_b and first and second derivatives of v_bg along x = r theta
+  kTabDnbZ, kTabD1VZ, kTabD2VZ,
   // per step: sin / cos of dt (vrot - v_y) / 4, half the angle the planet's azimuth moves
   // against the slot's frame in half a step (rewritten with vrot)
   kTabRotS, kTabRotC,
@@ -93,6 +96,11 @@ extern const int kF32VariantsY[kNumF32Variants][2];
 // sweep; inviscid sweeps give the same result for any window length.
 inline bool f32_y_variant_allowed(int variant, bool viscous) {
   return !viscous || kF32VariantsY[variant][0] == 2;
+}
+// The theta viscosity (viscosity.cu:474-518, in the theta sweep) also runs after the remap
+// and reads c +- 1, so a viscous 3-D disk keeps 128-cell theta windows (K = 2) as well.
+inline bool f32_z_variant_allowed(int variant, bool viscous) {
+  return !viscous || kF32VariantsZ[variant][0] == 2;
 }
 
 int f32_windows(int len, int K);

@@ -121,7 +121,7 @@ __device__ __forceinline__ C<float, K> lds_k(const float* p) {
 // cells with halos) of NC = W Q consecutive lab columns, stages the window's NF fields in LDS
 // with loads that are contiguous along phi, runs Q columns per wave as pencils out of LDS,
 // writes its results back in place, and stores the window's interior contiguously again.
-// VISC: 0 none, 1 always, 2 as the disk's parameters say (radial, 2-D).
+// VISC: 0 none, 1 always, 2 as the disk's parameters say.
 template <int K, int W, int Q, int NF>
 struct XTile {
   static constexpr int WIN = 64 * K, NC = W * Q, LD = WIN + 2;
@@ -298,8 +298,11 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
     ppmlr_pert<K, G, (G == kGeoRcyl ? 2 : 3), TabLane<K>, ADIA, SL>(Wp, tx, P0, lim, dt, 0.0f, pl, prm.plnt > 0,
                                                                      prm.flat != 0, sqrt_gam, 0.0f, false,
                                                                      (float)prm.gam);
-    if constexpr (G == kGeoRcyl)
-      if (VISC == 1 || (VISC == 2 && prm.visc)) visc_r_pert<K, SL>(Wp, tx, lim, dt);
+    if (VISC == 1 || (VISC == 2 && prm.visc)) {
+      if constexpr (G == kGeoRcyl) visc_r_pert<K, SL>(Wp, tx, lim, dt);
+      else if constexpr (G == kGeoRsph) visc_r_pert<K, SL, true>(Wp, tx, lim, dt);
+      else visc_z_pert<K, SL>(Wp, tx, lim, dt, (float)prm.vis_nu);
+    }
     const C<float, K> irc = tx.get(kTabIrc);
     C<float, K> ixc;
     if constexpr (G == kGeoRsph) ixc = tx.get(kTabIxc);
@@ -394,7 +397,7 @@ __global__ __launch_bounds__(64 * W) PN_WAVES((pencil_waves<K, W, Q, ADIA ? 4 : 
 // 3-D: the spherical radial sweep (item = theta row k, radial window, column group) and the
 // theta sweep (item = ring i, theta window, column group; with do_cfl it is the step's last
 // sweep and writes one CFL partial per workgroup)
-template <int K, int W, int Q, int G>
+template <int K, int W, int Q, int G, int VISC>
 __global__ __launch_bounds__(64 * W) PN_WAVES((pencil_waves<K, W, Q, 4>())) void k_sweep_pencil3_f32(DevGrid g, DevF32 f, DevParams prm,
                                                               const StepScalars* __restrict__ ss, double* cflp,
                                                               int src, int do_cfl) {
@@ -407,10 +410,10 @@ __global__ __launch_bounds__(64 * W) PN_WAVES((pencil_waves<K, W, Q, 4>())) void
   const int l = rest % nwin, row = rest / nwin;
   if (row >= nrow) return;   // workgroup-uniform, before any barrier
   if (do_cfl) {
-    const float m = pencil_item<K, W, Q, G, 0, true>(g, f, prm, ss, src, grp, l, row, T);
+    const float m = pencil_item<K, W, Q, G, VISC, true>(g, f, prm, ss, src, grp, l, row, T);
     if (threadIdx.x == 0) cflp[blockIdx.x] = (double)m;
   } else {
-    pencil_item<K, W, Q, G, 0, false>(g, f, prm, ss, src, grp, l, row, T);
+    pencil_item<K, W, Q, G, VISC, false>(g, f, prm, ss, src, grp, l, row, T);
   }
 }
 
@@ -540,8 +543,7 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
   const Para<float, K> P0 = f.uniform_phi ? paraset_uniform<K>() : paraset(0, lim, W.dx0);
   ppmlr_pert<K, kGeoPhi, NDIM, TabLane<K>, ADIA>(W, TabLane<K>{}, P0, lim, dt, wres, pl, prm.plnt > 0,
                                                  prm.flat != 0, sqrt_gam, rsph, f.uniform_phi != 0, (float)prm.gam);
-  if constexpr (NDIM == 2)
-    if (VISC == 1 || (VISC == 2 && prm.visc)) visc_p_pert<K>(W, lim, dt, T[kTabNu * ts], rc);
+  if (VISC == 1 || (VISC == 2 && prm.visc)) visc_p_pert<K, NDIM == 3>(W, lim, dt, T[kTabNu * ts], rc);
 
   const Buf dr(f.f[src ^ 1][0] + base, rb4), du(f.f[src ^ 1][1] + base, rb4), dv(f.f[src ^ 1][2] + base, rb4);
   const float csnd = sqrt_gam * cs;
@@ -854,7 +856,8 @@ hipError_t sweep_y_k(const LaunchCtx& c, const DevF32& f0, int src, int do_cfl) 
   const long nblk = (waves + WPB - 1) / WPB;
   if (do_cfl && (nblk > c.ncflp || c.g.ndim == 3)) return hipErrorInvalidValue;
   dim3 grid((unsigned)nblk), block(64 * WPB);
-  if (c.g.ndim == 3) k_sweep_y_f32<K, WPB, 0, 3, false><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, 0);
+  if (c.g.ndim == 3 && c.p.visc) k_sweep_y_f32<K, WPB, 1, 3, false><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, 0);
+  else if (c.g.ndim == 3) k_sweep_y_f32<K, WPB, 0, 3, false><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, 0);
   else if (c.p.eos == 3)
     k_sweep_y_f32<K, WPB, 0, 2, true><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
   else k_sweep_y_f32<K, WPB, 2, 2, false><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
@@ -877,7 +880,9 @@ hipError_t sweep_pencil3_k(const LaunchCtx& c, const DevF32& f0, int src, int do
   const long nblk = ngrp * (th ? (long)f.nwin_z * c.g.imax : (long)f.nwin_x * c.g.kmax);
   if (do_cfl && (!th || nblk > c.ncflp)) return hipErrorInvalidValue;
   dim3 grid((unsigned)nblk), block(64 * W);
-  k_sweep_pencil3_f32<K, W, Q, G><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  // a viscous disk runs its own instantiation, so the inviscid one keeps its registers
+  if (c.p.visc) k_sweep_pencil3_f32<K, W, Q, G, 1><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  else k_sweep_pencil3_f32<K, W, Q, G, 0><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
   return hipGetLastError();
 }
 

@@ -101,12 +101,12 @@ int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision)
   const bool sph = (p->ngeomx == 2 && p->ngeomy == 4 && p->ngeomz == 5);
   if (p->ndim == 2 && !cyl) return fail(c, PN_ERR_UNSUPPORTED, "2-D needs cylindrical geometry (1,3,0)");
   if (p->ndim == 3 && !sph) return fail(c, PN_ERR_UNSUPPORTED, "3-D needs spherical geometry (2,4,5)");
-  if (precision == PN_FP32 && p->ndim == 3 && p->visc_flag)
-    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for the inviscid 3-D disk");
   if (p->ndim == 3 && (p->eos == 1 || p->eos == 3)) return fail(c, PN_ERR_UNSUPPORTED, "3-D EOS %d is not built", p->eos);
   if (p->fargo_flag < 0 || p->fargo_flag > 2) return fail(c, PN_ERR_ARG, "fargo_flag %d", p->fargo_flag);
   if (p->plnt_flag < 0 || p->plnt_flag > 2) return fail(c, PN_ERR_ARG, "plnt_flag %d", p->plnt_flag);
-  if (p->plnt_flag == 2 && p->ndim == 3) return fail(c, PN_ERR_UNSUPPORTED, "moving planet needs the 2-D force reductions");
+  // a moving planet in 3-D: the disk-on-planet reductions (modes 8 / 9) are 2-D only in the
+  // reference (output_lv1, device_func.cu:81-108, returns 0 for them in 3-D), so the planet
+  // moves under the star alone, as there
   if (p->nlft < 0 || p->nlft > 4 || p->nrgh < 0 || p->nrgh > 4)
     return fail(c, PN_ERR_ARG, "radial boundary codes (%d, %d) outside 0..4", p->nlft, p->nrgh);
   if (p->kill_flag != 0 && p->kill_flag != 1) return fail(c, PN_ERR_ARG, "kill_flag %d", p->kill_flag);
@@ -629,6 +629,23 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
         const double is = 1.0 / st_;
         q[kTabGc1Z] = (-is * is - 3.0 * ct_ * ct_ * is * is * is * is) / r3;
         q[kTabGc2Z] = (8.0 * ct_ * is * is * is + 12.0 * ct_ * ct_ * ct_ * is * is * is * is * is) / r3;
+        // the theta viscosity's fit through the left edges r z(s - sr), r z(s), r z(s + sr)
+        if (hpz && hnz) {
+          const double x2 = rad * (sz[s] - sz[s - sr]), x3 = rad * (sz[s + sr] - sz[s - sr]);
+          auto fitz = [&](double um, double u0, double up, double& first, double& second) {
+            const double h2 = (u0 - um) / x2, h3 = (up - um) / x3;
+            const double A = (h3 - h2) / (x3 - x2), B = (x3 * h2 - x2 * h3) / (x3 - x2);
+            second = 2.0 * A;
+            first = second * x2 + B;
+          };
+          double d1, d2;
+          const double nv = c->P.vis_nu;
+          fitz(nv * rbg[s - sr], nv * rbg[s], nv * rbg[s + sr], d1, d2);
+          q[kTabDnbZ] = d1;
+          fitz(svbg[s - sr], svbg[s], svbg[s + sr], d1, d2);
+          q[kTabD1VZ] = d1;
+          q[kTabD2VZ] = d2;
+        }
       }
       for (int e = 0; e < kTabEntries; ++e) {
         tab[e * ts + s] = static_cast<float>(q[e]);
@@ -962,7 +979,8 @@ int pn_tune(pn_ctx* c, int nrep, int* var_x, int* var_y) {
     int rc = PN_OK;
     const bool xy = v < kNumF32Variants;
     if (xy) rc = event_time(c, c->stream, nrep, &tx, [&]() { return f32_x(c, L, v, c->cur); });
-    if (rc == PN_OK && c->P.ndim == 3) {
+    const bool z_ok = c->P.ndim == 3 && f32_z_variant_allowed(v, c->P.visc_flag != 0);
+    if (rc == PN_OK && z_ok) {
       // the theta sweep's windows follow its own variant
       DevF32 fz = c->df;
       fz.nwin_z = f32_windows(c->g.kmax, kF32VariantsZ[v][0]);
@@ -974,7 +992,7 @@ int pn_tune(pn_ctx* c, int nrep, int* var_x, int* var_y) {
     if (rc != PN_OK) return rc;
     if (xy && tx < best_x) { best_x = tx; bx = v; }
     if (y_ok && ty < best_y) { best_y = ty; by = v; }
-    if (c->P.ndim == 3 && tz < best_z) { best_z = tz; bz = v; }
+    if (z_ok && tz < best_z) { best_z = tz; bz = v; }
   }
   c->var_x = bx;
   c->var_y = by;
@@ -1004,6 +1022,8 @@ int pn_set_variant_z(pn_ctx* c, int var_z) {
   if (!is_f32(c) || c->P.ndim != 3) return var_z == 0 ? PN_OK : fail(c, PN_ERR_ARG, "no FP32 theta sweep here");
   if (var_z < 0 || var_z >= kNumF32VariantsZ)
     return fail(c, PN_ERR_ARG, "theta variant %d outside [0, %d)", var_z, kNumF32VariantsZ);
+  if (!f32_z_variant_allowed(var_z, c->P.visc_flag != 0))
+    return fail(c, PN_ERR_ARG, "theta variant %d: a viscous disk keeps 128-cell theta windows", var_z);
   c->var_z = var_z;
   c->df.nwin_z = f32_windows(c->g.kmax, kF32VariantsZ[var_z][0]);
   c->tuned = true;

@@ -875,27 +875,33 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
 }
 
 // Explicit viscosity after the radial remap (device_viscosity_r, viscosity.cu:370-421) on
-// the perturbation state: n = nu rho and Omega = l / R^2 are background + deviation, and
-// the quadratic-fit derivatives are linear in the values, so each derivative is the
-// background's (from the table, FP64 rounded once) plus the deviation's (FP32, from the
-// cell widths: the left edges of c-1, c, c+1 sit at 0, dx(c-1), dx(c-1) + dx(c)).
-// l_new = l + dt R ddv / rho leaves L_b in place: dl += dt R ddv / rho.
-template <int K, int SL = 64>
+// the perturbation state: n = nu rho and Omega = l / R^2 (R = r_cyl) are background +
+// deviation, and the quadratic-fit derivatives are linear in the values, so each derivative
+// is the background's (from the table, FP64 rounded once) plus the deviation's (FP32, from
+// the cell widths: the left edges of c-1, c, c+1 sit at 0, dx(c-1), dx(c-1) + dx(c)).
+// l_new = l + dt R ddv / rho leaves L_b in place: dl += dt R ddv / rho.  SPH (3-D spherical
+// r): also w = r v_theta, whose background is zero.
+template <int K, int SL = 64, bool SPH = false>
 __device__ __forceinline__ void visc_r_pert(WinP<K>& W, const TabLane<K>& tx, int lim, float dt) {
   const C<float, K> nu = tx.get(kTabNu), dnb = tx.get(kTabDnb), d1ob = tx.get(kTabD1Ob), d2ob = tx.get(kTabD2Ob),
-                    ixc = tx.get(kTabIxc);
+                    irc = tx.get(kTabIrc), rcyl = tx.get(kTabRc);
   C<float, K> np_, om;
 #pragma unroll
   for (int h = 0; h < K; ++h) {
     np_[h] = nu[h] * W.r[h];
-    om[h] = W.v[h] * (ixc[h] * ixc[h]);
+    om[h] = W.v[h] * (irc[h] * irc[h]);
   }
   C<float, K> wm = prev1(W.dx0);
   C<float, K> nm = prev1(np_), npp = next1(np_), um = prev1(W.u), up = next1(W.u), omm = prev1(om), omp = next1(om);
+  C<float, K> tm, tp;
+  if constexpr (SPH) {
+    tm = prev1(W.w);
+    tp = next1(W.w);
+  }
 #pragma unroll
   for (int h = 0; h < K; ++h) {
     const int c = cell_in<SL>(K, h);
-    const float x2 = wm[h], x3 = wm[h] + W.dx0[h], R = W.rad[h];
+    const float x2 = wm[h], x3 = wm[h] + W.dx0[h], R = rcyl[h];
     const float rho = W.rb[h] + W.r[h], n = nu[h] * rho;
     float dn, d1, d2, e1, e2;
     derivs<float>(0.0f, x2, x3, nm[h], np_[h], npp[h], dn, d2);
@@ -906,18 +912,24 @@ __device__ __forceinline__ void visc_r_pert(WinP<K>& W, const TabLane<K>& tx, in
     e1 += d1ob[h];
     e2 += d2ob[h];
     const float ddv = R * n * e2 + R * dn * e1 + 3.0f * n * e1;
+    float ddw = 0.0f;
+    if constexpr (SPH) {
+      derivs<float>(0.0f, x2, x3, tm[h], W.w[h], tp[h], d1, d2);
+      ddw = n * d2 + dn * d1 + n * d1 / R;
+    }
     if (c >= kNpad && c < lim - kNpad) {
       W.u[h] = W.u[h] + dt * ddu / rho;
       W.v[h] = W.v[h] + dt * R * ddv / rho;
+      if constexpr (SPH) W.w[h] = W.w[h] + dt * ddw / rho;
     }
   }
 }
 
 // device_viscosity_p (viscosity.cu:423-472) on the azimuthal sweep's state: sweep u =
-// v_phi - v_bg and sweep v = v_r.  nu and the backgrounds are uniform along the ring, so
-// only deviations enter the derivatives.  The coordinates are arc lengths along the ring,
-// the left edges times R = r_cyl (ppm.cu:277-278).
-template <int K>
+// v_phi - v_bg, sweep v = v_r and (3-D, W3) sweep w = v_theta.  nu and the backgrounds are
+// uniform along the ring, so only deviations enter the derivatives.  The coordinates are
+// arc lengths along the ring, the left edges times R = r_cyl (ppm.cu:277-278).
+template <int K, bool W3 = false>
 __device__ __forceinline__ void visc_p_pert(WinP<K>& W, int lim, float dt, float nu, float R) {
   C<float, K> np_, wx;
 #pragma unroll
@@ -928,6 +940,11 @@ __device__ __forceinline__ void visc_p_pert(WinP<K>& W, int lim, float dt, float
   C<float, K> wm = prev1(wx);
   C<float, K> nm = prev1(np_), npp = next1(np_);
   C<float, K> vm = prev1(W.u), vp = next1(W.u), rm = prev1(W.v), rp = next1(W.v);
+  C<float, K> tm, tp;
+  if constexpr (W3) {
+    tm = prev1(W.w);
+    tp = next1(W.w);
+  }
 #pragma unroll
   for (int h = 0; h < K; ++h) {
     const int c = cell_of(K, h);
@@ -939,9 +956,57 @@ __device__ __forceinline__ void visc_p_pert(WinP<K>& W, int lim, float dt, float
     const float ddv = 2.0f * n * d2 + 2.0f * dn * d1;
     derivs<float>(0.0f, x2, x3, rm[h], W.v[h], rp[h], d1, d2);
     const float ddu = n * d2 + dn * d1;
+    float ddw = 0.0f;
+    if constexpr (W3) {
+      derivs<float>(0.0f, x2, x3, tm[h], W.w[h], tp[h], d1, d2);
+      ddw = n * d2 + dn * d1;
+    }
     if (c >= kNpad && c < lim - kNpad) {
       W.u[h] = W.u[h] + dt * ddv / rho;
       W.v[h] = W.v[h] + dt * ddu / rho;
+      if constexpr (W3) W.w[h] = W.w[h] + dt * ddw / rho;
+    }
+  }
+}
+
+// device_viscosity_z (viscosity.cu:474-518) on the theta sweep's state: sweep u = v_theta
+// (along the pencil: 2 n d2 + 2 dn d1), sweep v = v_r and v_phi (n d2 + dn d1), with
+// n = vis_nu rho and no curvature terms.  The coordinates are r times the theta left edges.
+// rho_b and v_bg vary along theta: their derivatives come from the table (FP64 rounded once),
+// the deviations' from the cell widths.  v_phi is carried as l - L_b = r_cyl (v_phi - v_bg):
+// the update adds dt r_cyl ddv / rho to it.
+template <int K, int SL = 64>
+__device__ __forceinline__ void visc_z_pert(WinP<K>& W, const TabLane<K>& tx, int lim, float dt, float vis_nu) {
+  const C<float, K> dnb = tx.get(kTabDnbZ), d1vb = tx.get(kTabD1VZ), d2vb = tx.get(kTabD2VZ), irc = tx.get(kTabIrc),
+                    rcyl = tx.get(kTabRc);
+  C<float, K> np_, dv, wx;
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    np_[h] = vis_nu * W.r[h];
+    dv[h] = W.w[h] * irc[h];   // v_phi - v_bg
+    wx[h] = W.dx0[h] * W.rad[h];
+  }
+  C<float, K> wm = prev1(wx);
+  C<float, K> nm = prev1(np_), npp = next1(np_), tm = prev1(W.u), tp = next1(W.u), rm = prev1(W.v), rp = next1(W.v),
+              vm = prev1(dv), vp = next1(dv);
+#pragma unroll
+  for (int h = 0; h < K; ++h) {
+    const int c = cell_in<SL>(K, h);
+    const float x2 = wm[h], x3 = wm[h] + wx[h];
+    const float rho = W.rb[h] + W.r[h], n = vis_nu * rho;
+    float dn, d1, d2;
+    derivs<float>(0.0f, x2, x3, nm[h], np_[h], npp[h], dn, d2);
+    dn += dnb[h];
+    derivs<float>(0.0f, x2, x3, tm[h], W.u[h], tp[h], d1, d2);
+    const float ddt = 2.0f * n * d2 + 2.0f * dn * d1;
+    derivs<float>(0.0f, x2, x3, rm[h], W.v[h], rp[h], d1, d2);
+    const float ddr = n * d2 + dn * d1;
+    derivs<float>(0.0f, x2, x3, vm[h], dv[h], vp[h], d1, d2);
+    const float ddv = n * (d2 + d2vb[h]) + dn * (d1 + d1vb[h]);
+    if (c >= kNpad && c < lim - kNpad) {
+      W.u[h] = W.u[h] + dt * ddt / rho;
+      W.v[h] = W.v[h] + dt * ddr / rho;
+      W.w[h] = W.w[h] + dt * rcyl[h] * ddv / rho;
     }
   }
 }

@@ -259,6 +259,83 @@ def test_f32_3d_vs_fp64(w, n):
         assert l1 <= t1 and li <= tinf, (f, l1, li)
 
 
+def visc3d(nr, nphi, nth, alpha=3e-3):
+    w = disk3d(nr, nphi, nth)
+    w.params = w.params.replace(visc_flag=1, ss_alpha=alpha, vis_nu=alpha * 0.05 * 0.05, M_p=1e-4)
+    w.name += f"_visc{alpha:g}"
+    return w
+
+
+@pytest.mark.parametrize("w,n", [(visc3d(24, 72, 12), 30), (visc3d(64, 192, 32, 1e-2), 50)],
+                         ids=["disk3d_24x72x12_visc", "disk3d_64x192x32_visc"])
+def test_f32_3d_viscous_vs_fp64(w, n):
+    """The viscous 3-D disk on the FP32 path: viscosity in perturbation form in the radial
+    (device_viscosity_r with its v_theta term), azimuthal (device_viscosity_p with v_theta)
+    and theta sweeps (device_viscosity_z, n = vis_nu rho), against the FP64 path with the 3-D
+    bars, in two theta shapes.  The theta viscosity reads halo cells at window edges, so the
+    192-cell theta windows are refused for a viscous disk."""
+    say("case: fp32 3-D viscous vs fp64", w.name, n, "steps")
+    b, sb = run(w, abi.PN_FP64, n)
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    err = {}
+    for vz in (0, 3):
+        with abi.Context(w.grid, w.params, abi.PN_FP32) as ctx:
+            ctx.load(d)
+            ctx.set_variants(0, 1)
+            ctx.set_variant_z(vz)
+            st = ctx.run_begin()
+            ctx.run(n, st, graph=True)
+            a = ctx.store()
+        assert np.array_equal(a["rot_j"], b["rot_j"])
+        assert abs(st.dt - sb.dt) <= 1e-5 * sb.dt
+        for f, t1, tinf in (("r", 1e-6, 1e-4), ("v", 1e-6, 1e-5), ("u", 1e-3, 1e-2), ("w", 1e-3, 1e-2)):
+            l1, li = rel(lab(a, f), lab(b, f))
+            err[f] = max(err.get(f, 0.0), l1)
+            say(f"  theta shape {vz}: {f} L1 {l1:.3e} Linf {li:.3e}")
+            assert l1 <= t1 and li <= tinf, (vz, f, l1, li)
+    # the viscosity moves the state by far more than the FP32 path's distance from FP64
+    p0 = w.params.replace(visc_flag=0, ss_alpha=0.0, vis_nu=0.0)
+    c, _ = run(configs.Workload(w.name + "_inviscid", w.grid, p0, seed=w.seed, steps=n), abi.PN_FP64, n, disk=d)
+    for f in ("r", "u", "w"):
+        l1, _ = rel(lab(c, f), lab(b, f))
+        say(f"  inviscid vs viscous FP64: {f} L1 {l1:.3e} (FP32 distance {err[f]:.3e})")
+        assert l1 > 10 * err[f], (f, l1, err[f])
+    with abi.Context(w.grid, w.params, abi.PN_FP32) as ctx:
+        with pytest.raises(abi.PnError) as e:
+            ctx.set_variant_z(4)
+        assert e.value.code == abi.PN_ERR_ARG
+        ctx.load(d)
+        ctx.tune()
+        assert ctx.variant_z() != 4
+
+
+def test_f32_3d_moving_planet_vs_fp64():
+    """A planet that moves in 3-D (under the star alone: the reference's disk-on-planet
+    reductions are 2-D only), in the inertial frame so that it crosses the grid: the FP32 path
+    with its CFL pass before each step and the per-step planet tables against FP64, fields
+    with the 3-D bars and the planet's position."""
+    w = disk3d(24, 72, 12)
+    w.params = w.params.replace(plnt_flag=2, M_p=1e-3, frrot_flag=0)
+    w.name += "_moving_planet_inertial"
+    n = 30
+    say("case: fp32 3-D moving planet vs fp64", w.name, n, "steps")
+    a, sa = run(w, abi.PN_FP32, n)
+    b, sb = run(w, abi.PN_FP64, n)
+    assert np.array_equal(a["rot_j"], b["rot_j"])
+    assert abs(sa.dt - sb.dt) <= 1e-5 * sb.dt
+    for f, t1, tinf in (("r", 1e-6, 1e-4), ("v", 1e-6, 1e-5), ("u", 1e-3, 1e-2), ("w", 1e-3, 1e-2)):
+        l1, li = rel(lab(a, f), lab(b, f))
+        say(f"  {f}: L1 {l1:.3e} Linf {li:.3e}")
+        assert l1 <= t1 and li <= tinf, (f, l1, li)
+    # the orbit is the star's alone, so the two planets sit on one circle at their own times:
+    # the azimuths differ by the angular speed times the difference of the summed dt
+    lag = (sa.planet.y - sb.planet.y) - sb.planet.vy * (sa.simtime - sb.simtime)
+    say(f"  planet ({sa.planet.x!r}, {sa.planet.y!r}) vs ({sb.planet.x!r}, {sb.planet.y!r}); "
+        f"simtime {sa.simtime!r} vs {sb.simtime!r}; azimuth beyond the time difference {lag:.3e}")
+    assert abs(sb.planet.y - 3.141592653589793) > 1.0   # it moved
+    assert abs(sa.planet.x - sb.planet.x) <= 1e-12 and abs(lag) <= 1e-11
+
+
 def test_f32_3d_every_variant_and_graph():
     """Every radial / theta variant is a valid 3-D step, and graph replay equals eager."""
     w = disk3d(24, 72, 12)

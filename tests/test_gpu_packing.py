@@ -1,8 +1,8 @@
 """Two short last windows per wavefront (pn_set_packing) change nothing but the schedule: the
 FP32 radial and theta sweeps with packing on and off give the same bytes, step after step,
 for every pencil geometry that packs -- the 2-D cylindrical radial sweep (inviscid, viscous,
-EOS 3), the 3-D spherical radial sweep, and a batch of uneven disks -- in every radial
-variant.  Short last windows: 128 rings (24 cells with halos), 48 rings (one 60-cell window),
+EOS 3), the 3-D spherical radial and theta sweeps (inviscid and viscous), and a batch of uneven
+disks -- in every radial variant.  Short last windows: 128 rings (24 cells with halos), 48 rings (one 60-cell window),
 24 x 72 x 12 (one 36-cell window)."""
 from __future__ import annotations
 
@@ -38,8 +38,15 @@ def eos3(nr, nphi):
     return w
 
 
+def visc3d(nr, nphi, nth):
+    w = disk3d(nr, nphi, nth)
+    w.params = w.params.replace(visc_flag=1, ss_alpha=3e-3, vis_nu=3e-3 * 0.05 * 0.05)
+    w.name += "_visc"
+    return w
+
+
 CASES = [configs.small(128, 384, M_p=1e-3), configs.small(48, 144), visc(128, 384, 3e-3), eos3(128, 384),
-         disk3d(24, 72, 12)]
+         disk3d(24, 72, 12), visc3d(24, 72, 12)]
 
 
 def run(w, disk, pack, var_x, n):

@@ -116,6 +116,11 @@ CASES = [
      ("r", "u", "v", "w"), (1e-12, 1e-9), (1e-10, 1e-7)),
     (named(configs.small(2048, 96, M_p=1e-3), "_grid_wide_rings"), ("r", "u", "v"), (1e-13, 1e-11),
      (1e-11, 1e-8)),
+    # a planet that moves in 3-D: the disk-on-planet reductions are 2-D only in the reference
+    # (device_func.cu:81-108 returns 0 for modes 8 / 9 in 3-D), so it moves under the star alone;
+    # the inertial frame makes it move across the grid
+    (named(reparam(disk3d(24, 72, 12), plnt_flag=2, M_p=1e-3, frrot_flag=0), "_moving_planet_inertial"),
+     ("r", "u", "v", "w"), (1e-12, 1e-9), (1e-10, 1e-7)),
     # 3-D with viscosity: device_viscosity_r / _p with their theta-velocity terms and
     # device_viscosity_z (viscosity.cu:474-518) in the theta sweep
     (named(reparam(disk3d(24, 72, 12), visc_flag=1, ss_alpha=3e-3, vis_nu=3e-3 * 0.05 * 0.05), "_visc"),

@@ -170,11 +170,11 @@ def test_fp32_parameter_errors_before_any_device_call():
     with pytest.raises(abi.PnError) as e:
         abi.Batch([PnGrid(32, 48, 1)], [configs.disk2d(32, 48, eos=3, gam=1.4)])
     assert e.value.code == abi.PN_ERR_UNSUPPORTED
-    # the viscous 3-D disk is FP64-only
-    p3 = configs.c3().params.replace(visc_flag=1, ss_alpha=1e-3, vis_nu=1e-3 * 0.05 * 0.05)
-    with pytest.raises(abi.PnError) as e:
-        abi.Context(PnGrid(24, 72, 12), p3, abi.PN_FP32)
-    assert e.value.code == abi.PN_ERR_UNSUPPORTED
+    # the FP32 path keeps fixed radial ghost rings and a star at the origin
+    for over in (dict(nlft=1), dict(bary_flag=1)):
+        with pytest.raises(abi.PnError) as e:
+            abi.Context(PnGrid(32, 48, 1), configs.disk2d(32, 48, **over), abi.PN_FP32)
+        assert e.value.code == abi.PN_ERR_UNSUPPORTED
 
 
 def test_eos3_is_two_dimensional():
