@@ -94,7 +94,8 @@ typedef struct pn_params {
    * 0 reflect (ghost t cells outside the edge cell take the cell t inside it, v_r negated),
    * 1 outflow (ghosts copy the edge cell), 2 periodic in radius, 3 fixed ghost rings (the
    * reference's only live choice, the default), 4 outflow of r, p, u (left) or r, p (right)
-   * with the fixed ghosts' other fields.  Codes other than 3 are FP64-path only. */
+   * with the fixed ghosts' other fields.  Both paths; on the FP32 path a ghost ring that a
+   * code copies into carries its source ring's backgrounds, so the copies are of deviations. */
   int32_t nlft, nrgh;
   /* 1: damping zones of width sc_h/2 at xmin and xmax relaxing every field towards the
    * background at rate (distance into the zone / width) r^-1.5 per unit time
@@ -103,7 +104,8 @@ typedef struct pn_params {
   int32_t kill_flag;
   /* 1: barycentric frame (bary_flag, global.h:17): star and planet orbit their barycentre,
    * the star's pull is computed from its offset position and the indirect term goes
-   * (forces.cu:28-182).  FP64 path only. */
+   * (forces.cu:28-182).  Both paths; on the FP32 path the star's offset pull is a deviation
+   * force beside the planet's, in kernels of their own (the disks of a batch share the frame). */
   int32_t bary_flag;
 } pn_params;
 

@@ -66,6 +66,9 @@ void drop_graph(pn_batch* b) {
 // a disk whose next dt cannot come from its last sweep: a moving planet or damping zones
 bool cfl_before_step(const pn_ctx* c) { return c->P.plnt_flag == 2 || c->P.kill_flag != 0; }
 
+// the batch's disks run in the barycentric frame (all or none): the kernels with its star terms
+bool any_bary(const pn_batch* b) { return b->disks[0]->P.bary_flag != 0; }
+
 // the per-disk records and item offsets for the current variants
 int upload_desc(pn_batch* b) {
   const int n = ndisk(b);
@@ -107,8 +110,8 @@ int upload_desc(pn_batch* b) {
   PB_HIP(b, hipStreamSynchronize(b->stream));
   // persistent grids: what is resident at once, or fewer when there are fewer items
   auto fit = [](int resident, int items) { return std::max(8, std::min(resident, (items + 7) / 8 * 8)); };
-  b->grid_x = fit(f32_batch_grid(0, b->var_x), bx[n]);
-  b->grid_y = fit(f32_batch_grid(1, b->var_y), by[n]);
+  b->grid_x = fit(f32_batch_grid(0, b->var_x, any_bary(b)), bx[n]);
+  b->grid_y = fit(f32_batch_grid(1, b->var_y, any_bary(b)), by[n]);
   if (b->grid_x < 8 || b->grid_y < 8) return bfail(b, PN_ERR_HIP, "occupancy query failed");
   return PN_OK;
 }
@@ -121,7 +124,8 @@ LaunchCtx on_batch(pn_batch* b, pn_ctx* c) {
 }
 
 // one step of every disk: the CFL pass of each disk whose planet moves or which has damping
-// zones, the batched pre-step and sweeps, each damped disk's damping, the batched post-step,
+// zones, the batched pre-step, each disk's radial ghost copies, the batched sweeps, each
+// damped disk's damping, the batched post-step,
 // then each moving planet's time, drift, disk forces (modes 8 / 9 from its FP32 state) and
 // closing kick -- the order pn_run gives a single disk (damping is ring-uniform, so it commutes
 // with the roll that the post-step does for a moving planet's disk)
@@ -134,8 +138,12 @@ int launch_batch_step(pn_batch* b, int cur) {
     e = f32_cfl(on_batch(b, c), c->df, cur, &ncfl);
   }
   if (e == hipSuccess) e = launch_pre_step_batch(b->stream, b->dd, n);
-  if (e == hipSuccess) e = f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x);
-  if (e == hipSuccess) e = f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur ^ 1, b->grid_y);
+  for (pn_ctx* c : b->disks) {   // radial boundary codes other than 3 (launches nothing for 3)
+    if (e != hipSuccess) break;
+    e = f32_fill_ghosts(on_batch(b, c), c->df, cur);
+  }
+  if (e == hipSuccess) e = f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x, any_bary(b));
+  if (e == hipSuccess) e = f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur ^ 1, b->grid_y, any_bary(b));
   for (pn_ctx* c : b->disks) {
     if (!c->P.kill_flag || e != hipSuccess) continue;
     e = f32_damp(on_batch(b, c), c->df, cur);
@@ -233,6 +241,12 @@ int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int pr
   *out = b;   // the caller reads the message of a failure, then pn_batch_destroy
   if (n < 1 || !grids || !params) return bfail(b, PN_ERR_ARG, "batch of %d disks", n);
   if (precision != PN_FP32) return bfail(b, PN_ERR_UNSUPPORTED, "the batched launch is built for the FP32 path");
+  // the barycentric frame's star terms are compiled into kernels of their own; one launch runs
+  // one kind, so the disks of a batch share the frame (a compile-time choice of the reference,
+  // forces.cu:28)
+  for (int d = 1; d < n; ++d)
+    if ((params[d].bary_flag != 0) != (params[0].bary_flag != 0))
+      return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: the disks of a batch share the frame (bary_flag)", d);
   for (int d = 0; d < n; ++d) {
     if (params[d].ndim != 2)
       return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches are built for 2-D disks", d);
@@ -329,12 +343,12 @@ int pn_batch_tune(pn_batch* b, int nrep, int* var_x, int* var_y) {
     double tx = 0.0, ty = 0.0;
     const int n = ndisk(b);
     rc = btime(b, nrep, &tx, [&]() {
-      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x);
+      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x, any_bary(b));
     });
     const bool y_ok = f32_y_variant_allowed(v, any_viscous(b));
     if (rc == PN_OK && y_ok)
       rc = btime(b, nrep, &ty, [&]() {
-        return f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur, b->grid_y);
+        return f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur, b->grid_y, any_bary(b));
       });
     if (rc != PN_OK) return rc;
     if (tx < best_x) { best_x = tx; bx = v; }
@@ -438,11 +452,11 @@ int pn_batch_time_kernels(pn_batch* b, int nrep, double* ms4) {
   rc = btime(b, nrep, &ms4[0], [&]() { return launch_pre_step_batch(b->stream, b->dd, n); });
   if (rc == PN_OK)
     rc = btime(b, nrep, &ms4[1], [&]() {
-      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x);
+      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x, any_bary(b));
     });
   if (rc == PN_OK)
     rc = btime(b, nrep, &ms4[2], [&]() {
-      return f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur, b->grid_y);
+      return f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur, b->grid_y, any_bary(b));
     });
   if (rc == PN_OK) rc = btime(b, nrep, &ms4[3], [&]() { return launch_post_step_batch(b->stream, b->dd, n); });
   return rc;

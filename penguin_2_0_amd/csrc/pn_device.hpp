@@ -77,4 +77,18 @@ __host__ __device__ inline int slot_of(const DevGrid& g, int i, int k) {
   return (i + kNpad) + g.sr * (k + g.k0);
 }
 
+// set_boundx (boundary.cu:86-206): the interior ring whose values radial ghost ring t
+// (1..kNpad cells outside the edge, on the left or right) takes under boundary code
+// 0 (mirrored, v_r negated), 1 and 4 (the edge ring) or 2 (periodic); code 3 keeps the
+// ghost ring fixed (-1)
+__host__ __device__ inline int ghost_source(int code, bool left, int t, int imax) {
+  switch (code) {
+    case 0: return left ? t : imax - 1 - t;
+    case 1:
+    case 4: return left ? 0 : imax - 1;
+    case 2: return left ? imax - t : t - 1;
+    default: return -1;
+  }
+}
+
 }  // namespace pn

@@ -119,13 +119,14 @@ int f32_cfl_partials_z(const DevGrid& g, int variant);
 // batches of disks: one persistent launch per sweep over every disk's work items.
 // items: workgroup-sized work items of one disk (radial: window x column group; azimuthal:
 // the y partial count).  base: ndisk + 1 item offsets in device memory; dd / dfs: per-disk
-// records in device memory.  grid: f32_batch_grid(axis, variant).
+// records in device memory.  grid: f32_batch_grid(axis, variant, bary).  bary: the disks run
+// in the barycentric frame (the kernels with its star terms; the others lack them).
 int f32_items_x(const DevGrid& g, int variant);
-int f32_batch_grid(int axis, int variant);
+int f32_batch_grid(int axis, int variant, bool bary);
 hipError_t f32_sweep_x_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
-                             int variant, int src, int grid);
+                             int variant, int src, int grid, bool bary);
 hipError_t f32_sweep_y_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
-                             int variant, int src, int grid);
+                             int variant, int src, int grid, bool bary);
 // CFL maxima per block into c.cflp; returns the partial count through *n
 hipError_t f32_cfl(const LaunchCtx& c, const DevF32& f, int src, int* n);
 // CFL partials f32_cfl writes for this grid
@@ -136,5 +137,8 @@ hipError_t f32_totals(const LaunchCtx& c, const DevF32& f, int src, double* out2
 hipError_t f32_ring_means(const LaunchCtx& c, const DevF32& f, int src, int field, double* out);
 // damping zones at the radial edges with ss->dt (the FP64 path's launch_damp on the FP32 state)
 hipError_t f32_damp(const LaunchCtx& c, const DevF32& f, int src);
+// radial boundary codes other than 3 (the FP64 path's launch_fill_ghosts on the FP32 state);
+// nothing is launched when both codes are 3
+hipError_t f32_fill_ghosts(const LaunchCtx& c, const DevF32& f, int src);
 
 }  // namespace pn

@@ -70,14 +70,8 @@ __global__ void k_fill_ghosts(DevGrid g, DevParams prm, int src) {
   const bool left = gi < kNpad;
   const int code = left ? prm.nlft : prm.nrgh;
   const int t = left ? kNpad - gi : gi - kNpad + 1;   // cells outside the edge cell
-  int srci;
-  switch (code) {
-    case 0: srci = left ? t : g.imax - 1 - t; break;
-    case 1:
-    case 4: srci = left ? 0 : g.imax - 1; break;
-    case 2: srci = left ? g.imax - t : t - 1; break;
-    default: return;
-  }
+  const int srci = ghost_source(code, left, t, g.imax);
+  if (srci < 0) return;
   const int sg = left ? slot_of(g, -t, k) : slot_of(g, g.imax - 1 + t, k);
   const int ss = slot_of(g, srci, k);
   int jg = jl - g.rot_j[sg], js = jl - g.rot_j[ss];

@@ -54,13 +54,30 @@ __device__ __forceinline__ int win_len_k(int len, int nwin, int l) {
 }
 
 // the planet as the FP32 forces see it; the spherical forms use the reduced mass m/(1+m)
-__device__ __forceinline__ PlanetF planet_f(const StepScalars* ss, bool sph = false) {
+// sph: the spherical forms take the reduced mass m/(1+m); bary (read only by kernels built
+// with BARY, which a batch runs for all its disks when one needs them): both bodies about
+// their barycentre (the FP64 forms' BaryPos), with no indirect term
+template <bool BARY>
+__device__ __forceinline__ PlanetF planet_f(const StepScalars* ss, bool sph, int bary) {
+  const double m = ss->planet.m, x = ss->planet.x;
   PlanetF p;
-  p.m = sph ? (float)(ss->planet.m / (1.0 + ss->planet.m)) : (float)ss->planet.m;
-  p.x = (float)ss->planet.x;
   p.rs2 = (float)(ss->planet.rs * ss->planet.rs);
-  p.inv_x2 = (float)(1.0 / (ss->planet.x * ss->planet.x));
   p.vy = (float)ss->planet.vy;
+  p.bary = BARY && bary;
+  if (p.bary) {
+    const double stm = 1.0 / (1.0 + m);
+    p.m = (float)(m * stm);
+    p.x = (float)(x * stm);
+    p.inv_x2 = 0.0f;
+    p.stx = (float)(x * m * stm);
+    p.stm = (float)stm;
+  } else {
+    p.m = sph ? (float)(m / (1.0 + m)) : (float)m;
+    p.x = (float)x;
+    p.inv_x2 = (float)(1.0 / (x * x));
+    p.stx = 0.0f;
+    p.stm = 1.0f;
+  }
   return p;
 }
 
@@ -144,7 +161,7 @@ constexpr int pencil_waves() {
 // column group grp.  Workgroup-uniform; starts with a barrier so that a workgroup can run
 // items back to back on the same LDS.  With CFL, returns in thread 0 the largest CFL term of
 // the cells it stored (the last sweep of a 3-D step).
-template <int K, int W, int Q, int G, int VISC, bool CFL, bool ADIA = false>
+template <int K, int W, int Q, int G, int VISC, bool CFL, bool ADIA = false, bool BARY = false>
 __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, const DevParams& prm,
                                              const StepScalars* __restrict__ ss, int src, int grp, int l, int row,
                                              XTile<K, W, Q, (G == kGeoRcyl && !ADIA ? 3 : 4)>& T) {
@@ -201,7 +218,7 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform, in a scalar register
   const float dt = (float)ss->dt;
-  const PlanetF pl = planet_f(ss, G != kGeoRcyl);
+  const PlanetF pl = planet_f<BARY>(ss, G != kGeoRcyl, prm.bary);
   const float sqrt_gam = sqrtf((float)prm.gam);
   // radial: overhang cells read copies of the last slot; theta: reads clamped to the window
   // theta: the theta-major table, whose slots run along the pencil (reads past the window
@@ -295,7 +312,7 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
       P0.c3 = tx.get(kTabP3);
       P0.c4 = tx.get(kTabP4);
     }
-    ppmlr_pert<K, G, (G == kGeoRcyl ? 2 : 3), TabLane<K>, ADIA, SL>(Wp, tx, P0, lim, dt, 0.0f, pl, prm.plnt > 0,
+    ppmlr_pert<K, G, (G == kGeoRcyl ? 2 : 3), TabLane<K>, ADIA, SL, BARY>(Wp, tx, P0, lim, dt, 0.0f, pl, prm.plnt > 0,
                                                                      prm.flat != 0, sqrt_gam, 0.0f, false,
                                                                      (float)prm.gam);
     if (VISC == 1 || (VISC == 2 && prm.visc)) {
@@ -383,7 +400,7 @@ __device__ __forceinline__ float pencil_item(const DevGrid& g, const DevF32& f, 
 }
 
 // one disk: one workgroup per item, XCD-contiguous
-template <int K, int W, int Q, int VISC, bool ADIA>
+template <int K, int W, int Q, int VISC, bool ADIA, bool BARY>
 __global__ __launch_bounds__(64 * W) PN_WAVES((pencil_waves<K, W, Q, ADIA ? 4 : 3>())) void k_sweep_x_f32(DevGrid g, DevF32 f, DevParams prm,
                                                         const StepScalars* __restrict__ ss, int src) {
   __shared__ XTile<K, W, Q, ADIA ? 4 : 3> T;
@@ -391,13 +408,13 @@ __global__ __launch_bounds__(64 * W) PN_WAVES((pencil_waves<K, W, Q, ADIA ? 4 : 
   const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
   const int grp = bid % ngrp, l = bid / ngrp;
   if (l >= f.nwin_x) return;   // workgroup-uniform, before any barrier
-  pencil_item<K, W, Q, kGeoRcyl, VISC, false, ADIA>(g, f, prm, ss, src, grp, l, 0, T);
+  pencil_item<K, W, Q, kGeoRcyl, VISC, false, ADIA, BARY>(g, f, prm, ss, src, grp, l, 0, T);
 }
 
 // 3-D: the spherical radial sweep (item = theta row k, radial window, column group) and the
 // theta sweep (item = ring i, theta window, column group; with do_cfl it is the step's last
 // sweep and writes one CFL partial per workgroup)
-template <int K, int W, int Q, int G, int VISC>
+template <int K, int W, int Q, int G, int VISC, bool BARY>
 __global__ __launch_bounds__(64 * W) PN_WAVES((pencil_waves<K, W, Q, 4>())) void k_sweep_pencil3_f32(DevGrid g, DevF32 f, DevParams prm,
                                                               const StepScalars* __restrict__ ss, double* cflp,
                                                               int src, int do_cfl) {
@@ -410,10 +427,10 @@ __global__ __launch_bounds__(64 * W) PN_WAVES((pencil_waves<K, W, Q, 4>())) void
   const int l = rest % nwin, row = rest / nwin;
   if (row >= nrow) return;   // workgroup-uniform, before any barrier
   if (do_cfl) {
-    const float m = pencil_item<K, W, Q, G, VISC, true>(g, f, prm, ss, src, grp, l, row, T);
+    const float m = pencil_item<K, W, Q, G, VISC, true, false, BARY>(g, f, prm, ss, src, grp, l, row, T);
     if (threadIdx.x == 0) cflp[blockIdx.x] = (double)m;
   } else {
-    pencil_item<K, W, Q, G, VISC, false>(g, f, prm, ss, src, grp, l, row, T);
+    pencil_item<K, W, Q, G, VISC, false, false, BARY>(g, f, prm, ss, src, grp, l, row, T);
   }
 }
 
@@ -440,7 +457,7 @@ constexpr int batch_pencil_waves() {
   return W == 4 ? 4 : 1;
 }
 
-template <int K, int W, int Q>
+template <int K, int W, int Q, bool BARY>
 __global__ __launch_bounds__(64 * W) PN_WAVES(batch_pencil_waves<W>()) void k_sweep_x_f32_batch(const DiskDev* __restrict__ dd,
                                                               const DevF32* __restrict__ dfs,
                                                               const int* __restrict__ base, int ndisk, int src) {
@@ -452,7 +469,8 @@ __global__ __launch_bounds__(64 * W) PN_WAVES(batch_pencil_waves<W>()) void k_sw
     const DevGrid g = dd[d].g;
     const int ngrp = (g.jmax + W * Q - 1) / (W * Q);
     const int local = it - base[d];
-    pencil_item<K, W, Q, kGeoRcyl, 2, false>(g, dfs[d], dd[d].p, dd[d].ss, src, local % ngrp, local / ngrp, 0, T);
+    pencil_item<K, W, Q, kGeoRcyl, 2, false, false, BARY>(g, dfs[d], dd[d].p, dd[d].ss, src, local % ngrp, local / ngrp,
+                                                         0, T);
   }
 }
 
@@ -460,7 +478,7 @@ __global__ __launch_bounds__(64 * W) PN_WAVES(batch_pencil_waves<W>()) void k_sw
 // Azimuthal sweep of window l of interior ring n (3-D: ring n % imax of theta row n / imax)
 // by one wavefront; returns the wave's maximum CFL term of the cells it stored (2-D).  The
 // sweep's radius is r_cyl; in 3-D it also carries v_theta and the planet's spherical forms.
-template <int K, int VISC, int NDIM, bool ADIA = false>
+template <int K, int VISC, int NDIM, bool ADIA = false, bool BARY = false>
 __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, const DevParams& prm,
                                           const StepScalars* __restrict__ ss, int src, int n, int l) {
   constexpr int REAL = 64 * K - 2 * kNpad;
@@ -469,7 +487,7 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
   const int jmax = g.jmax;
   const float dt = (float)ss->dt;
   const double FrRot = ss->FrRot;
-  const PlanetF pl = planet_f(ss, NDIM == 3);
+  const PlanetF pl = planet_f<BARY>(ss, NDIM == 3, prm.bary);
   const int lim = win_len_k<K>(jmax, f.nwin_y, l);
   const int rot = g.rot_j[s];
   const double rcd = g.radc[s];
@@ -541,8 +559,9 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
   }
   const float sqrt_gam = sqrtf((float)prm.gam);
   const Para<float, K> P0 = f.uniform_phi ? paraset_uniform<K>() : paraset(0, lim, W.dx0);
-  ppmlr_pert<K, kGeoPhi, NDIM, TabLane<K>, ADIA>(W, TabLane<K>{}, P0, lim, dt, wres, pl, prm.plnt > 0,
-                                                 prm.flat != 0, sqrt_gam, rsph, f.uniform_phi != 0, (float)prm.gam);
+  ppmlr_pert<K, kGeoPhi, NDIM, TabLane<K>, ADIA, 64, BARY>(W, TabLane<K>{}, P0, lim, dt, wres, pl, prm.plnt > 0,
+                                                           prm.flat != 0, sqrt_gam, rsph, f.uniform_phi != 0,
+                                                           (float)prm.gam);
   if (VISC == 1 || (VISC == 2 && prm.visc)) visc_p_pert<K, NDIM == 3>(W, lim, dt, T[kTabNu * ts], rc);
 
   const Buf dr(f.f[src ^ 1][0] + base, rb4), du(f.f[src ^ 1][1] + base, rb4), dv(f.f[src ^ 1][2] + base, rb4);
@@ -572,7 +591,7 @@ __device__ __forceinline__ float y_pencil(const DevGrid& g, const DevF32& f, con
 
 // one wavefront per (ring, window); with do_cfl each workgroup writes the maximum CFL term
 // of its waves to cflp[workgroup] (2-D)
-template <int K, int WPB, int VISC, int NDIM, bool ADIA>
+template <int K, int WPB, int VISC, int NDIM, bool ADIA, bool BARY>
 __global__ __launch_bounds__(64 * WPB) PN_WAVES((K == 2 && NDIM == 2 && !ADIA) ? 8 : 1) void k_sweep_y_f32(DevGrid g, DevF32 f, DevParams prm,
                                                           const StepScalars* __restrict__ ss, double* cflp, int src,
                                                           int do_cfl) {
@@ -582,7 +601,7 @@ __global__ __launch_bounds__(64 * WPB) PN_WAVES((K == 2 && NDIM == 2 && !ADIA) ?
   const int l = gw % f.nwin_y;
   const int n = gw / f.nwin_y;
   float m = 0.0f;
-  if (n < g.imax * g.kmax) m = y_pencil<K, VISC, NDIM, ADIA>(g, f, prm, ss, src, n, l);   // wave-uniform
+  if (n < g.imax * g.kmax) m = y_pencil<K, VISC, NDIM, ADIA, BARY>(g, f, prm, ss, src, n, l);   // wave-uniform
   if (do_cfl) {   // kernel-uniform: every wave reaches the barrier
     if ((threadIdx.x & 63) == 0) shm[wv] = m;
     __syncthreads();
@@ -595,7 +614,7 @@ __global__ __launch_bounds__(64 * WPB) PN_WAVES((K == 2 && NDIM == 2 && !ADIA) ?
 
 // many disks, persistent (as k_sweep_x_f32_batch): item = one workgroup's WPB waves of one
 // disk; each item writes its CFL partial to its disk's cflp[item - base[d]]
-template <int K, int WPB>
+template <int K, int WPB, bool BARY>
 __global__ __launch_bounds__(64 * WPB) PN_WAVES(K == 2 ? 6 : 1) void k_sweep_y_f32_batch(const DiskDev* __restrict__ dd,
                                                               const DevF32* __restrict__ dfs,
                                                               const int* __restrict__ base, int ndisk, int src) {
@@ -612,7 +631,7 @@ __global__ __launch_bounds__(64 * WPB) PN_WAVES(K == 2 ? 6 : 1) void k_sweep_y_f
     const int l = gw % f.nwin_y;
     const int n = gw / f.nwin_y;
     float m = 0.0f;
-    if (n < g.imax) m = y_pencil<K, 2, 2>(g, f, dd[d].p, dd[d].ss, src, n, l);   // wave-uniform
+    if (n < g.imax) m = y_pencil<K, 2, 2, false, BARY>(g, f, dd[d].p, dd[d].ss, src, n, l);   // wave-uniform
     __syncthreads();   // the previous item's partial has been read
     if ((threadIdx.x & 63) == 0) shm[wv] = m;
     __syncthreads();
@@ -705,6 +724,35 @@ __global__ __launch_bounds__(256) void k_reduce_f32(DevGrid g, DevF32 f, DevPara
 // so Sigma - Sigma_b, v_phi - v_bg and v_theta scale by 1 - k and v_r relaxes towards the
 // record's v_r; EOS 3 relaxes p and Sigma and carries q = p / Sigma - q_b again, in FP64.
 // Thread: one cell of an interior ring.
+// set_boundx (boundary.cu:86-206) on the FP32 state of buffer src: the copies of the FP64
+// path's k_fill_ghosts, of deviations.  A ghost ring a code copies into carries its source
+// ring's backgrounds (set on load), so the copied deviation is the copied value; code 0
+// negates v_r, code 4 copies Sigma and P (and v_r on the left) and leaves v_phi and v_theta
+// fixed.  Fields: 0 Sigma, 1 v_r, 2 v_phi, 3 v_theta (3-D) or q = P / Sigma (EOS 3).
+// Thread: one lab column of one of the 12 ghost rings of a theta row.
+__global__ void k_fill_ghosts_f32(DevGrid g, DevF32 f, DevParams prm, int src) {
+  const int jl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (jl >= g.jmax) return;
+  const int gi = blockIdx.y % (2 * kNpad), k = blockIdx.y / (2 * kNpad);
+  const bool left = gi < kNpad;
+  const int code = left ? prm.nlft : prm.nrgh;
+  const int t = left ? kNpad - gi : gi - kNpad + 1;   // cells outside the edge cell
+  const int srci = ghost_source(code, left, t, g.imax);
+  if (srci < 0) return;
+  const int sg = left ? slot_of(g, -t, k) : slot_of(g, g.imax - 1 + t, k);
+  const int ss = slot_of(g, srci, k);
+  int jg = jl - g.rot_j[sg], js = jl - g.rot_j[ss];
+  jg += (jg < 0) ? g.jmax : 0;
+  jg -= (jg >= g.jmax) ? g.jmax : 0;
+  js += (js < 0) ? g.jmax : 0;
+  js -= (js >= g.jmax) ? g.jmax : 0;
+  const long ag = (long)sg * g.jmax + jg, as = (long)ss * g.jmax + js;
+  f.f[src][0][ag] = f.f[src][0][as];
+  if (code != 4 || left) f.f[src][1][ag] = (code == 0) ? -f.f[src][1][as] : f.f[src][1][as];
+  if (code != 4) f.f[src][2][ag] = f.f[src][2][as];
+  if ((g.ndim == 3 && code != 4) || prm.eos == 3) f.f[src][3][ag] = f.f[src][3][as];
+}
+
 __global__ void k_damp_f32(DevGrid g, DevF32 f, DevParams prm, const StepScalars* __restrict__ ss, int src) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= g.jmax) return;
@@ -843,8 +891,15 @@ hipError_t sweep_x_k(const LaunchCtx& c, const DevF32& f0, int src) {
   dim3 grid((unsigned)(ngrp * f.nwin_x)), block(64 * W);
   // the same body as the batched kernels (viscosity switched by the parameters), so a disk
   // computes the same bits alone or in a batch
-  if (c.p.eos == 3) k_sweep_x_f32<K, W, Q, 0, true><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
-  else k_sweep_x_f32<K, W, Q, 2, false><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+  // the barycentric frame's star terms are compiled into kernels of their own (BARY), so the
+  // others run without them
+  auto go = [&](auto bary) {
+    constexpr bool B = decltype(bary)::value;
+    if (c.p.eos == 3) k_sweep_x_f32<K, W, Q, 0, true, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+    else k_sweep_x_f32<K, W, Q, 2, false, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+  };
+  if (c.p.bary) go(std::true_type{});
+  else go(std::false_type{});
   return hipGetLastError();
 }
 
@@ -856,11 +911,18 @@ hipError_t sweep_y_k(const LaunchCtx& c, const DevF32& f0, int src, int do_cfl) 
   const long nblk = (waves + WPB - 1) / WPB;
   if (do_cfl && (nblk > c.ncflp || c.g.ndim == 3)) return hipErrorInvalidValue;
   dim3 grid((unsigned)nblk), block(64 * WPB);
-  if (c.g.ndim == 3 && c.p.visc) k_sweep_y_f32<K, WPB, 1, 3, false><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, 0);
-  else if (c.g.ndim == 3) k_sweep_y_f32<K, WPB, 0, 3, false><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, 0);
-  else if (c.p.eos == 3)
-    k_sweep_y_f32<K, WPB, 0, 2, true><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
-  else k_sweep_y_f32<K, WPB, 2, 2, false><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  auto go = [&](auto bary) {
+    constexpr bool B = decltype(bary)::value;
+    if (c.g.ndim == 3 && c.p.visc)
+      k_sweep_y_f32<K, WPB, 1, 3, false, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, 0);
+    else if (c.g.ndim == 3)
+      k_sweep_y_f32<K, WPB, 0, 3, false, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, 0);
+    else if (c.p.eos == 3)
+      k_sweep_y_f32<K, WPB, 0, 2, true, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+    else k_sweep_y_f32<K, WPB, 2, 2, false, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  };
+  if (c.p.bary) go(std::true_type{});
+  else go(std::false_type{});
   return hipGetLastError();
 }
 
@@ -881,8 +943,14 @@ hipError_t sweep_pencil3_k(const LaunchCtx& c, const DevF32& f0, int src, int do
   if (do_cfl && (!th || nblk > c.ncflp)) return hipErrorInvalidValue;
   dim3 grid((unsigned)nblk), block(64 * W);
   // a viscous disk runs its own instantiation, so the inviscid one keeps its registers
-  if (c.p.visc) k_sweep_pencil3_f32<K, W, Q, G, 1><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
-  else k_sweep_pencil3_f32<K, W, Q, G, 0><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  auto go = [&](auto bary) {
+    constexpr bool B = decltype(bary)::value;
+    if (c.p.visc)
+      k_sweep_pencil3_f32<K, W, Q, G, 1, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+    else k_sweep_pencil3_f32<K, W, Q, G, 0, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
+  };
+  if (c.p.bary) go(std::true_type{});
+  else go(std::false_type{});
   return hipGetLastError();
 }
 
@@ -971,39 +1039,48 @@ int persistent_grid(Kern k, int block) {
 
 template <int K, int W, int Q>
 hipError_t sweep_x_batch_k(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
-                           int src, int grid) {
+                           int src, int grid, bool bary) {
   if (grid < 8 || grid % 8) return hipErrorInvalidValue;
-  k_sweep_x_f32_batch<K, W, Q><<<grid, 64 * W, 0, s>>>(dd, dfs, base, ndisk, src);
+  if (bary) k_sweep_x_f32_batch<K, W, Q, true><<<grid, 64 * W, 0, s>>>(dd, dfs, base, ndisk, src);
+  else k_sweep_x_f32_batch<K, W, Q, false><<<grid, 64 * W, 0, s>>>(dd, dfs, base, ndisk, src);
   return hipGetLastError();
 }
 
 template <int K, int WPB>
 hipError_t sweep_y_batch_k(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
-                           int src, int grid) {
+                           int src, int grid, bool bary) {
   if (grid < 8 || grid % 8) return hipErrorInvalidValue;
-  k_sweep_y_f32_batch<K, WPB><<<grid, 64 * WPB, 0, s>>>(dd, dfs, base, ndisk, src);
+  if (bary) k_sweep_y_f32_batch<K, WPB, true><<<grid, 64 * WPB, 0, s>>>(dd, dfs, base, ndisk, src);
+  else k_sweep_y_f32_batch<K, WPB, false><<<grid, 64 * WPB, 0, s>>>(dd, dfs, base, ndisk, src);
   return hipGetLastError();
 }
 
 }  // namespace
 
-int f32_batch_grid(int axis, int variant) {
+namespace {
+template <bool B>
+int batch_grid(int axis, int variant) {
   if (axis == 0) {
     switch (variant) {
-      case 0: return persistent_grid(k_sweep_x_f32_batch<2, 4, 2>, 256);
-      case 1: return persistent_grid(k_sweep_x_f32_batch<2, 8, 1>, 512);
-      case 2: return persistent_grid(k_sweep_x_f32_batch<2, 16, 1>, 1024);
-      case 3: return persistent_grid(k_sweep_x_f32_batch<2, 4, 4>, 256);
+      case 0: return persistent_grid(k_sweep_x_f32_batch<2, 4, 2, B>, 256);
+      case 1: return persistent_grid(k_sweep_x_f32_batch<2, 8, 1, B>, 512);
+      case 2: return persistent_grid(k_sweep_x_f32_batch<2, 16, 1, B>, 1024);
+      case 3: return persistent_grid(k_sweep_x_f32_batch<2, 4, 4, B>, 256);
     }
   } else {
     switch (variant) {
-      case 0: return persistent_grid(k_sweep_y_f32_batch<2, 2>, 128);
-      case 1: return persistent_grid(k_sweep_y_f32_batch<2, 4>, 256);
-      case 2: return persistent_grid(k_sweep_y_f32_batch<4, 2>, 128);
-      case 3: return persistent_grid(k_sweep_y_f32_batch<4, 4>, 256);
+      case 0: return persistent_grid(k_sweep_y_f32_batch<2, 2, B>, 128);
+      case 1: return persistent_grid(k_sweep_y_f32_batch<2, 4, B>, 256);
+      case 2: return persistent_grid(k_sweep_y_f32_batch<4, 2, B>, 128);
+      case 3: return persistent_grid(k_sweep_y_f32_batch<4, 4, B>, 256);
     }
   }
   return 0;
+}
+}  // namespace
+
+int f32_batch_grid(int axis, int variant, bool bary) {
+  return bary ? batch_grid<true>(axis, variant) : batch_grid<false>(axis, variant);
 }
 
 int f32_items_x(const DevGrid& g, int variant) {
@@ -1012,23 +1089,23 @@ int f32_items_x(const DevGrid& g, int variant) {
 }
 
 hipError_t f32_sweep_x_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
-                             int variant, int src, int grid) {
+                             int variant, int src, int grid, bool bary) {
   switch (variant) {
-    case 0: return sweep_x_batch_k<2, 4, 2>(s, dd, dfs, base, ndisk, src, grid);
-    case 1: return sweep_x_batch_k<2, 8, 1>(s, dd, dfs, base, ndisk, src, grid);
-    case 2: return sweep_x_batch_k<2, 16, 1>(s, dd, dfs, base, ndisk, src, grid);
-    case 3: return sweep_x_batch_k<2, 4, 4>(s, dd, dfs, base, ndisk, src, grid);
+    case 0: return sweep_x_batch_k<2, 4, 2>(s, dd, dfs, base, ndisk, src, grid, bary);
+    case 1: return sweep_x_batch_k<2, 8, 1>(s, dd, dfs, base, ndisk, src, grid, bary);
+    case 2: return sweep_x_batch_k<2, 16, 1>(s, dd, dfs, base, ndisk, src, grid, bary);
+    case 3: return sweep_x_batch_k<2, 4, 4>(s, dd, dfs, base, ndisk, src, grid, bary);
   }
   return hipErrorInvalidValue;
 }
 
 hipError_t f32_sweep_y_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
-                             int variant, int src, int grid) {
+                             int variant, int src, int grid, bool bary) {
   switch (variant) {
-    case 0: return sweep_y_batch_k<2, 2>(s, dd, dfs, base, ndisk, src, grid);
-    case 1: return sweep_y_batch_k<2, 4>(s, dd, dfs, base, ndisk, src, grid);
-    case 2: return sweep_y_batch_k<4, 2>(s, dd, dfs, base, ndisk, src, grid);
-    case 3: return sweep_y_batch_k<4, 4>(s, dd, dfs, base, ndisk, src, grid);
+    case 0: return sweep_y_batch_k<2, 2>(s, dd, dfs, base, ndisk, src, grid, bary);
+    case 1: return sweep_y_batch_k<2, 4>(s, dd, dfs, base, ndisk, src, grid, bary);
+    case 2: return sweep_y_batch_k<4, 2>(s, dd, dfs, base, ndisk, src, grid, bary);
+    case 3: return sweep_y_batch_k<4, 4>(s, dd, dfs, base, ndisk, src, grid, bary);
   }
   return hipErrorInvalidValue;
 }
@@ -1052,6 +1129,13 @@ hipError_t f32_reduce(const LaunchCtx& c, const DevF32& f, int src, int mode, do
 hipError_t f32_damp(const LaunchCtx& c, const DevF32& f, int src) {
   dim3 grid((c.g.jmax + 255) / 256, c.g.imax * c.g.kmax);
   k_damp_f32<<<grid, 256, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+  return hipGetLastError();
+}
+
+hipError_t f32_fill_ghosts(const LaunchCtx& c, const DevF32& f, int src) {
+  if (c.p.nlft == 3 && c.p.nrgh == 3) return hipSuccess;
+  dim3 grid((c.g.jmax + 255) / 256, 2 * kNpad * c.g.kmax);
+  k_fill_ghosts_f32<<<grid, 256, 0, c.stream>>>(c.g, f, c.p, src);
   return hipGetLastError();
 }
 

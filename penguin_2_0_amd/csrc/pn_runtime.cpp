@@ -116,10 +116,6 @@ int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision)
   if (precision == PN_FP32 && p->eos != 0 && !(p->eos == 3 && p->ndim == 2 && !p->visc_flag))
     return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for EOS 0 and the inviscid 2-D EOS 3 disk");
   if (p->bary_flag != 0 && p->bary_flag != 1) return fail(c, PN_ERR_ARG, "bary_flag %d", p->bary_flag);
-  if (precision == PN_FP32 && (p->nlft != 3 || p->nrgh != 3))
-    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path keeps fixed radial ghost rings");
-  if (precision == PN_FP32 && p->bary_flag)
-    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path balances its backgrounds about a star at the origin");
   return PN_OK;
 }
 
@@ -158,6 +154,7 @@ int launch_sweeps(pn_ctx* c, int& cur, int do_cfl, bool prep, bool roll) {
   LaunchCtx L = lctx(c);
   if (is_f32(c)) {
     if (prep) PN_LAUNCH(c, "prep", f32_prep(L, c->df));
+    PN_LAUNCH(c, "ghosts", f32_fill_ghosts(L, c->df, cur));
     PN_LAUNCH(c, "sweep_x", f32_x(c, L, c->var_x, cur));
     if (c->P.ndim == 3) {
       PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, cur ^ 1, 0));
@@ -477,13 +474,31 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
     for (long s = 0; s < nslot; ++s) rbg[s] = F[FR][static_cast<size_t>(s) * J];
     for (int k = 0; k < K; ++k)
       for (int i = 0; i < I; ++i) rbg[slot_of(d, i, k)] = val[i + static_cast<long>(I) * k].r;
-    c->h_rbg = rbg;
     // EOS 3: the background pressure of the same records, and q_b = p_b / Sigma_b
     std::vector<double> pbg(nslot), qbg(nslot);
     for (long s = 0; s < nslot; ++s) pbg[s] = adia ? F[FP][static_cast<size_t>(s) * J] : scs2[s] * rbg[s];
     if (adia)
       for (int k = 0; k < K; ++k)
         for (int i = 0; i < I; ++i) pbg[slot_of(d, i, k)] = val[i + static_cast<long>(I) * k].p;
+    // a ghost ring whose values a radial boundary code copies from an interior ring each step
+    // (set_boundx, boundary.cu:86-206) carries that ring's backgrounds: Sigma_b and P_b (so
+    // c_s^2 = P / Sigma as the copies make it), and v_bg unless the code leaves v_phi fixed
+    // (code 4).  A copied deviation is then the copied value.
+    std::vector<double> bcs2 = scs2, bvbg = svbg;
+    for (int side = 0; side < 2; ++side) {
+      const int code = side == 0 ? c->P.nlft : c->P.nrgh;
+      for (int k = 0; k < K; ++k)
+        for (int t = 1; t <= PN_NPAD; ++t) {
+          const int si = ghost_source(code, side == 0, t, I);
+          if (si < 0) continue;
+          const int sg = slot_of(d, side == 0 ? -t : I - 1 + t, k), ss = slot_of(d, si, k);
+          rbg[sg] = rbg[ss];
+          pbg[sg] = pbg[ss];
+          bcs2[sg] = bcs2[ss];
+          if (code != 4) bvbg[sg] = bvbg[ss];
+        }
+    }
+    c->h_rbg = rbg;
     for (long s = 0; s < nslot; ++s) qbg[s] = pbg[s] / rbg[s];
     c->h_qbg = qbg;
     // Sigma - Sigma_b, v_r, v_phi - v_bg and (3-D) v_theta, rounded once
@@ -494,7 +509,7 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
         const size_t a = static_cast<size_t>(s) * J + j;
         G[0][a] = static_cast<float>(F[FR][a] - rbg[s]);
         G[1][a] = static_cast<float>(F[FU][a]);
-        G[2][a] = static_cast<float>(F[FV][a] - svbg[s]);
+        G[2][a] = static_cast<float>(F[FV][a] - bvbg[s]);
         if (d3) G[3][a] = static_cast<float>(F[FW][a]);
         if (adia) G[3][a] = static_cast<float>(F[FP][a] / F[FR][a] - qbg[s]);
       }
@@ -510,7 +525,7 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
     std::vector<double> pb(nslot), Lb(nslot);
     for (long s = 0; s < nslot; ++s) {
       pb[s] = pbg[s];
-      Lb[s] = sradc[s] * svbg[s];
+      Lb[s] = sradc[s] * bvbg[s];
     }
     for (long s = 0; s < nslot; ++s) {
       const long ir = s % sr, kz = s / sr;
@@ -539,8 +554,8 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
       q[kTabDrb] = hn ? rbg[s + 1] - rbg[s] : 0.0;
       q[kTabPb] = pb[s];
       q[kTabDpb] = hn ? pb[s + 1] - pb[s] : 0.0;
-      q[kTabCs2] = scs2[s];
-      q[kTabCs] = std::sqrt(scs2[s]);
+      q[kTabCs2] = bcs2[s];
+      q[kTabCs] = std::sqrt(bcs2[s]);
       q[kTabLb] = Lb[s];
       q[kTabDLb] = hn ? Lb[s + 1] - Lb[s] : 0.0;
       q[kTabResBg] = fb - abg;
@@ -642,7 +657,7 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
           const double nv = c->P.vis_nu;
           fitz(nv * rbg[s - sr], nv * rbg[s], nv * rbg[s + sr], d1, d2);
           q[kTabDnbZ] = d1;
-          fitz(svbg[s - sr], svbg[s], svbg[s + sr], d1, d2);
+          fitz(bvbg[s - sr], bvbg[s], bvbg[s + sr], d1, d2);
           q[kTabD1VZ] = d1;
           q[kTabD2VZ] = d2;
         }
@@ -672,7 +687,7 @@ int pn_load_rings(pn_ctx* c, const void* rings, size_t rings_bytes, const void* 
     PN_HIP(c, hipMemcpyAsync(const_cast<float*>(c->df.dy), t_dy.data(), J * sizeof(float), hipMemcpyHostToDevice, st));
     PN_HIP(c, hipMemcpyAsync(const_cast<float*>(c->df.yvol), t_yvol.data(), J * sizeof(float), hipMemcpyHostToDevice, st));
     PN_HIP(c, hipMemcpyAsync(const_cast<double*>(c->df.rbg), c->h_rbg.data(), nslot * sizeof(double), hipMemcpyHostToDevice, st));
-    PN_HIP(c, hipMemcpyAsync(const_cast<double*>(c->df.vbg), svbg.data(), nslot * sizeof(double), hipMemcpyHostToDevice, st));
+    PN_HIP(c, hipMemcpyAsync(const_cast<double*>(c->df.vbg), bvbg.data(), nslot * sizeof(double), hipMemcpyHostToDevice, st));
     c->df.nwin_x = f32_windows(I, kF32VariantsX[c->var_x][0]);
     c->df.nwin_y = f32_windows(J, kF32VariantsY[c->var_y][0]);
     c->df.nwin_z = f32_windows(K, kF32VariantsZ[c->var_z][0]);
@@ -863,6 +878,7 @@ int pn_sweep_axis(pn_ctx* c, int axis, double dt, const pn_body* planet, double 
   LaunchCtx L = lctx(c);
   if (is_f32(c)) {
     PN_LAUNCH(c, "prep", f32_prep(L, c->df));
+    if (axis == 0) PN_LAUNCH(c, "ghosts", f32_fill_ghosts(L, c->df, c->cur));
     if (axis == 0) PN_LAUNCH(c, "sweep_x", f32_x(c, L, c->var_x, c->cur));
     else if (axis == 1) PN_LAUNCH(c, "sweep_y", f32_sweep_y(L, c->df, c->var_y, c->cur, 0));
     else PN_LAUNCH(c, "sweep_z", f32_sweep_z(L, c->df, c->var_z, c->cur, 0));

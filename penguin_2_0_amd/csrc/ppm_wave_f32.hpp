@@ -57,9 +57,31 @@ struct Buf {
   }
 };
 
+// m, x: the planet's mass and distance as the force forms take them (about the star, or in
+// the barycentric frame m/(1+m) and x/(1+m)); inv_x2: the indirect term's 1/x^2 (0 in the
+// barycentric frame); stx, stm: the star's distance from the barycentre and its mass 1/(1+m)
+// (barycentric frame only, bary = 1)
 struct PlanetF {
-  float m, x, rs2, inv_x2, vy;
+  float m, x, rs2, inv_x2, vy, stx, stm;
+  int bary;
 };
+
+// The star's pull in the barycentric frame (bary_flag, forces.cu:28-40, 116-126): the star
+// sits at stx opposite the planet, so with cf = cos(dphi) (times sin theta in 3-D)
+// Rs^2 = r^2 + A, A = stx (stx + 2 r cf).  star_ri3 is 1/Rs^3; star_fr_rest the radial pull
+// beyond the -1/r^2 the backgrounds balance and the monopole m/(1+m)/r^2 the callers add,
+//   -stm (r + stx cf)/Rs^3 + stm/r^2 = stm (A (Rs^2 + Rs r + r^2) / ((Rs + r) r^2) - stx cf) / Rs^3,
+// every term of the order of m (no cancellation against 1/r^2).
+__device__ __forceinline__ float star_ri3(const PlanetF& p, float r, float cf) {
+  const float ri = __frsqrt_rn(r * r + p.stx * (p.stx + 2.0f * r * cf));
+  return ri * ri * ri;
+}
+__device__ __forceinline__ float star_fr_rest(const PlanetF& p, float r, float cf) {
+  const float A = p.stx * (p.stx + 2.0f * r * cf);
+  const float Rs2 = r * r + A;
+  const float ri = __frsqrt_rn(Rs2), Rs = Rs2 * ri;
+  return p.stm * (ri * ri * ri) * (A * (Rs2 + Rs * r + r * r) / ((Rs + r) * r * r) - p.stx * cf);
+}
 
 // sin / cos of (A + e)/2 from sin / cos of A/2 and a small e (|e| < ~0.5)
 __device__ __forceinline__ void rot_half(float s, float c, float e, float& so, float& co) {
@@ -71,21 +93,27 @@ __device__ __forceinline__ void rot_half(float s, float c, float e, float& so, f
 }
 
 // planet gravity with the indirect term (forces.cu:112-159) from the half angle to the planet
+template <bool BARY = false>
 __device__ __forceinline__ float planet_fr(const PlanetF& p, float rad, float s, float c) {
   const float s2 = s * s;
   const float cosd = 1.0f - 2.0f * s2;
   const float dr = rad - p.x;
   const float R2 = dr * dr + 4.0f * rad * p.x * s2 + p.rs2;
   const float ri = __frsqrt_rn(R2);
-  return -p.m * (dr + 2.0f * p.x * s2) * (ri * ri * ri) - p.m * cosd * p.inv_x2;
+  float f = -p.m * (dr + 2.0f * p.x * s2) * (ri * ri * ri) - p.m * cosd * p.inv_x2;
+  if constexpr (BARY) if (p.bary) f += p.m / (rad * rad) + star_fr_rest(p, rad, cosd);
+  return f;
 }
+template <bool BARY = false>
 __device__ __forceinline__ float planet_fphi(const PlanetF& p, float rad, float s, float c) {
   const float s2 = s * s;
   const float sind = 2.0f * s * c;
   const float dr = rad - p.x;
   const float R2 = dr * dr + 4.0f * rad * p.x * s2 + p.rs2;
   const float ri = __frsqrt_rn(R2);
-  return -p.m * p.x * sind * (ri * ri * ri) + p.m * sind * p.inv_x2;
+  float f = -p.m * p.x * sind * (ri * ri * ri) + p.m * sind * p.inv_x2;
+  if constexpr (BARY) if (p.bary) f += p.stm * p.stx * sind * star_ri3(p, rad, 1.0f - 2.0f * s2);
+  return f;
 }
 
 // spherical forms (forces.cu:23-110) for a planet in the midplane and a cell at (r, theta),
@@ -93,22 +121,29 @@ __device__ __forceinline__ float planet_fphi(const PlanetF& p, float rad, float 
 // = (r - r_p)^2 + 2 r r_p k with k = oms + 2 st sin^2(dphi/2) = 1 - cos(dphi) st.  Here p.m is
 // the reduced mass m/(1+m) of these forms; the star's -1/(1+m)/r^2 = -1/r^2 + p.m/r^2 is
 // left to the caller.
+template <bool BARY = false>
 __device__ __forceinline__ float planet_fr_sph(const PlanetF& p, float r, float st, float oms, float s, float c) {
   const float s2 = s * s;
   const float cosd = 1.0f - 2.0f * s2;
   const float k = oms + 2.0f * st * s2;
   const float dr = r - p.x;
   const float ri = __frsqrt_rn(dr * dr + 2.0f * r * p.x * k + p.rs2);
-  return -p.m * (dr + p.x * k) * (ri * ri * ri) - p.m * st * cosd * p.inv_x2;
+  float f = -p.m * (dr + p.x * k) * (ri * ri * ri) - p.m * st * cosd * p.inv_x2;
+  if constexpr (BARY) if (p.bary) f += star_fr_rest(p, r, st * cosd);
+  return f;
 }
+template <bool BARY = false>
 __device__ __forceinline__ float planet_fphi_sph(const PlanetF& p, float r, float st, float oms, float s, float c) {
   const float s2 = s * s;
   const float sind = 2.0f * s * c;
   const float k = oms + 2.0f * st * s2;
   const float dr = r - p.x;
   const float ri = __frsqrt_rn(dr * dr + 2.0f * r * p.x * k + p.rs2);
-  return -p.m * p.x * sind * (ri * ri * ri) + p.m * st * sind * p.inv_x2;
+  float f = -p.m * p.x * sind * (ri * ri * ri) + p.m * st * sind * p.inv_x2;
+  if constexpr (BARY) if (p.bary) f += p.stm * p.stx * sind * star_ri3(p, r, st * (1.0f - 2.0f * s2));
+  return f;
 }
+template <bool BARY = false>
 __device__ __forceinline__ float planet_fth_sph(const PlanetF& p, float r, float st, float ct, float oms, float s,
                                                 float c) {
   const float s2 = s * s;
@@ -116,7 +151,9 @@ __device__ __forceinline__ float planet_fth_sph(const PlanetF& p, float r, float
   const float k = oms + 2.0f * st * s2;
   const float dr = r - p.x;
   const float ri = __frsqrt_rn(dr * dr + 2.0f * r * p.x * k + p.rs2);
-  return p.m * p.x * cosd * ct * (ri * ri * ri) - p.m * cosd * ct / st * p.inv_x2;
+  float f = p.m * p.x * cosd * ct * (ri * ri * ri) - p.m * cosd * ct / st * p.inv_x2;
+  if constexpr (BARY) if (p.bary) f -= p.stm * p.stx * cosd * ct * star_ri3(p, r, st * cosd);
+  return f;
 }
 
 // true in every lane when the condition holds in any lane of the wavefront (wave-uniform)
@@ -319,7 +356,7 @@ __device__ __forceinline__ void rot_theta(float st, float ct, float oms, float e
 // Eulerian grid (its values outside the window's paraset ranges never reach a parabola the
 // sweep keeps).  wres: the azimuthal grid's uniform angular rate.  rsph: the spherical
 // radius of an azimuthal 3-D ring.  On return cells [6, nmax-6) hold the update.
-template <int K, int G, int NDIM, class Tab, bool ADIA = false, int SL = 64>
+template <int K, int G, int NDIM, class Tab, bool ADIA = false, int SL = 64, bool BARY = false>
 __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para<float, K>& P0, int nmax,
                                            float dt, float wres, const PlanetF& pl, bool planet, bool flat_on,
                                            float sqrt_gam, float rsph = 0.0f, bool phi_rows = false,
@@ -393,10 +430,10 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
         float s, c;
         rot_by(W.ts[h], W.tc[h], W.rs[h], W.rcs[h], s, c);
         if constexpr (G == kGeoRcyl) {
-          f += planet_fr(pl, W.xa0[h], s, c);
+          f += planet_fr<BARY>(pl, W.xa0[h], s, c);
         } else {
           const float ix = 1.0f / W.xa0[h];
-          f += pl.m * ix * ix + planet_fr_sph(pl, W.xa0[h], W.sth[h], W.oms[h], s, c);
+          f += pl.m * ix * ix + planet_fr_sph<BARY>(pl, W.xa0[h], W.sth[h], W.oms[h], s, c);
         }
       }
       fhalf[h] = hdt * f;
@@ -411,8 +448,8 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
         // rotation rows on a uniform grid, else the series
         if (phi_rows) rot_by(W.ts[h], W.tc[h], W.rs[h], W.rcs[h], s, c);
         else rot_half(W.ts[h], W.tc[h], -0.5f * W.dx0[h] + hdt * (W.vrot[h] - pl.vy), s, c);
-        if constexpr (NDIM == 2) f = planet_fphi(pl, W.rad[h], s, c);
-        else f = planet_fphi_sph(pl, rsph, W.sth[h], W.oms[h], s, c);
+        if constexpr (NDIM == 2) f = planet_fphi<BARY>(pl, W.rad[h], s, c);
+        else f = planet_fphi_sph<BARY>(pl, rsph, W.sth[h], W.oms[h], s, c);
       }
       fhalf[h] = hdt * f;
     }
@@ -432,7 +469,7 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
       if (planet) {
         float s, c;
         rot_by(W.ts[h], W.tc[h], W.rs[h], W.rcs[h], s, c);
-        f += planet_fth_sph(pl, W.rad[h], szl[h], czl[h], 1.0f - szl[h], s, c);
+        f += planet_fth_sph<BARY>(pl, W.rad[h], szl[h], czl[h], 1.0f - szl[h], s, c);
       }
       fhalf[h] = hdt * f;
     }
@@ -633,13 +670,13 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
           float s, cc;
           rot_by2(W.ts[h], W.tc[h], W.rs[h], W.rcs[h], s, cc);
           if constexpr (G == kGeoRcyl) {
-            const float p1 = planet_fr(pl, W.rad[h], W.ts[h], W.tc[h]), p2 = planet_fr(pl, W.rad[h] + dxc, s, cc);
+            const float p1 = planet_fr<BARY>(pl, W.rad[h], W.ts[h], W.tc[h]), p2 = planet_fr<BARY>(pl, W.rad[h] + dxc, s, cc);
             fdiff += p1 + p2;
             f21 += p2 - p1;
           } else {
             // the star pulls with 1/(1+m): -1/r^2 + plm/r^2 at both centres
-            fdiff += pl.m * ixc * ixc * (2.0f + g2) + planet_fr_sph(pl, W.rad[h], W.sth[h], W.oms[h], W.ts[h], W.tc[h]) +
-                     planet_fr_sph(pl, W.rad[h] + dxc, W.sth[h], W.oms[h], s, cc);
+            fdiff += pl.m * ixc * ixc * (2.0f + g2) + planet_fr_sph<BARY>(pl, W.rad[h], W.sth[h], W.oms[h], W.ts[h], W.tc[h]) +
+                     planet_fr_sph<BARY>(pl, W.rad[h] + dxc, W.sth[h], W.oms[h], s, cc);
           }
         }
         // pressure: (p*(c+1) - p*(c)) area / (rho dvol0), area = V0 + dV, and
@@ -703,12 +740,12 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
           float s, cc;
           rot_half(W.ts[h], W.tc[h], off, s, cc);
           if constexpr (NDIM == 2) {
-            const float p1 = planet_fphi(pl, W.rad[h], W.ts[h], W.tc[h]), p2 = planet_fphi(pl, W.rad[h], s, cc);
+            const float p1 = planet_fphi<BARY>(pl, W.rad[h], W.ts[h], W.tc[h]), p2 = planet_fphi<BARY>(pl, W.rad[h], s, cc);
             f = p1 + p2;
             f21 = p2 - p1;
           } else {
-            f = planet_fphi_sph(pl, rsph, W.sth[h], W.oms[h], W.ts[h], W.tc[h]) +
-                planet_fphi_sph(pl, rsph, W.sth[h], W.oms[h], s, cc);
+            f = planet_fphi_sph<BARY>(pl, rsph, W.sth[h], W.oms[h], W.ts[h], W.tc[h]) +
+                planet_fphi_sph<BARY>(pl, rsph, W.sth[h], W.oms[h], s, cc);
           }
         }
         const float rho = W.rb[h] + W.r[h];
@@ -756,8 +793,8 @@ __device__ __forceinline__ void ppmlr_pert(WinP<K>& W, const Tab& tx, const Para
           float s, cc, so, co, omso;
           rot_by2(W.ts[h], W.tc[h], W.rs[h], W.rcs[h], s, cc);
           rot_theta(W.sth[h], cth[h], W.oms[h], e, so, co, omso);
-          fdiff += 2.0f * (planet_fth_sph(pl, W.rad[h], W.sth[h], cth[h], W.oms[h], W.ts[h], W.tc[h]) +
-                           planet_fth_sph(pl, W.rad[h], so, co, omso, s, cc));
+          fdiff += 2.0f * (planet_fth_sph<BARY>(pl, W.rad[h], W.sth[h], cth[h], W.oms[h], W.ts[h], W.tc[h]) +
+                           planet_fth_sph<BARY>(pl, W.rad[h], so, co, omso, s, cc));
         }
         const float dp = W.pb[h] * emp[h] - pbm[h] * em[h];
         const float V0 = 0.5f * (szl[h] + szlp[h]);

@@ -258,6 +258,57 @@ def test_batch_with_damping_zones_equals_single_disks():
             assert li <= 1e-12, (w.name, f, l1, li)
 
 
+def test_batch_with_radial_boundary_codes_equals_single_disks():
+    """Radial boundary codes (set_boundx, boundary.cu:86-206): each disk's ghost rings are
+    copied from its interior between the batched pre-step and the radial sweep, so every disk
+    -- each with its own codes, one of them damped with a moving planet -- equals its own
+    single-context FP32 run."""
+    ws = []
+    for d, (nr, q, alpha, codes, extra) in enumerate([(48, 1e-4, 0.0, (0, 0), {}), (64, 3e-4, 3e-3, (4, 4), {}),
+                                                      (96, 1e-3, 0.0, (1, 2), dict(plnt_flag=2, kill_flag=1)),
+                                                      (130, 1e-4, 1e-3, (3, 3), {})]):
+        over = dict(visc_flag=1, ss_alpha=alpha, vis_nu=alpha * 0.05 * 0.05) if alpha > 0 else {}
+        w = configs.small(nr, 3 * nr, M_p=q, nlft=codes[0], nrgh=codes[1], **over, **extra)
+        w.seed = 500 + d
+        ws.append(w)
+    n = 30
+    say("case: batch with radial boundary codes,", len(ws), "disks,", n, "steps")
+    outs, sts = run_batch(ws, n)
+    for w, o, s in zip(ws, outs, sts):
+        ref, rs = run_single(w, abi.PN_FP32, n)
+        say(f"  {w.name} codes ({w.params.nlft}, {w.params.nrgh}): dt {s.dt!r} vs {rs.dt!r}")
+        assert s.dt == rs.dt and s.simtime == rs.simtime and s.FrRot == rs.FrRot
+        assert s.planet.x == rs.planet.x and s.planet.y == rs.planet.y
+        assert np.array_equal(o["rot_j"], ref["rot_j"])
+        for f in ("r", "u", "v"):
+            l1, li = rel(o[f], ref[f])
+            assert li <= 1e-12, (w.name, f, l1, li)
+
+
+def test_batch_in_the_barycentric_frame_equals_single_disks():
+    """The barycentric frame (bary_flag, forces.cu:28-40, 116-126) runs kernels of its own, so
+    the disks of a batch share it: every disk of a barycentric batch -- fixed and moving planets,
+    viscous or not -- equals its own single-context FP32 run."""
+    ws = []
+    for d, (nr, q, alpha, flag) in enumerate([(48, 1e-3, 0.0, 1), (64, 3e-4, 3e-3, 2), (96, 1e-4, 1e-3, 1)]):
+        over = dict(visc_flag=1, ss_alpha=alpha, vis_nu=alpha * 0.05 * 0.05) if alpha > 0 else {}
+        w = configs.small(nr, 3 * nr, M_p=q, plnt_flag=flag, bary_flag=1, **over)
+        w.seed = 600 + d
+        ws.append(w)
+    n = 30
+    say("case: batch in the barycentric frame,", len(ws), "disks,", n, "steps")
+    outs, sts = run_batch(ws, n)
+    for w, o, s in zip(ws, outs, sts):
+        ref, rs = run_single(w, abi.PN_FP32, n)
+        say(f"  {w.name} plnt_flag {w.params.plnt_flag}: dt {s.dt!r} vs {rs.dt!r}")
+        assert s.dt == rs.dt and s.simtime == rs.simtime and s.FrRot == rs.FrRot
+        assert s.planet.x == rs.planet.x and s.planet.y == rs.planet.y
+        assert np.array_equal(o["rot_j"], ref["rot_j"])
+        for f in ("r", "u", "v"):
+            l1, li = rel(o[f], ref[f])
+            assert li <= 1e-12, (w.name, f, l1, li)
+
+
 def test_batch_landing_matches_single_disks():
     """Picture times and end times differ per disk: the batch stops after the first step in
     which any disk lands, and every disk -- landed or not -- equals its own single-context

@@ -245,8 +245,24 @@ def damped(w):
     return w
 
 
-@pytest.mark.parametrize("w,n", [(disk3d(24, 72, 12), 20), (disk3d(64, 192, 32), 50), (damped(disk3d(64, 192, 32)), 50)],
-                         ids=["disk3d_24x72x12", "disk3d_64x192x32", "disk3d_64x192x32_damping"])
+def bary3d(w):
+    w.params = w.params.replace(bary_flag=1, M_p=1e-3)
+    w.name += "_bary"
+    return w
+
+
+def bounded(w, nlft, nrgh):
+    w.params = w.params.replace(nlft=nlft, nrgh=nrgh)
+    w.name += f"_bound{nlft}{nrgh}"
+    return w
+
+
+@pytest.mark.parametrize("w,n", [(disk3d(24, 72, 12), 20), (disk3d(64, 192, 32), 50), (damped(disk3d(64, 192, 32)), 50),
+                                 (bounded(disk3d(24, 72, 12), 1, 0), 30), (bounded(disk3d(64, 192, 32), 4, 2), 30),
+                                 (bary3d(disk3d(64, 192, 32)), 30)],
+                         ids=["disk3d_24x72x12", "disk3d_64x192x32", "disk3d_64x192x32_damping",
+                              "disk3d_24x72x12_outflow_reflect", "disk3d_64x192x32_mixed_periodic",
+                              "disk3d_64x192x32_bary"])
 def test_f32_3d_vs_fp64(w, n):
     say("case: fp32 3-D vs fp64", w.name, n, "steps")
     a, sa = run(w, abi.PN_FP32, n)
@@ -403,7 +419,21 @@ FLAG_CASES = [flags_case("_fargo2", fargo_flag=2), flags_case("_inertial", frrot
               # damping zones at both radial edges (the separate CFL kernel then gives dt)
               flags_case("_damping", kill_flag=1),
               # more than 2048 slots: Index_Shift, the table rows and the roll over the whole grid
-              configs.small(2048, 116, M_p=1e-3)]
+              configs.small(2048, 116, M_p=1e-3),
+              # radial boundary codes (set_boundx, boundary.cu:86-206): ghost rings copied from the
+              # interior before each radial sweep, carrying their source rings' backgrounds (draws
+              # where no ring's FARGO shift per step lies within the two paths' dt difference of a
+              # half cell, so that rot_j agrees; with the reflecting inner edge and M_p = 1e-4,
+              # ring 4 rounds its shift the other way near step 20)
+              flags_case("_reflect", nlft=0, nrgh=0, M_p=1e-3), flags_case("_outflow", nlft=1, nrgh=1),
+              flags_case("_periodic_r", nlft=2, nrgh=2), flags_case("_outflow_reflect", nlft=1, nrgh=0),
+              flags_case("_mixed_viscous", nlft=4, nrgh=4, M_p=1e-3, visc_flag=1, ss_alpha=1e-2, vis_nu=2.5e-5),
+              flags_case("_outflow_damping_moving_planet", nlft=1, nrgh=1, kill_flag=1, plnt_flag=2, M_p=1e-3),
+              # the barycentric frame (bary_flag, forces.cu:28-40, 116-126): the star's offset pull
+              # as a deviation force beside the planet's, no indirect term
+              flags_case("_bary", bary_flag=1, M_p=1e-3), flags_case("_bary_weak_planet", bary_flag=1),
+              flags_case("_bary_moving_planet", bary_flag=1, plnt_flag=2, M_p=1e-3),
+              flags_case("_bary_inertial", bary_flag=1, frrot_flag=0, M_p=1e-3)]
 
 
 @pytest.mark.parametrize("w", FLAG_CASES, ids=lambda w: w.name)
@@ -435,7 +465,10 @@ EOS3_CASES = [flags_case("_eos3", eos=3, gam=1.4, init_p_iso=1),
               flags_case("_eos3_inertial", eos=3, gam=1.4, init_p_iso=1, frrot_flag=0),
               flags_case("_eos3_noflat", eos=3, gam=1.4, init_p_iso=1, flat_flag=0),
               flags_case("_eos3_moving_planet", eos=3, gam=1.4, init_p_iso=1, plnt_flag=2, M_p=1e-3),
-              flags_case("_eos3_damping", eos=3, gam=1.4, init_p_iso=1, kill_flag=1)]
+              flags_case("_eos3_damping", eos=3, gam=1.4, init_p_iso=1, kill_flag=1),
+              flags_case("_eos3_reflect", eos=3, gam=1.4, init_p_iso=1, nlft=0, nrgh=0),
+              flags_case("_eos3_mixed", eos=3, gam=1.4, init_p_iso=1, nlft=4, nrgh=4),
+              flags_case("_eos3_bary", eos=3, gam=1.4, init_p_iso=1, bary_flag=1, M_p=1e-3)]
 
 
 @pytest.mark.parametrize("w", EOS3_CASES, ids=lambda w: w.name)

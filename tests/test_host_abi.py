@@ -154,6 +154,10 @@ def test_batch_argument_errors_before_any_device_call():
     with pytest.raises(abi.PnError) as e:
         abi.Batch([], [])
     assert e.value.code == abi.PN_ERR_ARG
+    # the disks of a batch share the frame
+    with pytest.raises(abi.PnError) as e:
+        abi.Batch([PnGrid(32, 48, 1)] * 2, [configs.disk2d(32, 48), configs.disk2d(32, 48, bary_flag=1)])
+    assert e.value.code == abi.PN_ERR_UNSUPPORTED
     # batches hold 2-D disks
     p3 = configs.disk2d(24, 72).replace(ndim=3, ngeomx=2, ngeomy=4, ngeomz=5, zmin=1.4, zmax=1.7)
     with pytest.raises(abi.PnError) as e:
@@ -170,11 +174,11 @@ def test_fp32_parameter_errors_before_any_device_call():
     with pytest.raises(abi.PnError) as e:
         abi.Batch([PnGrid(32, 48, 1)], [configs.disk2d(32, 48, eos=3, gam=1.4)])
     assert e.value.code == abi.PN_ERR_UNSUPPORTED
-    # the FP32 path keeps fixed radial ghost rings and a star at the origin
-    for over in (dict(nlft=1), dict(bary_flag=1)):
+    # parameter values outside their ranges are argument errors on either path
+    for over in (dict(bary_flag=2), dict(nlft=5), dict(nrgh=-1)):
         with pytest.raises(abi.PnError) as e:
             abi.Context(PnGrid(32, 48, 1), configs.disk2d(32, 48, **over), abi.PN_FP32)
-        assert e.value.code == abi.PN_ERR_UNSUPPORTED
+        assert e.value.code == abi.PN_ERR_ARG
 
 
 def test_eos3_is_two_dimensional():
