@@ -71,7 +71,8 @@ typedef struct pn_params {
   int32_t ndim;            /* 2 or 3 (variable_types.h:11) */
   int32_t eos;             /* 0 isothermal, 1 isentropic, 2 adiabatic (global.h:18); 3 adiabatic
                               with the energy-consistent Lagrangian step that evolve.cu:146 leaves
-                              commented out (2-D; not in the reference's live code) */
+                              commented out (2-D; not in the reference's live code).  PN_FP32
+                              contexts take 0 and 3 */
   int32_t flat_flag;       /* 1: PPM flattening at shocks (global.h:16) */
   int32_t plnt_flag;       /* 0 none, 1 fixed, 2 moving planet (global.h:20) */
   int32_t frrot_flag;      /* 1: frame co-rotates with the planet (global.h:21) */

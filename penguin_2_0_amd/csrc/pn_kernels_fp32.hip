@@ -895,7 +895,9 @@ hipError_t sweep_x_k(const LaunchCtx& c, const DevF32& f0, int src) {
   // others run without them
   auto go = [&](auto bary) {
     constexpr bool B = decltype(bary)::value;
-    if (c.p.eos == 3) k_sweep_x_f32<K, W, Q, 0, true, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+    // EOS 3: a viscous disk runs its own instantiation (the viscosity updates velocities only)
+    if (c.p.eos == 3 && c.p.visc) k_sweep_x_f32<K, W, Q, 1, true, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
+    else if (c.p.eos == 3) k_sweep_x_f32<K, W, Q, 0, true, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
     else k_sweep_x_f32<K, W, Q, 2, false, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, src);
   };
   if (c.p.bary) go(std::true_type{});
@@ -917,6 +919,8 @@ hipError_t sweep_y_k(const LaunchCtx& c, const DevF32& f0, int src, int do_cfl) 
       k_sweep_y_f32<K, WPB, 1, 3, false, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, 0);
     else if (c.g.ndim == 3)
       k_sweep_y_f32<K, WPB, 0, 3, false, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, 0);
+    else if (c.p.eos == 3 && c.p.visc)
+      k_sweep_y_f32<K, WPB, 1, 2, true, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
     else if (c.p.eos == 3)
       k_sweep_y_f32<K, WPB, 0, 2, true, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);
     else k_sweep_y_f32<K, WPB, 2, 2, false, B><<<grid, block, 0, c.stream>>>(c.g, f, c.p, c.ss, c.cflp, src, do_cfl);

@@ -113,8 +113,8 @@ int check_params(pn_ctx* c, const pn_grid* g, const pn_params* p, int precision)
   for (int code : {p->nlft, p->nrgh})
     if ((code == 0 || code == 2) && g->imax < kNpad + 1)
       return fail(c, PN_ERR_ARG, "radial boundary code %d needs imax > %d", code, kNpad);
-  if (precision == PN_FP32 && p->eos != 0 && !(p->eos == 3 && p->ndim == 2 && !p->visc_flag))
-    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for EOS 0 and the inviscid 2-D EOS 3 disk");
+  if (precision == PN_FP32 && p->eos != 0 && !(p->eos == 3 && p->ndim == 2))
+    return fail(c, PN_ERR_UNSUPPORTED, "the FP32 path is built for EOS 0 and the 2-D EOS 3 disk");
   if (p->bary_flag != 0 && p->bary_flag != 1) return fail(c, PN_ERR_ARG, "bary_flag %d", p->bary_flag);
   return PN_OK;
 }

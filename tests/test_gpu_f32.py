@@ -468,7 +468,11 @@ EOS3_CASES = [flags_case("_eos3", eos=3, gam=1.4, init_p_iso=1),
               flags_case("_eos3_damping", eos=3, gam=1.4, init_p_iso=1, kill_flag=1),
               flags_case("_eos3_reflect", eos=3, gam=1.4, init_p_iso=1, nlft=0, nrgh=0),
               flags_case("_eos3_mixed", eos=3, gam=1.4, init_p_iso=1, nlft=4, nrgh=4),
-              flags_case("_eos3_bary", eos=3, gam=1.4, init_p_iso=1, bary_flag=1, M_p=1e-3)]
+              flags_case("_eos3_bary", eos=3, gam=1.4, init_p_iso=1, bary_flag=1, M_p=1e-3),
+              # viscosity (velocities only, as on the FP64 path) with either initial pressure
+              flags_case("_eos3_viscous", eos=3, gam=1.4, init_p_iso=1, visc_flag=1, ss_alpha=1e-2, vis_nu=2.5e-5,
+                         M_p=1e-3),
+              flags_case("_eos3_viscous_adiabatic_ic", eos=3, gam=1.4, visc_flag=1, ss_alpha=3e-3, vis_nu=7.5e-6)]
 
 
 @pytest.mark.parametrize("w", EOS3_CASES, ids=lambda w: w.name)

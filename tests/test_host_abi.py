@@ -166,8 +166,8 @@ def test_batch_argument_errors_before_any_device_call():
 
 
 def test_fp32_parameter_errors_before_any_device_call():
-    # the FP32 path is built for the isothermal closure and the inviscid 2-D EOS 3 disk
-    for over in (dict(eos=1), dict(eos=2), dict(eos=3, visc_flag=1)):
+    # the FP32 path is built for the isothermal closure and the 2-D EOS 3 disk
+    for over in (dict(eos=1), dict(eos=2)):
         with pytest.raises(abi.PnError) as e:
             abi.Context(PnGrid(32, 48, 1), configs.disk2d(32, 48, gam=1.4, **over), abi.PN_FP32)
         assert e.value.code == abi.PN_ERR_UNSUPPORTED
