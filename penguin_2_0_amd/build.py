@@ -5,6 +5,7 @@
 
 Outputs:
     penguin_2_0_amd/libpenguin.so   the product: C ABI of include/penguin.h
+    penguin_2_0_amd/penguin_main    the reference's run loop as a C++ caller of that ABI
     oracle/libpn_oracle.so          the FP64 CPU restatement used by the tests as checker
 """
 from __future__ import annotations
@@ -22,6 +23,8 @@ PKG = ROOT / "penguin_2_0_amd"
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libpenguin.so"
+DRIVER_SRC = PKG / "host" / "penguin_main.cpp"
+DRIVER = PKG / "penguin_main"
 ORACLE_SRC = ROOT / "oracle" / "pn_oracle.c"
 ORACLE_LIB = ROOT / "oracle" / "libpn_oracle.so"
 
@@ -79,6 +82,16 @@ def build_lib(jobs: int = 8) -> Path:
     return LIB
 
 
+def build_driver() -> Path:
+    """The C++ run loop over the C ABI, linked against the in-tree libpenguin.so."""
+    cxx = shutil.which("g++") or "c++"
+    tmp = DRIVER.with_suffix(".tmp")
+    _run([cxx, "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", "-o", str(tmp), str(DRIVER_SRC),
+          f"-L{PKG}", "-lpenguin", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 def build_oracle() -> Path:
     cc = shutil.which("gcc") or "cc"
     tmp = ORACLE_LIB.with_suffix(".so.tmp")
@@ -96,6 +109,7 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     if not a.oracle:
         print("built", build_lib(a.j))
+        print("built", build_driver())
     if not a.lib:
         print("built", build_oracle())
     return 0

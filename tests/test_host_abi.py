@@ -276,3 +276,18 @@ def test_para_and_initial_files_read_back_as_the_reader_reads_them(tmp_path):
     assert rows.shape == (n, 2)
     np.testing.assert_allclose(rows[:, 0], xa, rtol=5e-6)
     np.testing.assert_allclose(rows[:, 1], dx, rtol=5e-6)
+
+
+def test_run_loop_driver_parses_its_arguments_without_a_device():
+    """penguin_main (the reference's run loop over the C ABI) is built beside the library; its
+    argument errors end before any device call."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parent.parent / "penguin_2_0_amd" / "penguin_main"
+    assert exe.exists()
+    res = subprocess.run([str(exe), "--help"], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0 and "usage: penguin_main" in res.stdout
+    for bad in ([], ["--out", "x", "--set", "no_such_field=1"], ["--out", "x", "--set", "plnt_flag=1.5"],
+                ["--out", "x", "--nprin", "0"], ["--bogus"]):
+        res = subprocess.run([str(exe), *bad], capture_output=True, text=True, timeout=60)
+        assert res.returncode == 2, (bad, res.stdout, res.stderr)
