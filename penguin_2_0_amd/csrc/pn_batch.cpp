@@ -30,8 +30,10 @@ struct pn_batch {
   int grid_x = 0, grid_y = 0;
   DiskDev* dd = nullptr;      // device, one record per disk
   DevF32* dfs = nullptr;      // device
-  int* base_x = nullptr;      // device, ndisk + 1 item offsets
-  int* base_y = nullptr;
+  int* order_x = nullptr;     // device, (disk, item) per slot of the radial launch (build_order_x)
+  size_t order_cap = 0;       // ints order_x holds
+  int nslot_x = 0;
+  int* base_y = nullptr;     // device, ndisk + 1 azimuthal item offsets
   hipGraphExec_t gexec = nullptr;
   int gexec_cur = 0;          // the buffer the captured step reads (it returns there)
 };
@@ -69,12 +71,66 @@ bool cfl_before_step(const pn_ctx* c) { return c->P.plnt_flag == 2 || c->P.kill_
 // the batch's disks run in the barycentric frame (all or none): the kernels with its star terms
 bool any_bary(const pn_batch* b) { return b->disks[0]->P.bary_flag != 0; }
 
+// The radial launch's items in the order its workgroups take them.  The persistent kernel's
+// workgroup b runs slots b, b + G, ... (G a multiple of 8) on XCD b % 8, so slot t runs on
+// XCD t % 8.  Each XCD gets a contiguous run of the disks' items (neighbouring column groups
+// share L2 lines) of an equal share of the estimated cost, laid out by decreasing cost (a
+// stable sort, so equal items keep their neighbours): every XCD's last, partial round then
+// holds its cheapest items -- last windows short enough to run two columns per wavefront pass
+// -- instead of a random mix, whose most expensive workgroup would set the launch's length.
+// Cost of a window item: 0.3 for the staging (scaled by the window's cells) and 0.7 for the
+// pencil passes (halved when the window runs two columns per pass, a third for three).
+int build_order_x(pn_batch* b, std::vector<int>& order, int& nslots) {
+  const int K = kF32VariantsX[b->var_x][0];
+  const int nc = kF32VariantsX[b->var_x][1] * kF32VariantsX[b->var_x][2];
+  const int real = 64 * K - 2 * kNpad;
+  struct Item { int d, local; double cost; };
+  std::vector<Item> items;
+  double total = 0.0;
+  for (int d = 0; d < ndisk(b); ++d) {
+    const pn_ctx* c = b->disks[d];
+    const int nwin = f32_windows(c->g.imax, K), ngrp = (c->g.jmax + nc - 1) / nc;
+    for (int l = 0; l < nwin; ++l) {
+      const int lim = (l < nwin - 1) ? 64 * K : c->g.imax - real * (nwin - 1) + 2 * kNpad;
+      double passes = 1.0;
+      if (c->df.pack_tails && K == 4 && lim <= 21 * K) passes = 1.0 / 3.0;
+      else if (c->df.pack_tails && lim <= 32 * K) passes = 0.5;
+      const double cost = 0.3 * lim / (64.0 * K) + 0.7 * passes;
+      for (int g = 0; g < ngrp; ++g) {
+        items.push_back({d, l * ngrp + g, cost});
+        total += cost;
+      }
+    }
+  }
+  std::vector<std::vector<Item>> share(8);
+  double cum = 0.0;
+  for (const Item& it : items) {
+    const int x = std::min(7, static_cast<int>(8.0 * (cum + 0.5 * it.cost) / total));
+    share[x].push_back(it);
+    cum += it.cost;
+  }
+  size_t len = 0;
+  for (auto& s : share) {
+    std::stable_sort(s.begin(), s.end(), [](const Item& a, const Item& c) { return a.cost > c.cost; });
+    len = std::max(len, s.size());
+  }
+  nslots = static_cast<int>(8 * len);
+  order.assign(2 * static_cast<size_t>(nslots), -1);
+  for (int x = 0; x < 8; ++x)
+    for (size_t i = 0; i < share[x].size(); ++i) {
+      const size_t t = x + 8 * i;
+      order[2 * t] = share[x][i].d;
+      order[2 * t + 1] = share[x][i].local;
+    }
+  return PN_OK;
+}
+
 // the per-disk records and item offsets for the current variants
 int upload_desc(pn_batch* b) {
   const int n = ndisk(b);
   std::vector<DiskDev> dd(n);
   std::vector<DevF32> df(n);
-  std::vector<int> bx(n + 1, 0), by(n + 1, 0);
+  std::vector<int> by(n + 1, 0);
   for (int d = 0; d < n; ++d) {
     pn_ctx* c = b->disks[d];
     if (!c->loaded) return bfail(b, PN_ERR_STATE, "disk %d has no state loaded", d);
@@ -100,17 +156,27 @@ int upload_desc(pn_batch* b) {
     df[d].nwin_y = f32_windows(c->g.jmax, kF32VariantsY[b->var_y][0]);
     if (c->df.tstride < c->dg.sr * c->dg.sk + 64 * kF32VariantsX[b->var_x][0])
       return bfail(b, PN_ERR_ARG, "disk %d: table overhang too short", d);
-    bx[d + 1] = bx[d] + f32_items_x(c->dg, b->var_x);
     by[d + 1] = by[d] + ny;
   }
   PB_HIP(b, hipMemcpyAsync(b->dd, dd.data(), n * sizeof(DiskDev), hipMemcpyHostToDevice, b->stream));
   PB_HIP(b, hipMemcpyAsync(b->dfs, df.data(), n * sizeof(DevF32), hipMemcpyHostToDevice, b->stream));
-  PB_HIP(b, hipMemcpyAsync(b->base_x, bx.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, b->stream));
   PB_HIP(b, hipMemcpyAsync(b->base_y, by.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice, b->stream));
+  std::vector<int> order;
+  build_order_x(b, order, b->nslot_x);
+  if (order.size() > b->order_cap) {   // a captured step names the old array: recapture
+    drop_graph(b);
+    PB_HIP(b, hipStreamSynchronize(b->stream));
+    if (b->order_x) PB_HIP(b, hipFree(b->order_x));
+    b->order_x = nullptr;
+    b->order_cap = 0;
+    PB_HIP(b, hipMalloc(reinterpret_cast<void**>(&b->order_x), order.size() * sizeof(int)));
+    b->order_cap = order.size();
+  }
+  PB_HIP(b, hipMemcpyAsync(b->order_x, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
   PB_HIP(b, hipStreamSynchronize(b->stream));
   // persistent grids: what is resident at once, or fewer when there are fewer items
   auto fit = [](int resident, int items) { return std::max(8, std::min(resident, (items + 7) / 8 * 8)); };
-  b->grid_x = fit(f32_batch_grid(0, b->var_x, any_bary(b)), bx[n]);
+  b->grid_x = fit(f32_batch_grid(0, b->var_x, any_bary(b)), b->nslot_x);
   b->grid_y = fit(f32_batch_grid(1, b->var_y, any_bary(b)), by[n]);
   if (b->grid_x < 8 || b->grid_y < 8) return bfail(b, PN_ERR_HIP, "occupancy query failed");
   return PN_OK;
@@ -142,7 +208,7 @@ int launch_batch_step(pn_batch* b, int cur) {
     if (e != hipSuccess) break;
     e = f32_fill_ghosts(on_batch(b, c), c->df, cur);
   }
-  if (e == hipSuccess) e = f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x, any_bary(b));
+  if (e == hipSuccess) e = f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->order_x, b->nslot_x, b->var_x, cur, b->grid_x, any_bary(b));
   if (e == hipSuccess) e = f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur ^ 1, b->grid_y, any_bary(b));
   for (pn_ctx* c : b->disks) {
     if (!c->P.kill_flag || e != hipSuccess) continue;
@@ -261,7 +327,6 @@ int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int pr
   PB_HIP(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   PB_HIP(b, hipMalloc(reinterpret_cast<void**>(&b->dd), n * sizeof(DiskDev)));
   PB_HIP(b, hipMalloc(reinterpret_cast<void**>(&b->dfs), n * sizeof(DevF32)));
-  PB_HIP(b, hipMalloc(reinterpret_cast<void**>(&b->base_x), (n + 1) * sizeof(int)));
   PB_HIP(b, hipMalloc(reinterpret_cast<void**>(&b->base_y), (n + 1) * sizeof(int)));
   return PN_OK;
 }
@@ -273,7 +338,7 @@ int pn_batch_destroy(pn_batch* b) {
   for (pn_ctx* c : b->disks) pn_destroy(c);
   if (b->dd) hipFree(b->dd);
   if (b->dfs) hipFree(b->dfs);
-  if (b->base_x) hipFree(b->base_x);
+  if (b->order_x) hipFree(b->order_x);
   if (b->base_y) hipFree(b->base_y);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
@@ -343,7 +408,7 @@ int pn_batch_tune(pn_batch* b, int nrep, int* var_x, int* var_y) {
     double tx = 0.0, ty = 0.0;
     const int n = ndisk(b);
     rc = btime(b, nrep, &tx, [&]() {
-      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x, any_bary(b));
+      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->order_x, b->nslot_x, b->var_x, cur, b->grid_x, any_bary(b));
     });
     const bool y_ok = f32_y_variant_allowed(v, any_viscous(b));
     if (rc == PN_OK && y_ok)
@@ -452,7 +517,7 @@ int pn_batch_time_kernels(pn_batch* b, int nrep, double* ms4) {
   rc = btime(b, nrep, &ms4[0], [&]() { return launch_pre_step_batch(b->stream, b->dd, n); });
   if (rc == PN_OK)
     rc = btime(b, nrep, &ms4[1], [&]() {
-      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->base_x, n, b->var_x, cur, b->grid_x, any_bary(b));
+      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->order_x, b->nslot_x, b->var_x, cur, b->grid_x, any_bary(b));
     });
   if (rc == PN_OK)
     rc = btime(b, nrep, &ms4[2], [&]() {

@@ -118,12 +118,13 @@ int f32_cfl_partials_z(const DevGrid& g, int variant);
 
 // batches of disks: one persistent launch per sweep over every disk's work items.
 // items: workgroup-sized work items of one disk (radial: window x column group; azimuthal:
-// the y partial count).  base: ndisk + 1 item offsets in device memory; dd / dfs: per-disk
-// records in device memory.  grid: f32_batch_grid(axis, variant, bary).  bary: the disks run
-// in the barycentric frame (the kernels with its star terms; the others lack them).
-int f32_items_x(const DevGrid& g, int variant);
+// the y partial count).  Radial: order holds nslots (disk, item) pairs (nslots a multiple of
+// 8; disk -1 for an empty slot), slot t taken by the workgroup that runs it.  Azimuthal:
+// base holds ndisk + 1 item offsets.  Both in device memory, as dd / dfs (per-disk records).
+// grid: f32_batch_grid(axis, variant, bary).  bary: the disks run in the barycentric frame
+// (the kernels with its star terms; the others lack them).
 int f32_batch_grid(int axis, int variant, bool bary);
-hipError_t f32_sweep_x_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
+hipError_t f32_sweep_x_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* order, int nslots,
                              int variant, int src, int grid, bool bary);
 hipError_t f32_sweep_y_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
                              int variant, int src, int grid, bool bary);

@@ -457,18 +457,20 @@ constexpr int batch_pencil_waves() {
   return W == 4 ? 4 : 1;
 }
 
+// order: nslots (disk, item) pairs, the item of slot t for the workgroup that takes it
+// (slot t runs on XCD t % 8; the host lays each XCD's share out by decreasing cost, disk -1
+// marks an empty slot)
 template <int K, int W, int Q, bool BARY>
 __global__ __launch_bounds__(64 * W) PN_WAVES(batch_pencil_waves<W>()) void k_sweep_x_f32_batch(const DiskDev* __restrict__ dd,
                                                               const DevF32* __restrict__ dfs,
-                                                              const int* __restrict__ base, int ndisk, int src) {
+                                                              const int* __restrict__ order, int nslots, int src) {
   __shared__ XTile<K, W, Q, 3> T;
-  const int nitems = base[ndisk];
-  for (int t = blockIdx.x; t < nitems; t += gridDim.x) {
-    const int it = xcd_swizzle(t, nitems);
-    const int d = disk_of(base, ndisk, it);
+  for (int t = blockIdx.x; t < nslots; t += gridDim.x) {
+    const int d = order[2 * t];
+    if (d < 0) continue;   // workgroup-uniform
     const DevGrid g = dd[d].g;
     const int ngrp = (g.jmax + W * Q - 1) / (W * Q);
-    const int local = it - base[d];
+    const int local = order[2 * t + 1];
     pencil_item<K, W, Q, kGeoRcyl, 2, false, false, BARY>(g, dfs[d], dd[d].p, dd[d].ss, src, local % ngrp, local / ngrp,
                                                          0, T);
   }
@@ -1042,11 +1044,11 @@ int persistent_grid(Kern k, int block) {
 }
 
 template <int K, int W, int Q>
-hipError_t sweep_x_batch_k(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
+hipError_t sweep_x_batch_k(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* order, int nslots,
                            int src, int grid, bool bary) {
-  if (grid < 8 || grid % 8) return hipErrorInvalidValue;
-  if (bary) k_sweep_x_f32_batch<K, W, Q, true><<<grid, 64 * W, 0, s>>>(dd, dfs, base, ndisk, src);
-  else k_sweep_x_f32_batch<K, W, Q, false><<<grid, 64 * W, 0, s>>>(dd, dfs, base, ndisk, src);
+  if (grid < 8 || grid % 8 || nslots % 8) return hipErrorInvalidValue;
+  if (bary) k_sweep_x_f32_batch<K, W, Q, true><<<grid, 64 * W, 0, s>>>(dd, dfs, order, nslots, src);
+  else k_sweep_x_f32_batch<K, W, Q, false><<<grid, 64 * W, 0, s>>>(dd, dfs, order, nslots, src);
   return hipGetLastError();
 }
 
@@ -1087,18 +1089,13 @@ int f32_batch_grid(int axis, int variant, bool bary) {
   return bary ? batch_grid<true>(axis, variant) : batch_grid<false>(axis, variant);
 }
 
-int f32_items_x(const DevGrid& g, int variant) {
-  const int nc = kF32VariantsX[variant][1] * kF32VariantsX[variant][2];
-  return f32_windows(g.imax, kF32VariantsX[variant][0]) * ((g.jmax + nc - 1) / nc);
-}
-
-hipError_t f32_sweep_x_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
+hipError_t f32_sweep_x_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* order, int nslots,
                              int variant, int src, int grid, bool bary) {
   switch (variant) {
-    case 0: return sweep_x_batch_k<2, 4, 2>(s, dd, dfs, base, ndisk, src, grid, bary);
-    case 1: return sweep_x_batch_k<2, 8, 1>(s, dd, dfs, base, ndisk, src, grid, bary);
-    case 2: return sweep_x_batch_k<2, 16, 1>(s, dd, dfs, base, ndisk, src, grid, bary);
-    case 3: return sweep_x_batch_k<2, 4, 4>(s, dd, dfs, base, ndisk, src, grid, bary);
+    case 0: return sweep_x_batch_k<2, 4, 2>(s, dd, dfs, order, nslots, src, grid, bary);
+    case 1: return sweep_x_batch_k<2, 8, 1>(s, dd, dfs, order, nslots, src, grid, bary);
+    case 2: return sweep_x_batch_k<2, 16, 1>(s, dd, dfs, order, nslots, src, grid, bary);
+    case 3: return sweep_x_batch_k<2, 4, 4>(s, dd, dfs, order, nslots, src, grid, bary);
   }
   return hipErrorInvalidValue;
 }
