@@ -105,7 +105,7 @@ int build_order_x(pn_batch* b, std::vector<int>& order, int& nslots) {
   std::vector<std::vector<Item>> share(8);
   double cum = 0.0;
   for (const Item& it : items) {
-    const int x = std::min(7, static_cast<int>(8.0 * (cum + 0.5 * it.cost) / total));
+    const int x = total > 0.0 ? std::min(7, static_cast<int>(8.0 * (cum + 0.5 * it.cost) / total)) : 0;
     share[x].push_back(it);
     cum += it.cost;
   }
