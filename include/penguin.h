@@ -315,11 +315,13 @@ int pn_totals_noalloc(pn_ctx* ctx, double* mass, double* angmom);
  * operations of main.cu:343-387 exactly as pn_run performs them. */
 typedef struct pn_batch pn_batch;
 
-/* n disks of grids[d] / params[d].  precision must be PN_FP32; every disk 2-D isothermal,
- * viscous or not, with or without damping zones; its planet fixed, growing or moving under
- * the disk's forces (plnt_flag 0..2).  A disk with a moving planet or damping zones takes
- * its CFL pass before each step, as pn_run does.  On failure *out still holds a batch whose pn_batch_last_error explains
- * it; pn_batch_destroy it. */
+/* n disks of grids[d] / params[d].  precision must be PN_FP32; every disk 2-D, isothermal
+ * (EOS 0) or with the energy-consistent adiabatic step (EOS 3), viscous or not, with or without
+ * damping zones, with its own radial boundary codes; its planet fixed, growing or moving under
+ * the disk's forces (plnt_flag 0..2).  The disks share the closure and the frame (eos 3 or not,
+ * bary_flag), which select the kernels of a launch (PN_ERR_UNSUPPORTED otherwise).  A disk with
+ * a moving planet or damping zones takes its CFL pass before each step, as pn_run does.  On
+ * failure *out still holds a batch whose pn_batch_last_error explains it; pn_batch_destroy it. */
 int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int precision, pn_batch** out);
 int pn_batch_destroy(pn_batch* b);
 const char* pn_batch_last_error(const pn_batch* b);

@@ -70,6 +70,8 @@ bool cfl_before_step(const pn_ctx* c) { return c->P.plnt_flag == 2 || c->P.kill_
 
 // the batch's disks run in the barycentric frame (all or none): the kernels with its star terms
 bool any_bary(const pn_batch* b) { return b->disks[0]->P.bary_flag != 0; }
+// the batch's disks carry EOS 3 (all or none): the kernels with the fourth field
+bool any_adia(const pn_batch* b) { return b->disks[0]->P.eos == 3; }
 
 // The radial launch's items in the order its workgroups take them.  The persistent kernel's
 // workgroup b runs slots b, b + G, ... (G a multiple of 8) on XCD b % 8, so slot t runs on
@@ -176,8 +178,8 @@ int upload_desc(pn_batch* b) {
   PB_HIP(b, hipStreamSynchronize(b->stream));
   // persistent grids: what is resident at once, or fewer when there are fewer items
   auto fit = [](int resident, int items) { return std::max(8, std::min(resident, (items + 7) / 8 * 8)); };
-  b->grid_x = fit(f32_batch_grid(0, b->var_x, any_bary(b)), b->nslot_x);
-  b->grid_y = fit(f32_batch_grid(1, b->var_y, any_bary(b)), by[n]);
+  b->grid_x = fit(f32_batch_grid(0, b->var_x, any_bary(b), any_adia(b)), b->nslot_x);
+  b->grid_y = fit(f32_batch_grid(1, b->var_y, any_bary(b), any_adia(b)), by[n]);
   if (b->grid_x < 8 || b->grid_y < 8) return bfail(b, PN_ERR_HIP, "occupancy query failed");
   return PN_OK;
 }
@@ -208,8 +210,8 @@ int launch_batch_step(pn_batch* b, int cur) {
     if (e != hipSuccess) break;
     e = f32_fill_ghosts(on_batch(b, c), c->df, cur);
   }
-  if (e == hipSuccess) e = f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->order_x, b->nslot_x, b->var_x, cur, b->grid_x, any_bary(b));
-  if (e == hipSuccess) e = f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur ^ 1, b->grid_y, any_bary(b));
+  if (e == hipSuccess) e = f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->order_x, b->nslot_x, b->var_x, cur, b->grid_x, any_bary(b), any_adia(b));
+  if (e == hipSuccess) e = f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur ^ 1, b->grid_y, any_bary(b), any_adia(b));
   for (pn_ctx* c : b->disks) {
     if (!c->P.kill_flag || e != hipSuccess) continue;
     e = f32_damp(on_batch(b, c), c->df, cur);
@@ -309,15 +311,19 @@ int pn_batch_create(int n, const pn_grid* grids, const pn_params* params, int pr
   if (precision != PN_FP32) return bfail(b, PN_ERR_UNSUPPORTED, "the batched launch is built for the FP32 path");
   // the barycentric frame's star terms are compiled into kernels of their own; one launch runs
   // one kind, so the disks of a batch share the frame (a compile-time choice of the reference,
-  // forces.cu:28)
-  for (int d = 1; d < n; ++d)
+  // forces.cu:28), and the closure (global.h:18)
+  for (int d = 1; d < n; ++d) {
     if ((params[d].bary_flag != 0) != (params[0].bary_flag != 0))
       return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: the disks of a batch share the frame (bary_flag)", d);
+    // likewise the closure: EOS 3 disks carry a fourth field through kernels of their own
+    if ((params[d].eos == 3) != (params[0].eos == 3))
+      return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: the disks of a batch share the closure (eos)", d);
+  }
   for (int d = 0; d < n; ++d) {
     if (params[d].ndim != 2)
       return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches are built for 2-D disks", d);
-    if (params[d].eos != 0)
-      return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches are built for the isothermal closure (EOS 0)", d);
+    if (params[d].eos != 0 && params[d].eos != 3)
+      return bfail(b, PN_ERR_UNSUPPORTED, "disk %d: batches are built for EOS 0 and EOS 3", d);
     pn_ctx* c = nullptr;
     const int rc = pn_create(&grids[d], &params[d], precision, &c);
     if (c) b->disks.push_back(c);
@@ -408,12 +414,12 @@ int pn_batch_tune(pn_batch* b, int nrep, int* var_x, int* var_y) {
     double tx = 0.0, ty = 0.0;
     const int n = ndisk(b);
     rc = btime(b, nrep, &tx, [&]() {
-      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->order_x, b->nslot_x, b->var_x, cur, b->grid_x, any_bary(b));
+      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->order_x, b->nslot_x, b->var_x, cur, b->grid_x, any_bary(b), any_adia(b));
     });
     const bool y_ok = f32_y_variant_allowed(v, any_viscous(b));
     if (rc == PN_OK && y_ok)
       rc = btime(b, nrep, &ty, [&]() {
-        return f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur, b->grid_y, any_bary(b));
+        return f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur, b->grid_y, any_bary(b), any_adia(b));
       });
     if (rc != PN_OK) return rc;
     if (tx < best_x) { best_x = tx; bx = v; }
@@ -517,11 +523,11 @@ int pn_batch_time_kernels(pn_batch* b, int nrep, double* ms4) {
   rc = btime(b, nrep, &ms4[0], [&]() { return launch_pre_step_batch(b->stream, b->dd, n); });
   if (rc == PN_OK)
     rc = btime(b, nrep, &ms4[1], [&]() {
-      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->order_x, b->nslot_x, b->var_x, cur, b->grid_x, any_bary(b));
+      return f32_sweep_x_batch(b->stream, b->dd, b->dfs, b->order_x, b->nslot_x, b->var_x, cur, b->grid_x, any_bary(b), any_adia(b));
     });
   if (rc == PN_OK)
     rc = btime(b, nrep, &ms4[2], [&]() {
-      return f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur, b->grid_y, any_bary(b));
+      return f32_sweep_y_batch(b->stream, b->dd, b->dfs, b->base_y, n, b->var_y, cur, b->grid_y, any_bary(b), any_adia(b));
     });
   if (rc == PN_OK) rc = btime(b, nrep, &ms4[3], [&]() { return launch_post_step_batch(b->stream, b->dd, n); });
   return rc;

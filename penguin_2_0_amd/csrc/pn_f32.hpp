@@ -121,13 +121,14 @@ int f32_cfl_partials_z(const DevGrid& g, int variant);
 // the y partial count).  Radial: order holds nslots (disk, item) pairs (nslots a multiple of
 // 8; disk -1 for an empty slot), slot t taken by the workgroup that runs it.  Azimuthal:
 // base holds ndisk + 1 item offsets.  Both in device memory, as dd / dfs (per-disk records).
-// grid: f32_batch_grid(axis, variant, bary).  bary: the disks run in the barycentric frame
-// (the kernels with its star terms; the others lack them).
-int f32_batch_grid(int axis, int variant, bool bary);
+// grid: f32_batch_grid(axis, variant, bary, adia).  bary: the disks run in the barycentric
+// frame (the kernels with its star terms; the others lack them); adia: the disks carry EOS 3
+// (the kernels with the fourth field).
+int f32_batch_grid(int axis, int variant, bool bary, bool adia);
 hipError_t f32_sweep_x_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* order, int nslots,
-                             int variant, int src, int grid, bool bary);
+                             int variant, int src, int grid, bool bary, bool adia);
 hipError_t f32_sweep_y_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
-                             int variant, int src, int grid, bool bary);
+                             int variant, int src, int grid, bool bary, bool adia);
 // CFL maxima per block into c.cflp; returns the partial count through *n
 hipError_t f32_cfl(const LaunchCtx& c, const DevF32& f, int src, int* n);
 // CFL partials f32_cfl writes for this grid

@@ -460,19 +460,19 @@ constexpr int batch_pencil_waves() {
 // order: nslots (disk, item) pairs, the item of slot t for the workgroup that takes it
 // (slot t runs on XCD t % 8; the host lays each XCD's share out by decreasing cost, disk -1
 // marks an empty slot)
-template <int K, int W, int Q, bool BARY>
-__global__ __launch_bounds__(64 * W) PN_WAVES(batch_pencil_waves<W>()) void k_sweep_x_f32_batch(const DiskDev* __restrict__ dd,
+template <int K, int W, int Q, bool BARY, bool ADIA>
+__global__ __launch_bounds__(64 * W) PN_WAVES(ADIA ? 1 : batch_pencil_waves<W>()) void k_sweep_x_f32_batch(const DiskDev* __restrict__ dd,
                                                               const DevF32* __restrict__ dfs,
                                                               const int* __restrict__ order, int nslots, int src) {
-  __shared__ XTile<K, W, Q, 3> T;
+  __shared__ XTile<K, W, Q, ADIA ? 4 : 3> T;
   for (int t = blockIdx.x; t < nslots; t += gridDim.x) {
     const int d = order[2 * t];
     if (d < 0) continue;   // workgroup-uniform
     const DevGrid g = dd[d].g;
     const int ngrp = (g.jmax + W * Q - 1) / (W * Q);
     const int local = order[2 * t + 1];
-    pencil_item<K, W, Q, kGeoRcyl, 2, false, false, BARY>(g, dfs[d], dd[d].p, dd[d].ss, src, local % ngrp, local / ngrp,
-                                                         0, T);
+    pencil_item<K, W, Q, kGeoRcyl, 2, false, ADIA, BARY>(g, dfs[d], dd[d].p, dd[d].ss, src, local % ngrp, local / ngrp,
+                                                        0, T);
   }
 }
 
@@ -616,8 +616,8 @@ __global__ __launch_bounds__(64 * WPB) PN_WAVES((K == 2 && NDIM == 2 && !ADIA) ?
 
 // many disks, persistent (as k_sweep_x_f32_batch): item = one workgroup's WPB waves of one
 // disk; each item writes its CFL partial to its disk's cflp[item - base[d]]
-template <int K, int WPB, bool BARY>
-__global__ __launch_bounds__(64 * WPB) PN_WAVES(K == 2 ? 6 : 1) void k_sweep_y_f32_batch(const DiskDev* __restrict__ dd,
+template <int K, int WPB, bool BARY, bool ADIA>
+__global__ __launch_bounds__(64 * WPB) PN_WAVES((K == 2 && !ADIA) ? 6 : 1) void k_sweep_y_f32_batch(const DiskDev* __restrict__ dd,
                                                               const DevF32* __restrict__ dfs,
                                                               const int* __restrict__ base, int ndisk, int src) {
   __shared__ float shm[WPB];
@@ -633,7 +633,7 @@ __global__ __launch_bounds__(64 * WPB) PN_WAVES(K == 2 ? 6 : 1) void k_sweep_y_f
     const int l = gw % f.nwin_y;
     const int n = gw / f.nwin_y;
     float m = 0.0f;
-    if (n < g.imax) m = y_pencil<K, 2, 2, false, BARY>(g, f, dd[d].p, dd[d].ss, src, n, l);   // wave-uniform
+    if (n < g.imax) m = y_pencil<K, 2, 2, ADIA, BARY>(g, f, dd[d].p, dd[d].ss, src, n, l);   // wave-uniform
     __syncthreads();   // the previous item's partial has been read
     if ((threadIdx.x & 63) == 0) shm[wv] = m;
     __syncthreads();
@@ -1045,68 +1045,73 @@ int persistent_grid(Kern k, int block) {
 
 template <int K, int W, int Q>
 hipError_t sweep_x_batch_k(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* order, int nslots,
-                           int src, int grid, bool bary) {
+                           int src, int grid, bool bary, bool adia) {
   if (grid < 8 || grid % 8 || nslots % 8) return hipErrorInvalidValue;
-  if (bary) k_sweep_x_f32_batch<K, W, Q, true><<<grid, 64 * W, 0, s>>>(dd, dfs, order, nslots, src);
-  else k_sweep_x_f32_batch<K, W, Q, false><<<grid, 64 * W, 0, s>>>(dd, dfs, order, nslots, src);
+  if (bary && adia) k_sweep_x_f32_batch<K, W, Q, true, true><<<grid, 64 * W, 0, s>>>(dd, dfs, order, nslots, src);
+  else if (bary) k_sweep_x_f32_batch<K, W, Q, true, false><<<grid, 64 * W, 0, s>>>(dd, dfs, order, nslots, src);
+  else if (adia) k_sweep_x_f32_batch<K, W, Q, false, true><<<grid, 64 * W, 0, s>>>(dd, dfs, order, nslots, src);
+  else k_sweep_x_f32_batch<K, W, Q, false, false><<<grid, 64 * W, 0, s>>>(dd, dfs, order, nslots, src);
   return hipGetLastError();
 }
 
 template <int K, int WPB>
 hipError_t sweep_y_batch_k(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
-                           int src, int grid, bool bary) {
+                           int src, int grid, bool bary, bool adia) {
   if (grid < 8 || grid % 8) return hipErrorInvalidValue;
-  if (bary) k_sweep_y_f32_batch<K, WPB, true><<<grid, 64 * WPB, 0, s>>>(dd, dfs, base, ndisk, src);
-  else k_sweep_y_f32_batch<K, WPB, false><<<grid, 64 * WPB, 0, s>>>(dd, dfs, base, ndisk, src);
+  if (bary && adia) k_sweep_y_f32_batch<K, WPB, true, true><<<grid, 64 * WPB, 0, s>>>(dd, dfs, base, ndisk, src);
+  else if (bary) k_sweep_y_f32_batch<K, WPB, true, false><<<grid, 64 * WPB, 0, s>>>(dd, dfs, base, ndisk, src);
+  else if (adia) k_sweep_y_f32_batch<K, WPB, false, true><<<grid, 64 * WPB, 0, s>>>(dd, dfs, base, ndisk, src);
+  else k_sweep_y_f32_batch<K, WPB, false, false><<<grid, 64 * WPB, 0, s>>>(dd, dfs, base, ndisk, src);
   return hipGetLastError();
 }
 
 }  // namespace
 
 namespace {
-template <bool B>
+template <bool B, bool A>
 int batch_grid(int axis, int variant) {
   if (axis == 0) {
     switch (variant) {
-      case 0: return persistent_grid(k_sweep_x_f32_batch<2, 4, 2, B>, 256);
-      case 1: return persistent_grid(k_sweep_x_f32_batch<2, 8, 1, B>, 512);
-      case 2: return persistent_grid(k_sweep_x_f32_batch<2, 16, 1, B>, 1024);
-      case 3: return persistent_grid(k_sweep_x_f32_batch<2, 4, 4, B>, 256);
+      case 0: return persistent_grid(k_sweep_x_f32_batch<2, 4, 2, B, A>, 256);
+      case 1: return persistent_grid(k_sweep_x_f32_batch<2, 8, 1, B, A>, 512);
+      case 2: return persistent_grid(k_sweep_x_f32_batch<2, 16, 1, B, A>, 1024);
+      case 3: return persistent_grid(k_sweep_x_f32_batch<2, 4, 4, B, A>, 256);
     }
   } else {
     switch (variant) {
-      case 0: return persistent_grid(k_sweep_y_f32_batch<2, 2, B>, 128);
-      case 1: return persistent_grid(k_sweep_y_f32_batch<2, 4, B>, 256);
-      case 2: return persistent_grid(k_sweep_y_f32_batch<4, 2, B>, 128);
-      case 3: return persistent_grid(k_sweep_y_f32_batch<4, 4, B>, 256);
+      case 0: return persistent_grid(k_sweep_y_f32_batch<2, 2, B, A>, 128);
+      case 1: return persistent_grid(k_sweep_y_f32_batch<2, 4, B, A>, 256);
+      case 2: return persistent_grid(k_sweep_y_f32_batch<4, 2, B, A>, 128);
+      case 3: return persistent_grid(k_sweep_y_f32_batch<4, 4, B, A>, 256);
     }
   }
   return 0;
 }
 }  // namespace
 
-int f32_batch_grid(int axis, int variant, bool bary) {
-  return bary ? batch_grid<true>(axis, variant) : batch_grid<false>(axis, variant);
+int f32_batch_grid(int axis, int variant, bool bary, bool adia) {
+  if (bary) return adia ? batch_grid<true, true>(axis, variant) : batch_grid<true, false>(axis, variant);
+  return adia ? batch_grid<false, true>(axis, variant) : batch_grid<false, false>(axis, variant);
 }
 
 hipError_t f32_sweep_x_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* order, int nslots,
-                             int variant, int src, int grid, bool bary) {
+                             int variant, int src, int grid, bool bary, bool adia) {
   switch (variant) {
-    case 0: return sweep_x_batch_k<2, 4, 2>(s, dd, dfs, order, nslots, src, grid, bary);
-    case 1: return sweep_x_batch_k<2, 8, 1>(s, dd, dfs, order, nslots, src, grid, bary);
-    case 2: return sweep_x_batch_k<2, 16, 1>(s, dd, dfs, order, nslots, src, grid, bary);
-    case 3: return sweep_x_batch_k<2, 4, 4>(s, dd, dfs, order, nslots, src, grid, bary);
+    case 0: return sweep_x_batch_k<2, 4, 2>(s, dd, dfs, order, nslots, src, grid, bary, adia);
+    case 1: return sweep_x_batch_k<2, 8, 1>(s, dd, dfs, order, nslots, src, grid, bary, adia);
+    case 2: return sweep_x_batch_k<2, 16, 1>(s, dd, dfs, order, nslots, src, grid, bary, adia);
+    case 3: return sweep_x_batch_k<2, 4, 4>(s, dd, dfs, order, nslots, src, grid, bary, adia);
   }
   return hipErrorInvalidValue;
 }
 
 hipError_t f32_sweep_y_batch(hipStream_t s, const DiskDev* dd, const DevF32* dfs, const int* base, int ndisk,
-                             int variant, int src, int grid, bool bary) {
+                             int variant, int src, int grid, bool bary, bool adia) {
   switch (variant) {
-    case 0: return sweep_y_batch_k<2, 2>(s, dd, dfs, base, ndisk, src, grid, bary);
-    case 1: return sweep_y_batch_k<2, 4>(s, dd, dfs, base, ndisk, src, grid, bary);
-    case 2: return sweep_y_batch_k<4, 2>(s, dd, dfs, base, ndisk, src, grid, bary);
-    case 3: return sweep_y_batch_k<4, 4>(s, dd, dfs, base, ndisk, src, grid, bary);
+    case 0: return sweep_y_batch_k<2, 2>(s, dd, dfs, base, ndisk, src, grid, bary, adia);
+    case 1: return sweep_y_batch_k<2, 4>(s, dd, dfs, base, ndisk, src, grid, bary, adia);
+    case 2: return sweep_y_batch_k<4, 2>(s, dd, dfs, base, ndisk, src, grid, bary, adia);
+    case 3: return sweep_y_batch_k<4, 4>(s, dd, dfs, base, ndisk, src, grid, bary, adia);
   }
   return hipErrorInvalidValue;
 }

@@ -309,6 +309,31 @@ def test_batch_in_the_barycentric_frame_equals_single_disks():
             assert li <= 1e-12, (w.name, f, l1, li)
 
 
+def test_batch_of_adiabatic_disks_equals_single_disks():
+    """Disks with the energy-consistent adiabatic step (EOS 3) run the batch kernels that carry
+    q = P / Sigma as a fourth field; the disks of such a batch -- viscous or not, fixed or moving
+    planets, radial boundary codes -- each equal their own single-context FP32 run, P included."""
+    ws = []
+    for d, (nr, q, alpha, extra) in enumerate([(48, 1e-3, 0.0, {}), (64, 3e-4, 3e-3, dict(plnt_flag=2)),
+                                               (96, 1e-4, 1e-3, dict(nlft=1, nrgh=0, init_p_iso=0))]):
+        over = dict(visc_flag=1, ss_alpha=alpha, vis_nu=alpha * 0.05 * 0.05) if alpha > 0 else {}
+        w = configs.small(nr, 3 * nr, **{"M_p": q, "eos": 3, "gam": 1.4, "init_p_iso": 1, **over, **extra})
+        w.seed = 700 + d
+        ws.append(w)
+    n = 30
+    say("case: batch of EOS 3 disks,", len(ws), "disks,", n, "steps")
+    outs, sts = run_batch(ws, n)
+    for w, o, s in zip(ws, outs, sts):
+        ref, rs = run_single(w, abi.PN_FP32, n)
+        say(f"  {w.name} visc {w.params.visc_flag} plnt_flag {w.params.plnt_flag}: dt {s.dt!r} vs {rs.dt!r}")
+        assert s.dt == rs.dt and s.simtime == rs.simtime and s.FrRot == rs.FrRot
+        assert s.planet.x == rs.planet.x and s.planet.y == rs.planet.y
+        assert np.array_equal(o["rot_j"], ref["rot_j"])
+        for f in ("r", "p", "u", "v"):
+            l1, li = rel(o[f], ref[f])
+            assert li <= 1e-12, (w.name, f, l1, li)
+
+
 def test_batch_landing_matches_single_disks():
     """Picture times and end times differ per disk: the batch stops after the first step in
     which any disk lands, and every disk -- landed or not -- equals its own single-context

@@ -172,7 +172,11 @@ def test_fp32_parameter_errors_before_any_device_call():
             abi.Context(PnGrid(32, 48, 1), configs.disk2d(32, 48, gam=1.4, **over), abi.PN_FP32)
         assert e.value.code == abi.PN_ERR_UNSUPPORTED
     with pytest.raises(abi.PnError) as e:
-        abi.Batch([PnGrid(32, 48, 1)], [configs.disk2d(32, 48, eos=3, gam=1.4)])
+        abi.Batch([PnGrid(32, 48, 1)], [configs.disk2d(32, 48, eos=1, gam=1.4)])
+    assert e.value.code == abi.PN_ERR_UNSUPPORTED
+    # the disks of a batch share the closure (EOS 3 runs kernels of its own)
+    with pytest.raises(abi.PnError) as e:
+        abi.Batch([PnGrid(32, 48, 1)] * 2, [configs.disk2d(32, 48), configs.disk2d(32, 48, eos=3, gam=1.4)])
     assert e.value.code == abi.PN_ERR_UNSUPPORTED
     # parameter values outside their ranges are argument errors on either path
     for over in (dict(bary_flag=2), dict(nlft=5), dict(nrgh=-1)):
