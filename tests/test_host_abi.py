@@ -295,3 +295,19 @@ def test_run_loop_driver_parses_its_arguments_without_a_device():
                 ["--out", "x", "--nprin", "0"], ["--bogus"]):
         res = subprocess.run([str(exe), *bad], capture_output=True, text=True, timeout=60)
         assert res.returncode == 2, (bad, res.stdout, res.stderr)
+
+
+def test_run_loop_driver_sets_every_parameter(tmp_path):
+    """Every pn_params field can be set from penguin_main's command line: each --set parses
+    (a parse error exits 2 before any device call; without a device the run then stops at the
+    context with exit 1)."""
+    import subprocess
+    from pathlib import Path
+    from penguin_2_0_amd.abi import PnParams
+    exe = Path(__file__).resolve().parent.parent / "penguin_2_0_amd" / "penguin_main"
+    p = configs.disk2d(16, 48, endtime=0.2, tmovie=0.1)   # short, should a device be present
+    args = [str(exe), "--out", str(tmp_path / "run"), "--imax", "16", "--jmax", "48"]
+    for name, _ in PnParams._fields_:
+        args += ["--set", f"{name}={getattr(p, name)!r}"]
+    res = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert res.returncode != 2, res.stderr
