@@ -1,8 +1,8 @@
 """ctypes binding of the FP64 CPU oracle (libpn_oracle.so).
 
 TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
-import this module, and only as the checker or the timed CPU baseline.  The product library
-(penguin_2_0_amd/libpenguin.so) never loads it.
+import this module, and only as what results are compared with, or as the timed CPU
+baseline.  The product library (penguin_2_0_amd/libpenguin.so) never loads it.
 """
 from __future__ import annotations
 

@@ -1,7 +1,7 @@
 /*
  * pn_oracle.c -- FP64 CPU restatement of PEnGUIn's per-step hydro update.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke()
+ * TEST INFRASTRUCTURE ONLY.  This file is what results are compared with: tests/, the smoke()
  * entry and bench.py's cpu_baseline leg may load it; the product library
  * never links it and never falls back to it.
  *

@@ -2,8 +2,8 @@
  * pn_oracle.h -- FP64 CPU restatement of PEnGUIn's hydro step.
  *
  * TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg may load this library, and only as the checker (or the
- * timed CPU baseline) -- never as the product path.
+ * cpu_baseline leg may load this library, and only as what results are compared
+ * with (or the timed CPU baseline) -- never as the product path.
  *
  * The state is kept in the reference's own hydr_ring byte layout
  * (variable_types.h:67-97, runtime jmax) so that the oracle reads like the
