@@ -125,6 +125,34 @@ def test_strongly_forced_c4_disks_within_1e5_after_1000_steps(idx):
     assert l1 <= 1e-5
 
 
+@pytest.mark.parametrize("idx", [3, 12, 23, 29])
+def test_c4_disks_past_1e5_are_flattening_switches(idx):
+    """The disks of the 48-disk sweep that end more than 1e-5 from FP64 after 1000 steps
+    (1.8e-5, 1.1e-4, 1.8e-4 and 7.6e-5) do so through PPM's flattening, a switch on a
+    pressure-jump threshold that a shock in the planet's wake meets: FP64 runs whose start is
+    nudged by 2^-25 land as far from their own run.  With flattening off the same disks stay
+    within 1e-5 (4.0e-6, 1.3e-7, 3.7e-6 and 2.8e-7), and a nudge of 2^-40 moves FP64 by
+    1e-11 to 1e-10: no slow growth of the FP32 rounding hides behind the switch."""
+    w = configs.c4_members()[idx]
+    w.params.flat_flag = 0
+    n = 1000
+    say("case:", w.name, f"q {w.params.M_p:.2e}, flattening off, FP32 vs FP64,", n, "steps")
+    d = abi.init_disk(w.grid, w.params, w.seed, w.noise)
+    a, _ = run(w, d, abi.PN_FP32, n)
+    b, _ = run(w, d, abi.PN_FP64, n)
+    l1, li = rel(lab(a, "r"), lab(b, "r"))
+    say(f"  rho: L1 {l1:.3e} Linf {li:.3e}")
+    assert l1 <= 1e-5
+    rng = np.random.default_rng(4)
+    e = d.copy()
+    for f in ("r", "p", "u", "v"):
+        e.rings[f] = d.rings[f] * (1.0 + 2.0**-40 * rng.uniform(-1.0, 1.0, size=d.rings[f].shape))
+    c, _ = run(w, e, abi.PN_FP64, n)
+    l1n = rel(lab(c, "r"), lab(b, "r"))[0]
+    say(f"  FP64 nudged by 2^-40: L1 {l1n:.3e}")
+    assert l1n <= 1e-9
+
+
 def test_fused_cfl_folds_more_than_4096_partials():
     """On the 1024 x 3072 disk the 128-cell azimuthal windows make 6912 workgroups, each
     writing one CFL partial, which the step folds in 256 chunks before the post-step (the
