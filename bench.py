@@ -16,7 +16,9 @@ as one FP32-bulk batch -- one launch per sweep for all disks.  Beside it, on the
   accuracy_vs_fp64  a fresh batch and every disk's FP64 run, 1000 steps each (whatever K is),
                     relative L1 / Linf of Sigma in the lab frame per disk, each set beside the
                     disk's sensitivity floor (the FP64 run started from the initial state
-                    rounded to FP32: how far the flow itself carries a rounding of its start);
+                    rounded to FP32: how far the flow itself carries a rounding of its start)
+                    and the widest of three FP64 runs whose start is nudged by 2^-25 (every
+                    value times 1 + 2^-25 u, u uniform in [-1, 1], seeds 1..3);
   single_disk       the 1024 x 3072 disk on the FP32-bulk and the FP64 path (K timed steps
                     each), with their errors after 1000 steps;
   roofline          the batch's dominant sweep kernel: algorithmic bytes per launch (one read
@@ -78,6 +80,15 @@ def fp32_rounded(d: abi.Disk) -> abi.Disk:
     d = d.copy()
     for f in ("r", "p", "u", "v"):
         d.rings[f] = d.rings[f].astype(np.float32).astype(np.float64)
+    return d
+
+
+def nudged(d: abi.Disk, seed: int, rel: float = 2.0**-25) -> abi.Disk:
+    """The disk with every initial value multiplied by 1 + rel u, u uniform in [-1, 1]."""
+    d = d.copy()
+    rng = np.random.default_rng(seed)
+    for f in ("r", "p", "u", "v"):
+        d.rings[f] = d.rings[f] * (1.0 + rel * rng.uniform(-1.0, 1.0, size=d.rings[f].shape))
     return d
 
 
@@ -194,7 +205,7 @@ def batch_leg(ws: list[configs.Workload], disks: list[abi.Disk], steps: int, war
 
 def batch_accuracy(ws: list[configs.Workload], disks: list[abi.Disk], steps: int, graph: bool,
                    report: str | None) -> dict:
-    """A fresh batch and every disk's FP64 run (and its floor) for `steps` steps."""
+    """A fresh batch and every disk's FP64 run (its floor, three nudged starts) for `steps` steps."""
     with abi.Batch([w.grid for w in ws], [w.params for w in ws]) as b:
         for ctx, d in zip(b.disks, disks):
             ctx.load(d)
@@ -205,12 +216,15 @@ def batch_accuracy(ws: list[configs.Workload], disks: list[abi.Disk], steps: int
     for w, d, o in zip(ws, disks, out32):
         ref = run_ctx(w, d, abi.PN_FP64, steps, graph=graph)["state"]
         flo = run_ctx(w, fp32_rounded(d), abi.PN_FP64, steps, graph=graph)["state"]
+        nud = max(rel_err(lab(run_ctx(w, nudged(d, s), abi.PN_FP64, steps, graph=graph)["state"], "r"),
+                          lab(ref, "r"))[0] for s in (1, 2, 3))
         l1, li = rel_err(lab(o, "r"), lab(ref, "r"))
         rows.append({"disk": w.name, "q": w.params.M_p, "alpha": w.params.ss_alpha, "l1_rho": l1, "linf_rho": li,
                      "l1_vphi": rel_err(lab(o, "v"), lab(ref, "v"))[0],
-                     "floor_l1_rho": rel_err(lab(flo, "r"), lab(ref, "r"))[0]})
+                     "floor_l1_rho": rel_err(lab(flo, "r"), lab(ref, "r"))[0], "nudge_l1_rho": nud})
     l1s = np.array([r["l1_rho"] for r in rows])
     floors = np.array([r["floor_l1_rho"] for r in rows])
+    nudges = np.array([r["nudge_l1_rho"] for r in rows])
     ok = l1s <= np.maximum(1e-5, floors)
     worst = max(rows, key=lambda r: r["l1_rho"] / max(1e-5, r["floor_l1_rho"]))
     if report:
@@ -221,10 +235,12 @@ def batch_accuracy(ws: list[configs.Workload], disks: list[abi.Disk], steps: int
             "median_floor_l1_rho": float(np.median(floors)),
             "disks_floor_le_1e-5": int((floors <= 1e-5).sum()),
             "disks_l1_rho_le_max_1e-5_floor": int(ok.sum()),
+            "disks_l1_rho_le_max_1e-5_floor_nudge": int((l1s <= np.maximum(1e-5, np.maximum(floors, nudges))).sum()),
             "median_l1_over_floor": float(np.median(l1s / np.maximum(floors, 1e-300))),
             "worst_over_bar": worst,
             "per_disk_l1_rho": [float(f"{x:.3g}") for x in l1s],
-            "per_disk_floor_l1_rho": [float(f"{x:.3g}") for x in floors]}
+            "per_disk_floor_l1_rho": [float(f"{x:.3g}") for x in floors],
+            "per_disk_nudge_l1_rho": [float(f"{x:.3g}") for x in nudges]}
 
 
 def single_disk_accuracy(w: configs.Workload, steps: int, graph: bool, floor: bool) -> dict:
