@@ -3,6 +3,8 @@
 - the 256 x 768 disk (100 steps): the FP64 path and the FP32-bulk path against the oracle;
 - the 1024 x 3072 disk (1000 steps): the FP32-bulk path against the FP64 path, and the fold of
   more than 4096 fused-CFL partials that only disks of this size reach;
+- members of the 48-disk sweep (1000 steps): the strongly forced ones within 1e-5 of FP64, and
+  the ones past 1e-5 within it once flattening, the switch they cross, is off;
 - Index_Shift on rings whose whole-cell shift sits exactly on a half integer of either sign
   (orbital_advection.cu:35-39), against the oracle and the closed form;
 - the azimuthal ring means (wavefront-shuffle reduction) against the oracle.
